@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X coupled-wave solve and field evaluation.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config volume32|plain_small] [--eig-mode 0|1]
+
+One step = one scattervolume solve of the configuration's sample (already resident in HBM) through
+libcwx: Fourier coefficients, Phi, per-layer eigendecomposition, transfer chain, boundary LU, beta,
+Q/Ez rows and the .cw records on the device.  `value` is solves/s on one GPU.  The field evaluation
+of the solution on the configuration's grid is timed separately (field_gpts_per_s).  The roofline
+entry covers the eigensolver stage, timed with HIP events on the stream it runs on.  cpu_baseline
+times the oracle (numpy/MKL FP64 restatement of the reference algorithm) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def eig_flops(n: int) -> float:
+    """Nominal zgeev work: 80 n^3 real flops (a complex multiply-add counts 8)."""
+    return 80.0 * n ** 3
+
+
+def solve_flops(MF: int, L: int, eig_mode: int) -> float:
+    """Nominal FP64 flops of one solve in the conventions zgeev = 80n^3, getrf = 8/3 n^3,
+    getrs(nrhs) = 8 n^2 nrhs, zgemm(m,n,k) = 8mnk (the reference's own per-layer operations)."""
+    n4, n2, n6 = 4 * MF, 2 * MF, 6 * MF
+    n_eig = n4 if eig_mode == 0 else n2
+    f = L * eig_flops(n_eig)
+    f += L * 8 / 3 * n4 ** 3                      # LU of every Gd
+    f += (L - 1) * 8 * n4 ** 3                    # T_i = Gd_i^-1 Gc_{i-1}
+    f += max(L - 1, 0) * 8 * n4 ** 3              # transfer product
+    f += 8 / 3 * n4 ** 3 + 8 * n4 ** 2 * 3 * MF + 8 * n4 * n4 * 3 * MF   # boundary block
+    f += 8 / 3 * n6 ** 3
+    return f
+
+
+def cpu_baseline(sample, opts, layers: int):
+    """Oracle (FP64 numpy + MKL zgeev/zgesv through torch) on the first `layers` layers."""
+    import torch
+    from oracle import cw_oracle as o
+    o.set_lapack("torch")
+    threads = min(16, os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    p = o.VolumeParams(lam=opts["lam"], direction=opts["direction"], ex=opts["ex"], ey=opts["ey"],
+                       n=opts["n"], size=opts["size"], coef=opts["coef"])
+    sub = sample[:layers]
+    t0 = time.perf_counter()
+    sol = o.solve(sub, p)
+    o.build_cw(sol)
+    t = time.perf_counter() - t0
+    L = sample.shape[0]
+    return t * L / layers, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="volume32")
+    ap.add_argument("--eig-mode", type=int, default=0)
+    ap.add_argument("--cpu-layers", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-field", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (instances and layers "
+              "shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import scattersim_amd as sa
+    from scattersim_amd import workloads
+
+    sample, opts, grid = workloads.CONFIGS[args.config]()
+    L, s1, s2 = sample.shape
+    dev = torch.device("cuda:0")
+    d_sample = torch.from_numpy(sample).to(dev)
+    stream = torch.cuda.Stream()          # default priority; the library queues all work on it
+    ctx = sa.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    torch.cuda.synchronize()
+
+    def one():
+        sol = sa.scattervolume(ctx, d_sample.data_ptr(), shape=(L, s1, s2), eig_mode=args.eig_mode, **opts)
+        return sol
+
+    for _ in range(args.warmup):
+        one().free()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol = one()
+        if _ != args.steps - 1:
+            sol.free()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) / args.steps * 1e3
+    MF = sol.MF
+
+    # ---- eigensolver stage: HIP events on the stream it runs on ----
+    lib = sa.lib()
+    p = sa._params(eig_mode=args.eig_mode, **opts)
+    import ctypes as C
+    nmat = C.c_int()
+    blocks = 1 if args.eig_mode else 0
+    sa.check(lib.cwx_phi_build_dev(ctx.h, C.byref(p), None, L, s1, s2, blocks, None, C.byref(nmat)))
+    n = nmat.value
+    per = 2 if blocks else 1
+    phi = torch.zeros((per * L, n, n), dtype=torch.complex128, device=dev)
+    sa.check(lib.cwx_phi_build_dev(ctx.h, C.byref(p), C.c_void_p(d_sample.data_ptr()), L, s1, s2, blocks,
+                                   C.c_void_p(phi.data_ptr()), C.byref(nmat)))
+    if blocks:
+        Pm = phi[:L].transpose(1, 2).contiguous()
+        Qm = phi[L:].transpose(1, 2).contiguous()
+        A0 = torch.matmul(Pm, Qm).transpose(1, 2).contiguous()
+    else:
+        A0 = phi
+    W = torch.zeros((L, n), dtype=torch.complex128, device=dev)
+    V = torch.zeros((L, n, n), dtype=torch.complex128, device=dev)
+    reps = 2
+    times = []
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        A = A0.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        sa.check(lib.cwx_zgeev_batched_dev(ctx.h, n, L, A.data_ptr(), n * n, W.data_ptr(), n, V.data_ptr(), n * n,
+                                           None))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    t_eig = min(times)
+    fl = L * eig_flops(n)
+    achieved = fl / t_eig / 1e12
+
+    # ---- field evaluation of the solution on the configuration grid ----
+    field = None
+    if not args.no_field:
+        ctx.field_init_solution(sol)
+        g = grid
+        out = torch.zeros((g["nz"], g["ny"], g["nx"], 3), dtype=torch.complex64, device=dev)
+        ctx.field_grid_dev(out.data_ptr(), False, 0, g["x0"], g["y0"], g["z0"], g["dx"], g["dy"], g["dz"],
+                           g["nx"], g["ny"], g["nz"])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.field_grid_dev(out.data_ptr(), False, 0, g["x0"], g["y0"], g["z0"], g["dx"], g["dy"], g["dz"],
+                           g["nx"], g["ny"], g["nz"])
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tf = e0.elapsed_time(e1) * 1e-3
+        npts = g["nx"] * g["ny"] * g["nz"]
+        field = {"gpts_per_s": npts / tf / 1e9, "points": npts, "ms": tf * 1e3, "form": "direct FP64-reduced"}
+    sol.free()
+
+    cpu = None
+    if not args.no_cpu:
+        t_cpu, threads = cpu_baseline(sample, opts, min(args.cpu_layers, L))
+        cpu = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"oracle solve of the first {min(args.cpu_layers, L)} of {L} layers, scaled by "
+                         f"{L}/{min(args.cpu_layers, L)}"}
+
+    line = {
+        "metric": "sample solves/sec + field Gpts/s at <=1e-6 rel err vs FP64 (plain vs batched)",
+        "value": 1e3 / ms,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (f64)",
+        "data": "synthetic (seeded)",
+        "config": {"workload": args.config, "layers": L, "grid": [s1, s2], "orders": [sol.M0, sol.M1],
+                   "eig_n": n, "eig_mode": "full 4MF Phi" if args.eig_mode == 0 else "2MF PQ",
+                   "parallelism": "1 GPU"},
+        "solve_tflops_nominal": solve_flops(MF, L, args.eig_mode) / (ms * 1e-3) / 1e12,
+        "field": field,
+        "roofline": {"kernel": "eigensolver stage (gehrd+unghr+hqr+trevc+gemm+normalize)", "bound": "mfma",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "ms_per_launch": t_eig * 1e3, "flops_per_launch": fl},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

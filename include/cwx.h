@@ -1,0 +1,195 @@
+/*
+ * cwx — MI355X-native coupled-wave solver and plane-wave field evaluator (C ABI).
+ *
+ * Every entry point returns an int status: CWX_OK (0), a negative error class, or a positive
+ * LAPACK-style numerical failure index (non-convergence, zero pivot).  The message of the last
+ * failure on the calling thread is available from cwx_last_error().  Complex numbers are
+ * interleaved {re, im} pairs, binary compatible with std::complex<double>/<float>; matrices are
+ * column-major with explicit leading dimensions, as LAPACK.  A context owns one HIP device and one
+ * stream; contexts are independent, a single context must not be used by two threads at once.
+ *
+ * Reference interfaces replaced (paths relative to the scattersim checkout):
+ *   third_Lapack.h:38-85       MKL_eigensolve / MKL_linearsolve / MKL_inverse / MKL_multiply
+ *   FourierWave.h:303-397      volume::CalculateD / phi (the per-layer property matrix)
+ *   ScatterVolume.cpp:506-1019 the scattervolume solve (argv + .npy -> CoupledWaveStructure)
+ *   CoupledWaveStructure.h:63-125  CoupledWaveStructure::save / load
+ *   cpuEvaluator.h:6-24        UnpackedWave, cw_allocate, cw_unpack
+ *   gpuEvaluator.h:14-17       gpu_initialize, gpu_cw_evaluate
+ */
+#ifndef CWX_H
+#define CWX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cwx_context cwx_context;
+typedef struct cwx_solution cwx_solution;
+
+typedef struct { double re, im; } cwx_complex;
+typedef struct { float re, im; } cwx_complexf;
+
+/* UnpackedWave<float> (cpuEvaluator.h:6-10): 48 bytes */
+typedef struct { cwx_complexf E0[3]; cwx_complexf k[3]; } cwx_wave_f32;
+/* .cw record of this library: {E0.x, E0.y, E0.z, k.x, k.y, k.z}, 96 bytes */
+typedef struct { cwx_complex E0[3]; cwx_complex k[3]; } cwx_wave_f64;
+
+enum {
+  CWX_OK = 0,
+  CWX_ERR_ARG = -1,    /* invalid argument */
+  CWX_ERR_DEVICE = -2, /* HIP runtime failure or no device */
+  CWX_ERR_IO = -3,     /* file could not be read or written */
+  CWX_ERR_STATE = -4   /* call out of order (e.g. evaluate before init) */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cwx_last_error(void);
+/* Library version string. */
+const char* cwx_version(void);
+
+/* ---- context ------------------------------------------------------------------------------ */
+/* Creates a context on HIP device `device` with its own stream. */
+int cwx_create(int device, cwx_context** ctx);
+int cwx_destroy(cwx_context* ctx);
+/* Use a caller-owned hipStream_t (passed as void*) for all later work; NULL restores the own one. */
+int cwx_set_stream(cwx_context* ctx, void* hip_stream);
+int cwx_synchronize(cwx_context* ctx);
+
+/* ---- dense complex FP64 linear algebra: the LAPACK seam (third_Lapack.h:38-85) --------------- */
+/* Replaces MKL_eigensolve (third_Lapack.h:38-45, LAPACKE_zgeev 'N','V').  A (n x n, host) is
+ * overwritten by its Schur form; w (n) eigenvalues; VR (n x n) right eigenvectors of unit 2-norm
+ * with their largest component real.  Returns >0 when the QR iteration does not converge. */
+int cwx_zgeev(cwx_context* ctx, int n, cwx_complex* A, int lda, cwx_complex* w, cwx_complex* VR,
+              int ldvr);
+/* Replaces MKL_linearsolve (third_Lapack.h:47-54, LAPACKE_zgesv): A is overwritten by its LU
+ * factors, B (n x nrhs) by the solution, ipiv (n, may be NULL) receives 1-based pivots. */
+int cwx_zgesv(cwx_context* ctx, int n, int nrhs, cwx_complex* A, int lda, int* ipiv, cwx_complex* B,
+              int ldb);
+/* Replaces MKL_inverse (third_Lapack.h:56-64, zgesv with an identity right-hand side). */
+int cwx_zinverse(cwx_context* ctx, int n, const cwx_complex* A, int lda, cwx_complex* Ainv, int ldinv);
+/* Replaces MKL_multiply (third_Lapack.h:75-85, cblas_zgemm): C = alpha A B + beta C. */
+int cwx_zgemm(cwx_context* ctx, int m, int n, int k, cwx_complex alpha, const cwx_complex* A, int lda,
+              const cwx_complex* B, int ldb, cwx_complex beta, cwx_complex* C, int ldc);
+
+/* Device-pointer, batched forms of the same operations (no reference counterpart: the reference
+ * calls them one matrix at a time from ScatterVolume.cpp:262, 331-333, 739, 768-781).  Matrices
+ * are packed with leading dimension n at element strides; work is queued on the context stream.
+ * h_info (host, batch entries, may be NULL) receives per-matrix status after a synchronisation. */
+int cwx_zgeev_batched_dev(cwx_context* ctx, int n, int batch, void* dA, int64_t strideA, void* dW,
+                          int64_t strideW, void* dV, int64_t strideV, int* h_info);
+int cwx_zgetrf_batched_dev(cwx_context* ctx, int n, int batch, void* dA, int64_t strideA, void* dIpiv,
+                           int64_t strideP, int* h_info);
+int cwx_zgetrs_batched_dev(cwx_context* ctx, int n, int nrhs, int batch, const void* dLU,
+                           int64_t strideA, const void* dIpiv, int64_t strideP, void* dB, int64_t strideB);
+int cwx_zgemm_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n, int k,
+                          cwx_complex alpha, const void* dA, int lda, int64_t strideA, const void* dB,
+                          int ldb, int64_t strideB, cwx_complex beta, void* dC, int ldc, int64_t strideC,
+                          int batch);
+
+/* ---- the scattervolume solve (ScatterVolume.cpp:506-1019) ------------------------------------ */
+/* Options of ScatterVolume.cpp:510-530; cwx_volume_params_default fills the reference defaults. */
+typedef struct {
+  double lambda;       /* --lambda (1.0) */
+  double direction[3]; /* --direction (0 0 1) */
+  double ex[2];        /* --ex (0 0) as (re, im) */
+  double ey[2];        /* --ey (1 0) */
+  double ez[2];        /* --ez (0 0) */
+  double n[2];         /* --n (1 1): upper and lower half-space indices */
+  double kappa;        /* --kappa (0): absorbance of the lower half-space */
+  double size[3];      /* --size (10 10 2) */
+  double z;            /* --z (0) */
+  int ncoef;           /* number of --coef values: 0, 1 or 2 */
+  int coef[2];
+  int psf;             /* --psf */
+  int external;        /* --external */
+  int npz;             /* --npz */
+  int no_cutoff;       /* library only: bypass the silent order cut-off of ScatterVolume.cpp:613-642 */
+  int eig_mode;        /* 0: zgeev of the 4MF x 4MF Phi (reference); 1: 2MF eigenproblem of P Q */
+} cwx_volume_params;
+void cwx_volume_params_default(cwx_volume_params* p);
+
+/* Solve one sample: (L, s1, s2) complex128 refractive indices in C order, on the host
+ * (cwx_solve_volume) or already on the device (cwx_solve_volume_dev).  A 2-D sample is L = 1.
+ * Returns >0 if an eigensolve did not converge or a factorisation met a zero pivot (the solution
+ * object is still returned for inspection). */
+int cwx_solve_volume(cwx_context* ctx, const cwx_volume_params* p, const cwx_complex* sample, int L,
+                     int s1, int s2, cwx_solution** out);
+int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L,
+                         int s1, int s2, cwx_solution** out);
+
+/* Replaces volume::CalculateD / phi (FourierWave.h:303-397): the property matrix of every layer,
+ * Phi = [[0, P], [Q, 0]] (4MF x 4MF, column-major, blocks = 0, L matrices) or its blocks P and Q
+ * (2MF x 2MF each, blocks = 1: the L P blocks, then the L Q blocks) into device memory d_out.
+ * *n_out receives the matrix order; with d_out == NULL only the order is computed. */
+int cwx_phi_build_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1,
+                      int s2, int blocks, void* d_out, int* n_out);
+
+typedef struct {
+  int M0, M1, MF, L;   /* Fourier orders along x, y; MF = M0 M1; layers */
+  int cutoff;          /* the order cut-off heuristic fired */
+  int info;            /* numerical status of the solve */
+  double k;            /* 2 PI / lambda with PI = 3.141592653 (FourierWave.h:18) */
+  int64_t nrec;        /* plane-wave records of the .cw */
+  int nlists;          /* 1 + 2 (L + 1): Pi, then Pr and Pt of every boundary */
+} cwx_solution_info;
+int cwx_solution_get_info(const cwx_solution* sol, cwx_solution_info* info);
+
+enum {
+  CWX_SOL_X = 0,   /* 6MF complex: [Rx Ry Rz Tx Ty Tz] (ScatterVolume.cpp:126-131) */
+  CWX_SOL_GAMMA,   /* L x 4MF complex: sorted eigenvalues, even = down, odd = up */
+  CWX_SOL_BETA,    /* L x 4MF complex: mode amplitudes (ScatterVolume.cpp:766-783) */
+  CWX_SOL_QCH,     /* L x (3MF x 4MF) complex, column-major: [Q_check | Q_hat] */
+  CWX_SOL_Z,       /* L + 1 double: boundary positions */
+  CWX_SOL_RECORDS, /* nrec x 6 complex: .cw records in file order */
+  CWX_SOL_SX, CWX_SOL_SY, CWX_SOL_SZ0, CWX_SOL_SZ1, /* MF complex each */
+  CWX_SOL_EF,      /* 3MF complex */
+  CWX_SOL_GD,      /* L x 4MF x 4MF complex */
+  CWX_SOL_GC,
+  CWX_SOL_NIF      /* L x MF complex: Fourier coefficients of n^-2, (M1 x M0) column-major */
+};
+/* Number of elements (complex, or double for CWX_SOL_Z) of a solution array. */
+int64_t cwx_solution_size(const cwx_solution* sol, int what);
+int cwx_solution_copy(cwx_context* ctx, const cwx_solution* sol, int what, void* host_dst);
+/* Device address of a solution array (NULL for host-only arrays). */
+const void* cwx_solution_device_ptr(const cwx_solution* sol, int what);
+/* Offsets and counts (in records) of the Pi / Pr_l / Pt_l lists, nlists entries each. */
+int cwx_solution_lists(const cwx_solution* sol, int64_t* offsets, int64_t* counts);
+/* CoupledWaveStructure::save (CoupledWaveStructure.h:63-92) with 96-byte records. */
+int cwx_solution_write_cw(cwx_context* ctx, const cwx_solution* sol, const char* path);
+int cwx_solution_free(cwx_solution* sol);
+
+/* ---- .cw files (CoupledWaveStructure.h:94-125 + cpuEvaluator.cpp:34-102) --------------------- */
+/* Reads a .cw file (this library's 96-byte records, or the reference's legacy 80-byte records
+ * {E0.x, E0.y, k.x, k.y, k.z}, E0.z taken as 0) and unpacks it: call once with W == NULL to get the
+ * sizes, then with buffers of nwaves records, `layers` z values and layers + 1 region bounds. */
+int cwx_cw_read_unpacked(const char* path, int64_t* nwaves, int* layers, cwx_wave_f64* W,
+                         double* z_layers, int* waves_begin, int* waves_end);
+
+/* ---- field evaluation (gpuEvaluator.h:14-17) ------------------------------------------------- */
+/* gpu_initialize (gpuEvaluator.cu:15-26): upload an unpacked FP32 wave table and region map. */
+int cwx_field_init_f32(cwx_context* ctx, const cwx_wave_f32* W, int64_t nwaves, const float* z_layers,
+                       int layers, const int* waves_begin, const int* waves_end);
+/* The same with an FP64 table; evaluation then reduces phases in FP64 (~1e-7 relative error). */
+int cwx_field_init_f64(cwx_context* ctx, const cwx_wave_f64* W, int64_t nwaves, const float* z_layers,
+                       int layers, const int* waves_begin, const int* waves_end);
+/* Use the device-resident records of a solve as the FP64 table (no copy; the solution must
+ * outlive the evaluations). */
+int cwx_field_init_solution(cwx_context* ctx, const cwx_solution* sol);
+/* gpu_cw_evaluate (gpuEvaluator.cu:131-165): three N x N slices at z (xy), y (xz) and x (yz) into
+ * caller-owned host buffers of N*N*3 complex<float>, rows along the second axis. */
+int cwx_field_evaluate(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz,
+                       float x_start, float y_start, float z_start, float x, float y, float z, float d,
+                       int N);
+/* Evaluate a regular grid into device memory: out[iz][iy][ix][3] complex<float> (out_f64 = 0) or
+ * complex<double> (out_f64 = 1, FP64 tables only) at x0 + dx*ix etc.  tol_mode 0 uses the GPU
+ * region test z >= z_l - 0.001 (gpuEvaluator.cu:34), 1 the CPU one z >= z_l (cpuEvaluator.cpp:303). */
+int cwx_field_grid_dev(cwx_context* ctx, void* d_out, int out_f64, int tol_mode, float x0, float y0,
+                       float z0, float dx, float dy, float dz, int nx, int ny, int nz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CWX_H */

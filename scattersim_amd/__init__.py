@@ -1,0 +1,275 @@
+"""scattersim_amd: MI355X-native coupled-wave solver (libcwx) with a thin Python host layer.
+
+The compute path is the HIP library ``lib/libcwx.so`` (C ABI in ``include/cwx.h``); this module only
+marshals numpy arrays and device pointers into it.  There is no CPU fallback: every call raises when
+the library or a GPU is missing.
+
+Python mirrors of the reference's host interfaces:
+  * :func:`scattervolume`    the scattervolume solve (ScatterVolume.cpp:506-1019)
+  * :class:`Solution`        solution vectors and the CoupledWaveStructure records
+  * :func:`zgeev`, :func:`zgesv`, :func:`zinverse`, :func:`zgemm`   third_Lapack.h:38-85
+  * :meth:`Context.field_init_f32` / :meth:`Context.field_evaluate`  gpuEvaluator.h:14-17
+  * :func:`cw_read_unpacked` CoupledWaveStructure::load + cw_allocate/cw_unpack
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+__all__ = ["Context", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
+           "cw_read_unpacked", "WAVE_F32", "WAVE_F64"]
+
+WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
+WAVE_F64 = np.dtype([("E0", np.complex128, 3), ("k", np.complex128, 3)])
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP device and stream (cwx_create)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().cwx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().cwx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_stream(self, stream_handle: Optional[int]):
+        check(lib().cwx_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def synchronize(self):
+        check(lib().cwx_synchronize(self.h))
+
+    # ---- field evaluation (gpuEvaluator.h:14-17) ----
+    def field_init_f32(self, W: np.ndarray, z_layers, begin, end):
+        W = np.ascontiguousarray(W, dtype=WAVE_F32)
+        z = np.ascontiguousarray(z_layers, dtype=np.float32)
+        b = np.ascontiguousarray(begin, dtype=np.int32)
+        e = np.ascontiguousarray(end, dtype=np.int32)
+        check(lib().cwx_field_init_f32(self.h, _ptr(W), len(W), _ptr(z), len(z), _ptr(b), _ptr(e)))
+
+    def field_init_f64(self, W: np.ndarray, z_layers, begin, end):
+        W = np.ascontiguousarray(W, dtype=WAVE_F64)
+        z = np.ascontiguousarray(z_layers, dtype=np.float32)
+        b = np.ascontiguousarray(begin, dtype=np.int32)
+        e = np.ascontiguousarray(end, dtype=np.int32)
+        check(lib().cwx_field_init_f64(self.h, _ptr(W), len(W), _ptr(z), len(z), _ptr(b), _ptr(e)))
+
+    def field_init_solution(self, sol: "Solution"):
+        check(lib().cwx_field_init_solution(self.h, sol.h))
+
+    def field_evaluate(self, x_start, y_start, z_start, x, y, z, d, N):
+        """gpu_cw_evaluate: returns (E_xy, E_xz, E_yz), each (N, N, 3) complex64."""
+        out = [np.zeros((N, N, 3), np.complex64) for _ in range(3)]
+        check(lib().cwx_field_evaluate(self.h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), x_start, y_start,
+                                       z_start, x, y, z, d, N))
+        return tuple(out)
+
+    def field_grid_dev(self, d_out: int, out_f64: bool, tol_mode: int, x0, y0, z0, dx, dy, dz, nx, ny, nz):
+        """Evaluate a grid into device memory at address d_out ([nz][ny][nx][3] complex)."""
+        check(lib().cwx_field_grid_dev(self.h, C.c_void_p(d_out), int(out_f64), tol_mode, x0, y0, z0, dx, dy,
+                                       dz, nx, ny, nz))
+
+
+def _params(lam=1.0, direction=(0, 0, 1), ex=(0, 0), ey=(1, 0), ez=(0, 0), n=(1, 1), kappa=0.0,
+            size=(10, 10, 2), z=0.0, coef: Sequence[int] = (), psf=False, external=False, npz=False,
+            no_cutoff=False, eig_mode=0) -> _capi.cwx_volume_params:
+    p = _capi.cwx_volume_params()
+    lib().cwx_volume_params_default(C.byref(p))
+    p.lambda_ = lam
+    p.direction[:] = [float(v) for v in direction]
+    p.ex[:] = [float(v) for v in ex]
+    p.ey[:] = [float(v) for v in ey]
+    p.ez[:] = [float(v) for v in ez]
+    if len(n) != 2:
+        raise ValueError("n takes the two half-space indices")
+    p.n[:] = [float(v) for v in n]
+    p.kappa = float(kappa[0] if isinstance(kappa, (tuple, list)) else kappa)
+    p.size[:] = [float(v) for v in size]
+    p.z = float(z)
+    coef = list(coef)
+    if len(coef) > 2:
+        raise ValueError("coef takes 0, 1 or 2 values")
+    p.ncoef = len(coef)
+    for i, c in enumerate(coef):
+        p.coef[i] = int(c)
+    p.psf, p.external, p.npz, p.no_cutoff = int(psf), int(external), int(npz), int(no_cutoff)
+    p.eig_mode = int(eig_mode)
+    return p
+
+
+class Solution:
+    """Device-resident result of one solve (cwx_solution)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+        info = _capi.cwx_solution_info()
+        check(lib().cwx_solution_get_info(self.h, C.byref(info)))
+        self.M0, self.M1, self.MF, self.L = info.M0, info.M1, info.MF, info.L
+        self.cutoff, self.status, self.k, self.nrec = bool(info.cutoff), info.info, info.k, info.nrec
+        self.nlists = info.nlists
+
+    def free(self):
+        if self.h:
+            lib().cwx_solution_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def _get(self, what: int, dtype=np.complex128):
+        n = lib().cwx_solution_size(self.h, what)
+        out = np.zeros(max(n, 0), dtype=dtype)
+        if n > 0:
+            check(lib().cwx_solution_copy(self.ctx.h, self.h, what, _ptr(out)))
+        return out
+
+    def device_ptr(self, what: int) -> int:
+        return lib().cwx_solution_device_ptr(self.h, what) or 0
+
+    @property
+    def x(self):
+        return self._get(_capi.SOL_X)
+
+    @property
+    def gammas(self):
+        return self._get(_capi.SOL_GAMMA).reshape(self.L, 4 * self.MF)
+
+    @property
+    def betas(self):
+        return self._get(_capi.SOL_BETA).reshape(-1, 4 * self.MF)
+
+    @property
+    def Q_check_hat(self):
+        """(L, 3MF, 4MF) arrays [Q_check | Q_hat] (column-major storage reshaped)."""
+        a = self._get(_capi.SOL_QCH)
+        return a.reshape(-1, 4 * self.MF, 3 * self.MF).transpose(0, 2, 1)
+
+    @property
+    def z(self):
+        return self._get(_capi.SOL_Z, np.float64)
+
+    def setup_array(self, what: int):
+        return self._get(what)
+
+    @property
+    def records(self):
+        return self._get(_capi.SOL_RECORDS).reshape(-1, 6)
+
+    def lists(self):
+        off = np.zeros(self.nlists, np.int64)
+        cnt = np.zeros(self.nlists, np.int64)
+        check(lib().cwx_solution_lists(self.h, _ptr(off), _ptr(cnt)))
+        return off, cnt
+
+    def write_cw(self, path: str):
+        check(lib().cwx_solution_write_cw(self.ctx.h, self.h, path.encode()))
+
+
+def scattervolume(ctx: Context, sample: np.ndarray, **opts) -> Solution:
+    """Solve a sample of complex refractive indices, shape (L, s1, s2) or (s1, s2), C order.
+
+    Keyword options mirror the scattervolume command line (ScatterVolume.cpp:510-530): lam,
+    direction, ex, ey, ez, n, kappa, size, z, coef, psf, external, npz; plus no_cutoff and eig_mode.
+    ``sample`` may also be a device pointer (int) with ``shape=(L, s1, s2)`` given.
+    """
+    shape = opts.pop("shape", None)
+    p = _params(**opts)
+    h = C.c_void_p()
+    if isinstance(sample, int):
+        L, s1, s2 = shape
+        st = lib().cwx_solve_volume_dev(ctx.h, C.byref(p), C.c_void_p(sample), L, s1, s2, C.byref(h))
+    else:
+        a = np.asarray(sample, dtype=np.complex128)
+        if a.ndim == 2:
+            a = a[None]
+        if a.ndim != 3:
+            raise ValueError("sample must be 2-D or 3-D")
+        a = np.ascontiguousarray(a)
+        st = lib().cwx_solve_volume(ctx.h, C.byref(p), _ptr(a), a.shape[0], a.shape[1], a.shape[2], C.byref(h))
+    if st != _capi.CWX_OK:
+        msg = lib().cwx_last_error().decode()
+        if h:
+            lib().cwx_solution_free(h)
+        raise _capi.CwxError(st, msg)
+    return Solution(ctx, h)
+
+
+def zgeev(ctx: Context, A: np.ndarray):
+    """MKL_eigensolve contract: returns (w, VR); A is not modified."""
+    n = A.shape[0]
+    a = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
+    w = np.zeros(n, np.complex128)
+    V = np.zeros((n, n), np.complex128, order="F")
+    check(lib().cwx_zgeev(ctx.h, n, _ptr(a), n, _ptr(w), _ptr(V), n))
+    return w, V
+
+
+def zgesv(ctx: Context, A: np.ndarray, B: np.ndarray):
+    """MKL_linearsolve contract: returns X with A X = B."""
+    n = A.shape[0]
+    a = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
+    b2 = B.reshape(n, -1)
+    b = np.asfortranarray(b2, dtype=np.complex128).copy(order="F")
+    check(lib().cwx_zgesv(ctx.h, n, b.shape[1], _ptr(a), n, None, _ptr(b), n))
+    return b.reshape(B.shape)
+
+
+def zinverse(ctx: Context, A: np.ndarray):
+    n = A.shape[0]
+    a = np.asfortranarray(A, dtype=np.complex128)
+    out = np.zeros((n, n), np.complex128, order="F")
+    check(lib().cwx_zinverse(ctx.h, n, _ptr(a), n, _ptr(out), n))
+    return out
+
+
+def zgemm(ctx: Context, A: np.ndarray, B: np.ndarray, alpha=1.0):
+    m, k = A.shape
+    n = B.shape[1]
+    a = np.asfortranarray(A, dtype=np.complex128)
+    b = np.asfortranarray(B, dtype=np.complex128)
+    c = np.zeros((m, n), np.complex128, order="F")
+    al = _capi.cwx_complex(complex(alpha).real, complex(alpha).imag)
+    check(lib().cwx_zgemm(ctx.h, m, n, k, al, _ptr(a), m, _ptr(b), k, _capi.cwx_complex(0, 0), _ptr(c), m))
+    return c
+
+
+def cw_read_unpacked(path: str):
+    """Read a .cw file: returns (W [WAVE_F64], z_layers float64, begin, end)."""
+    nw = C.c_int64()
+    nl = C.c_int()
+    check(lib().cwx_cw_read_unpacked(path.encode(), C.byref(nw), C.byref(nl), None, None, None, None))
+    W = np.zeros(nw.value, WAVE_F64)
+    z = np.zeros(nl.value, np.float64)
+    b = np.zeros(nl.value + 1, np.int32)
+    e = np.zeros(nl.value + 1, np.int32)
+    check(lib().cwx_cw_read_unpacked(path.encode(), C.byref(nw), C.byref(nl), _ptr(W), _ptr(z), _ptr(b), _ptr(e)))
+    return W, z, b, e

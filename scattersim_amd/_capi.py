@@ -1,0 +1,118 @@
+"""ctypes binding of libcwx (include/cwx.h).  Loading fails loudly when the library is missing."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libcwx.so")
+
+
+class cwx_complex(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class cwx_volume_params(C.Structure):
+    _fields_ = [
+        ("lambda_", C.c_double),
+        ("direction", C.c_double * 3),
+        ("ex", C.c_double * 2),
+        ("ey", C.c_double * 2),
+        ("ez", C.c_double * 2),
+        ("n", C.c_double * 2),
+        ("kappa", C.c_double),
+        ("size", C.c_double * 3),
+        ("z", C.c_double),
+        ("ncoef", C.c_int),
+        ("coef", C.c_int * 2),
+        ("psf", C.c_int),
+        ("external", C.c_int),
+        ("npz", C.c_int),
+        ("no_cutoff", C.c_int),
+        ("eig_mode", C.c_int),
+    ]
+
+
+class cwx_solution_info(C.Structure):
+    _fields_ = [
+        ("M0", C.c_int), ("M1", C.c_int), ("MF", C.c_int), ("L", C.c_int),
+        ("cutoff", C.c_int), ("info", C.c_int), ("k", C.c_double), ("nrec", C.c_int64),
+        ("nlists", C.c_int),
+    ]
+
+
+CWX_OK, CWX_ERR_ARG, CWX_ERR_DEVICE, CWX_ERR_IO, CWX_ERR_STATE = 0, -1, -2, -3, -4
+(SOL_X, SOL_GAMMA, SOL_BETA, SOL_QCH, SOL_Z, SOL_RECORDS, SOL_SX, SOL_SY, SOL_SZ0, SOL_SZ1, SOL_EF,
+ SOL_GD, SOL_GC, SOL_NIF) = range(14)
+
+P = C.c_void_p
+I = C.c_int
+I64 = C.c_int64
+F = C.c_float
+D = C.c_double
+
+# name -> (restype, argtypes); mirrors include/cwx.h one to one
+SIGNATURES = {
+    "cwx_last_error": (C.c_char_p, []),
+    "cwx_version": (C.c_char_p, []),
+    "cwx_create": (I, [I, C.POINTER(P)]),
+    "cwx_destroy": (I, [P]),
+    "cwx_set_stream": (I, [P, P]),
+    "cwx_synchronize": (I, [P]),
+    "cwx_zgeev": (I, [P, I, P, I, P, P, I]),
+    "cwx_zgesv": (I, [P, I, I, P, I, P, P, I]),
+    "cwx_zinverse": (I, [P, I, P, I, P, I]),
+    "cwx_zgemm": (I, [P, I, I, I, cwx_complex, P, I, P, I, cwx_complex, P, I]),
+    "cwx_zgeev_batched_dev": (I, [P, I, I, P, I64, P, I64, P, I64, P]),
+    "cwx_zgetrf_batched_dev": (I, [P, I, I, P, I64, P, I64, P]),
+    "cwx_zgetrs_batched_dev": (I, [P, I, I, I, P, I64, P, I64, P, I64]),
+    "cwx_zgemm_batched_dev": (I, [P, I, I, I, I, I, cwx_complex, P, I, I64, P, I, I64, cwx_complex, P, I,
+                                  I64, I]),
+    "cwx_volume_params_default": (None, [C.POINTER(cwx_volume_params)]),
+    "cwx_solve_volume": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),
+    "cwx_solve_volume_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),
+    "cwx_phi_build_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, I, P, C.POINTER(I)]),
+    "cwx_solution_get_info": (I, [P, C.POINTER(cwx_solution_info)]),
+    "cwx_solution_size": (I64, [P, I]),
+    "cwx_solution_copy": (I, [P, P, I, P]),
+    "cwx_solution_device_ptr": (P, [P, I]),
+    "cwx_solution_lists": (I, [P, P, P]),
+    "cwx_solution_write_cw": (I, [P, P, C.c_char_p]),
+    "cwx_solution_free": (I, [P]),
+    "cwx_cw_read_unpacked": (I, [C.c_char_p, C.POINTER(I64), C.POINTER(I), P, P, P, P]),
+    "cwx_field_init_f32": (I, [P, P, I64, P, I, P, P]),
+    "cwx_field_init_f64": (I, [P, P, I64, P, I, P, P]),
+    "cwx_field_init_solution": (I, [P, P]),
+    "cwx_field_evaluate": (I, [P, P, P, P, F, F, F, F, F, F, F, I]),
+    "cwx_field_grid_dev": (I, [P, P, I, I, F, F, F, F, F, F, I, I, I]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libcwx; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libcwx is not built ({LIB_PATH} missing): run "
+                               "`python -m scattersim_amd.build` or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class CwxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cwx error {code}: {msg}")
+        self.code = code
+
+
+def check(status: int) -> int:
+    if status != CWX_OK:
+        raise CwxError(status, lib().cwx_last_error().decode())
+    return status

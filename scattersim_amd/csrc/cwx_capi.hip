@@ -1,0 +1,667 @@
+// extern "C" entry points of libcwx (declared in include/cwx.h).
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cwx.h"
+#include "cwx_solver.h"
+
+struct cwx_context {
+  cwx::Context c;
+};
+struct cwx_solution {
+  cwx::Solution s;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const cwx::Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CWX_ERR_DEVICE, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(CWX_ERR_ARG, e.what());
+  }
+}
+
+using cwx::zc;
+
+void set_device(cwx_context* ctx) { CWX_HIP(hipSetDevice(ctx->c.device)); }
+
+// host -> device copy of a column-major (rows x cols) block with leading dimension ld, packed
+void put_matrix(const cwx_complex* h, int rows, int cols, int ld, zc* d, hipStream_t s) {
+  if (ld == rows) {
+    CWX_HIP(hipMemcpyAsync(d, h, sizeof(zc) * (size_t)rows * cols, hipMemcpyHostToDevice, s));
+  } else {
+    CWX_HIP(hipMemcpy2DAsync(d, sizeof(zc) * rows, h, sizeof(zc) * ld, sizeof(zc) * rows, cols,
+                             hipMemcpyHostToDevice, s));
+  }
+}
+void get_matrix(const zc* d, int rows, int cols, cwx_complex* h, int ld, hipStream_t s) {
+  if (ld == rows) {
+    CWX_HIP(hipMemcpyAsync(h, d, sizeof(zc) * (size_t)rows * cols, hipMemcpyDeviceToHost, s));
+  } else {
+    CWX_HIP(hipMemcpy2DAsync(h, sizeof(zc) * ld, d, sizeof(zc) * rows, sizeof(zc) * rows, cols,
+                             hipMemcpyDeviceToHost, s));
+  }
+}
+
+cwx::VolumeOptions to_options(const cwx_volume_params* p) {
+  CWX_CHECK(p != nullptr, "params must not be NULL");
+  cwx::VolumeOptions o;
+  o.lambda = p->lambda;
+  for (int i = 0; i < 3; ++i) o.direction[i] = p->direction[i];
+  for (int i = 0; i < 2; ++i) {
+    o.ex[i] = p->ex[i];
+    o.ey[i] = p->ey[i];
+    o.ez[i] = p->ez[i];
+    o.n[i] = p->n[i];
+    o.coef[i] = p->coef[i];
+  }
+  o.kappa = p->kappa;
+  for (int i = 0; i < 3; ++i) o.size[i] = p->size[i];
+  o.z = p->z;
+  CWX_CHECK(p->ncoef >= 0 && p->ncoef <= 2, "--coef takes 0, 1 or 2 values");
+  o.ncoef = p->ncoef;
+  o.psf = p->psf != 0;
+  o.external = p->external != 0;
+  o.npz = p->npz != 0;
+  o.no_cutoff = p->no_cutoff != 0;
+  CWX_CHECK(p->eig_mode == 0 || p->eig_mode == 1, "eig_mode must be 0 or 1");
+  o.eig_mode = p->eig_mode;
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cwx_last_error(void) { return g_err.c_str(); }
+const char* cwx_version(void) { return "cwx 0.1 (gfx950)"; }
+
+int cwx_create(int device, cwx_context** ctx) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx != nullptr, "ctx must not be NULL");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(CWX_ERR_DEVICE, "no HIP device available");
+    CWX_CHECK(device >= 0 && device < count, "device index out of range");
+    auto c = std::make_unique<cwx_context>();
+    c->c.device = device;
+    CWX_HIP(hipSetDevice(device));
+    CWX_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    c->c.own_stream = true;
+    *ctx = c.release();
+    return CWX_OK;
+  });
+}
+
+int cwx_destroy(cwx_context* ctx) {
+  return guard([&]() -> int {
+    if (!ctx) return CWX_OK;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    hipStream_t own = ctx->c.own_stream ? ctx->c.stream : nullptr;
+    delete ctx;
+    if (own) (void)hipStreamDestroy(own);
+    return CWX_OK;
+  });
+}
+
+int cwx_set_stream(cwx_context* ctx, void* hip_stream) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx != nullptr, "ctx must not be NULL");
+    set_device(ctx);
+    if (ctx->c.own_stream) CWX_HIP(hipStreamSynchronize(ctx->c.stream));
+    if (hip_stream) {
+      if (ctx->c.own_stream) CWX_HIP(hipStreamDestroy(ctx->c.stream));
+      ctx->c.stream = (hipStream_t)hip_stream;
+      ctx->c.own_stream = false;
+    } else if (!ctx->c.own_stream) {
+      CWX_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+      ctx->c.own_stream = true;
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_synchronize(cwx_context* ctx) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx != nullptr, "ctx must not be NULL");
+    set_device(ctx);
+    CWX_HIP(hipStreamSynchronize(ctx->c.stream));
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+int cwx_zgeev(cwx_context* ctx, int n, cwx_complex* A, int lda, cwx_complex* w, cwx_complex* VR, int ldvr) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && A && w && VR, "NULL argument");
+    CWX_CHECK(n >= 1 && lda >= n && ldvr >= n, "bad matrix dimensions");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    cwx::DevBuf<zc> dA((size_t)n * n), dW(n), dV((size_t)n * n);
+    cwx::DevBuf<int> dinfo(1);
+    put_matrix(A, n, n, lda, dA.p, s);
+    cwx::zgeev_batched(n, dA.p, n, (long)n * n, dW.p, n, dV.p, n, (long)n * n, 1, ctx->c.eig, dinfo.p, s);
+    int info = 0;
+    get_matrix(dA.p, n, n, A, lda, s);
+    get_matrix(dV.p, n, n, VR, ldvr, s);
+    CWX_HIP(hipMemcpyAsync(w, dW.p, sizeof(zc) * n, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    if (info) return fail(info, "zgeev: QR iteration did not converge");
+    return CWX_OK;
+  });
+}
+
+int cwx_zgesv(cwx_context* ctx, int n, int nrhs, cwx_complex* A, int lda, int* ipiv, cwx_complex* B, int ldb) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && A && B, "NULL argument");
+    CWX_CHECK(n >= 1 && nrhs >= 1 && lda >= n && ldb >= n, "bad matrix dimensions");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    cwx::DevBuf<zc> dA((size_t)n * n), dB((size_t)n * nrhs);
+    cwx::DevBuf<int> dP(n), dinfo(1);
+    put_matrix(A, n, n, lda, dA.p, s);
+    put_matrix(B, n, nrhs, ldb, dB.p, s);
+    cwx::zgetrf_batched(n, dA.p, n, (long)n * n, dP.p, n, dinfo.p, 1, s);
+    cwx::zgetrs_batched(n, nrhs, dA.p, n, (long)n * n, dP.p, n, dB.p, n, (long)n * nrhs, 1, s);
+    get_matrix(dA.p, n, n, A, lda, s);
+    get_matrix(dB.p, n, nrhs, B, ldb, s);
+    std::vector<int> hp(n);
+    int info = 0;
+    CWX_HIP(hipMemcpyAsync(hp.data(), dP.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    if (ipiv)
+      for (int i = 0; i < n; ++i) ipiv[i] = hp[i] + 1;
+    if (info) return fail(info, "zgesv: exactly singular matrix");
+    return CWX_OK;
+  });
+}
+
+int cwx_zinverse(cwx_context* ctx, int n, const cwx_complex* A, int lda, cwx_complex* Ainv, int ldinv) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && A && Ainv, "NULL argument");
+    CWX_CHECK(n >= 1 && lda >= n && ldinv >= n, "bad matrix dimensions");
+    std::vector<cwx_complex> a((size_t)n * n), b((size_t)n * n);
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < n; ++r) {
+        a[(size_t)c * n + r] = A[(size_t)c * lda + r];
+        b[(size_t)c * n + r] = cwx_complex{r == c ? 1.0 : 0.0, 0.0};
+      }
+    int st = cwx_zgesv(ctx, n, n, a.data(), n, nullptr, b.data(), n);
+    if (st != CWX_OK) return st;
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < n; ++r) Ainv[(size_t)c * ldinv + r] = b[(size_t)c * n + r];
+    return CWX_OK;
+  });
+}
+
+int cwx_zgemm(cwx_context* ctx, int m, int n, int k, cwx_complex alpha, const cwx_complex* A, int lda,
+              const cwx_complex* B, int ldb, cwx_complex beta, cwx_complex* C, int ldc) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && A && B && C, "NULL argument");
+    CWX_CHECK(m >= 1 && n >= 1 && k >= 1 && lda >= m && ldb >= k && ldc >= m, "bad matrix dimensions");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    cwx::DevBuf<zc> dA((size_t)m * k), dB((size_t)k * n), dC((size_t)m * n);
+    put_matrix(A, m, k, lda, dA.p, s);
+    put_matrix(B, k, n, ldb, dB.p, s);
+    put_matrix(C, m, n, ldc, dC.p, s);
+    cwx::zgemm_batched(cwx::OpN, cwx::OpN, m, n, k, zc{alpha.re, alpha.im}, dA.p, m, 0, dB.p, k, 0,
+                       zc{beta.re, beta.im}, dC.p, m, 0, 1, s);
+    get_matrix(dC.p, m, n, C, ldc, s);
+    CWX_HIP(hipStreamSynchronize(s));
+    return CWX_OK;
+  });
+}
+
+int cwx_zgeev_batched_dev(cwx_context* ctx, int n, int batch, void* dA, int64_t strideA, void* dW,
+                          int64_t strideW, void* dV, int64_t strideV, int* h_info) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dA && dW && dV, "NULL argument");
+    CWX_CHECK(n >= 1 && batch >= 1 && strideA >= (int64_t)n * n && strideV >= (int64_t)n * n && strideW >= n,
+              "bad dimensions");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    ctx->c.info.ensure(batch);
+    cwx::zgeev_batched(n, (zc*)dA, n, strideA, (zc*)dW, strideW, (zc*)dV, n, strideV, batch, ctx->c.eig,
+                       ctx->c.info.p, s);
+    if (h_info) {
+      CWX_HIP(hipMemcpyAsync(h_info, ctx->c.info.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+      CWX_HIP(hipStreamSynchronize(s));
+      for (int b = 0; b < batch; ++b)
+        if (h_info[b]) return fail(h_info[b], "zgeev: QR iteration did not converge");
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_zgetrf_batched_dev(cwx_context* ctx, int n, int batch, void* dA, int64_t strideA, void* dIpiv,
+                           int64_t strideP, int* h_info) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dA && dIpiv, "NULL argument");
+    CWX_CHECK(n >= 1 && batch >= 1 && strideA >= (int64_t)n * n && strideP >= n, "bad dimensions");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    ctx->c.info.ensure(batch);
+    cwx::zgetrf_batched(n, (zc*)dA, n, strideA, (int*)dIpiv, strideP, ctx->c.info.p, batch, s);
+    if (h_info) {
+      CWX_HIP(hipMemcpyAsync(h_info, ctx->c.info.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+      CWX_HIP(hipStreamSynchronize(s));
+      for (int b = 0; b < batch; ++b)
+        if (h_info[b]) return fail(h_info[b], "zgetrf: exactly singular matrix");
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_zgetrs_batched_dev(cwx_context* ctx, int n, int nrhs, int batch, const void* dLU, int64_t strideA,
+                           const void* dIpiv, int64_t strideP, void* dB, int64_t strideB) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dLU && dIpiv && dB, "NULL argument");
+    CWX_CHECK(n >= 1 && nrhs >= 1 && batch >= 1, "bad dimensions");
+    set_device(ctx);
+    cwx::zgetrs_batched(n, nrhs, (const zc*)dLU, n, strideA, (const int*)dIpiv, strideP, (zc*)dB, n, strideB,
+                        batch, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_zgemm_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n, int k, cwx_complex alpha,
+                          const void* dA, int lda, int64_t strideA, const void* dB, int ldb, int64_t strideB,
+                          cwx_complex beta, void* dC, int ldc, int64_t strideC, int batch) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dA && dB && dC, "NULL argument");
+    CWX_CHECK(m >= 1 && n >= 1 && k >= 1 && batch >= 1, "bad dimensions");
+    CWX_CHECK(transa >= 0 && transa <= 2 && transb >= 0 && transb <= 2, "trans must be 0 (N), 1 (T) or 2 (C)");
+    set_device(ctx);
+    cwx::zgemm_batched((cwx::Op)transa, (cwx::Op)transb, m, n, k, zc{alpha.re, alpha.im}, (const zc*)dA, lda,
+                       strideA, (const zc*)dB, ldb, strideB, zc{beta.re, beta.im}, (zc*)dC, ldc, strideC, batch,
+                       ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+void cwx_volume_params_default(cwx_volume_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->lambda = 1.0;
+  p->direction[2] = 1.0;
+  p->ey[0] = 1.0;
+  p->n[0] = p->n[1] = 1.0;
+  p->size[0] = p->size[1] = 10.0;
+  p->size[2] = 2.0;
+}
+
+static int solve_common(cwx_context* ctx, const cwx_volume_params* p, const zc* d_sample, int L, int s1, int s2,
+                        cwx_solution** out) {
+  auto sol = std::make_unique<cwx_solution>();
+  cwx::solve_volume(ctx->c, to_options(p), d_sample, L, s1, s2, sol->s);
+  const int info = sol->s.info;
+  const std::string msg = sol->s.info_msg;
+  *out = sol.release();
+  if (info) return fail(info, msg);
+  return CWX_OK;
+}
+
+int cwx_solve_volume(cwx_context* ctx, const cwx_volume_params* p, const cwx_complex* sample, int L, int s1,
+                     int s2, cwx_solution** out) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && p && sample && out, "NULL argument");
+    CWX_CHECK(L >= 1 && s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
+    set_device(ctx);
+    const size_t n = (size_t)L * s1 * s2;
+    cwx::DevBuf<zc> d(n);
+    CWX_HIP(hipMemcpyAsync(d.p, sample, sizeof(zc) * n, hipMemcpyHostToDevice, ctx->c.stream));
+    return solve_common(ctx, p, d.p, L, s1, s2, out);
+  });
+}
+
+int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1,
+                         int s2, cwx_solution** out) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && p && d_sample && out, "NULL argument");
+    CWX_CHECK(L >= 1 && s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
+    set_device(ctx);
+    return solve_common(ctx, p, (const zc*)d_sample, L, s1, s2, out);
+  });
+}
+
+int cwx_phi_build_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
+                      int blocks, void* d_out, int* n_out) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && p && n_out, "NULL argument");
+    CWX_CHECK(L >= 1 && s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
+    CWX_CHECK(blocks == 0 || blocks == 1, "blocks must be 0 (Phi) or 1 (P and Q)");
+    const cwx::VolumeOptions o = to_options(p);
+    const cwx::Setup st = cwx::make_setup(o, L, s1, s2);
+    *n_out = blocks ? 2 * st.MF : 4 * st.MF;
+    if (!d_out) return CWX_OK;
+    CWX_CHECK(d_sample != nullptr, "NULL sample");
+    set_device(ctx);
+    cwx::build_phi_dev(o, (const zc*)d_sample, L, s1, s2, blocks, (zc*)d_out, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_solution_get_info(const cwx_solution* sol, cwx_solution_info* info) {
+  return guard([&]() -> int {
+    CWX_CHECK(sol && info, "NULL argument");
+    const auto& s = sol->s;
+    info->M0 = s.st.M0;
+    info->M1 = s.st.M1;
+    info->MF = s.st.MF;
+    info->L = s.st.L;
+    info->cutoff = s.st.cutoff ? 1 : 0;
+    info->info = s.info;
+    info->k = s.st.k;
+    info->nrec = s.nrec;
+    info->nlists = (int)s.seg_cnt.size();
+    return CWX_OK;
+  });
+}
+
+int64_t cwx_solution_size(const cwx_solution* sol, int what) {
+  if (!sol) return -1;
+  const auto& s = sol->s;
+  const int64_t MF = s.st.MF, L = s.st.L, n4 = 4 * MF;
+  switch (what) {
+    case CWX_SOL_X: return 6 * MF;
+    case CWX_SOL_GAMMA: return L * n4;
+    case CWX_SOL_BETA: return s.beta.p ? L * n4 : 0;
+    case CWX_SOL_QCH: return s.Qch.p ? L * 3 * MF * n4 : 0;
+    case CWX_SOL_Z: return L + 1;
+    case CWX_SOL_RECORDS: return s.nrec * 6;
+    case CWX_SOL_SX: case CWX_SOL_SY: case CWX_SOL_SZ0: case CWX_SOL_SZ1: return MF;
+    case CWX_SOL_EF: return 3 * MF;
+    case CWX_SOL_GD: case CWX_SOL_GC: return L * n4 * n4;
+    case CWX_SOL_NIF: return L * MF;
+    default: return -1;
+  }
+}
+
+const void* cwx_solution_device_ptr(const cwx_solution* sol, int what) {
+  if (!sol) return nullptr;
+  const auto& s = sol->s;
+  switch (what) {
+    case CWX_SOL_X: return s.x.p;
+    case CWX_SOL_GAMMA: return s.gam.p;
+    case CWX_SOL_BETA: return s.beta.p;
+    case CWX_SOL_QCH: return s.Qch.p;
+    case CWX_SOL_RECORDS: return s.rec.p;
+    case CWX_SOL_GD: return s.Gd.p;
+    case CWX_SOL_GC: return s.Gc.p;
+    case CWX_SOL_NIF: return s.NIF.p;
+    default: return nullptr;
+  }
+}
+
+int cwx_solution_copy(cwx_context* ctx, const cwx_solution* sol, int what, void* host_dst) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && sol && host_dst, "NULL argument");
+    const int64_t n = cwx_solution_size(sol, what);
+    CWX_CHECK(n >= 0, "unknown solution array");
+    const auto& s = sol->s;
+    auto hostcpy = [&](const std::vector<zc>& v) {
+      std::memcpy(host_dst, v.data(), sizeof(zc) * v.size());
+      return CWX_OK;
+    };
+    switch (what) {
+      case CWX_SOL_Z: std::memcpy(host_dst, s.st.z.data(), sizeof(double) * s.st.z.size()); return CWX_OK;
+      case CWX_SOL_SX: return hostcpy(s.st.Sx);
+      case CWX_SOL_SY: return hostcpy(s.st.Sy);
+      case CWX_SOL_SZ0: return hostcpy(s.st.Sz0);
+      case CWX_SOL_SZ1: return hostcpy(s.st.Sz1);
+      case CWX_SOL_EF: return hostcpy(s.st.EF);
+      default: break;
+    }
+    const void* d = cwx_solution_device_ptr(sol, what);
+    if (n == 0) return CWX_OK;
+    CWX_CHECK(d != nullptr, "solution array not available (--external skips beta and Q)");
+    set_device(ctx);
+    CWX_HIP(hipMemcpyAsync(host_dst, d, sizeof(zc) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+    CWX_HIP(hipStreamSynchronize(ctx->c.stream));
+    return CWX_OK;
+  });
+}
+
+int cwx_solution_lists(const cwx_solution* sol, int64_t* offsets, int64_t* counts) {
+  return guard([&]() -> int {
+    CWX_CHECK(sol && offsets && counts, "NULL argument");
+    for (size_t i = 0; i < sol->s.seg_cnt.size(); ++i) {
+      offsets[i] = sol->s.seg_off[i];
+      counts[i] = sol->s.seg_cnt[i];
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_solution_write_cw(cwx_context* ctx, const cwx_solution* sol, const char* path) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && sol && path, "NULL argument");
+    set_device(ctx);
+    try {
+      cwx::cw_write_file(sol->s, path, ctx->c.stream);
+    } catch (const cwx::Error& e) {
+      if (e.code == -1) return fail(CWX_ERR_IO, e.what());
+      throw;
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_solution_free(cwx_solution* sol) {
+  return guard([&]() -> int {
+    delete sol;
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+int cwx_cw_read_unpacked(const char* path, int64_t* nwaves, int* layers, cwx_wave_f64* W, double* z_layers,
+                         int* waves_begin, int* waves_end) {
+  return guard([&]() -> int {
+    CWX_CHECK(path && nwaves && layers, "NULL argument");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(CWX_ERR_IO, std::string("cannot open ") + path);
+    std::vector<unsigned char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    size_t off = 0;
+    auto need = [&](size_t n) {
+      if (off + n > buf.size()) throw cwx::Error(CWX_ERR_IO, std::string("truncated .cw file ") + path);
+    };
+    auto u64 = [&]() { need(8); uint64_t v; std::memcpy(&v, &buf[off], 8); off += 8; return v; };
+    auto f64 = [&]() { need(8); double v; std::memcpy(&v, &buf[off], 8); off += 8; return v; };
+    int rec_c = 5;
+    if (buf.size() >= 9 && (buf[0] == 0 || buf[0] == 1)) {
+      uint64_t p;
+      std::memcpy(&p, &buf[1], 8);
+      if (p == 4 || p == 8) { off = 1; rec_c = 6; }
+    }
+    const uint64_t prec = u64();
+    if (prec != 8) return fail(CWX_ERR_IO, "only double-precision .cw files are supported");
+    struct List { size_t off; uint64_t n; };
+    std::vector<List> lists;
+    std::vector<double> zs;
+    auto read_list = [&]() {
+      const uint64_t n = u64();
+      need(n * rec_c * 16);
+      lists.push_back(List{off, n});
+      off += n * rec_c * 16;
+    };
+    read_list();
+    const uint64_t nb = u64();
+    for (uint64_t l = 0; l < nb; ++l) {
+      zs.push_back(f64());
+      read_list();
+      read_list();
+    }
+    int64_t total = 0;
+    for (auto& L : lists) total += (int64_t)L.n;
+    *nwaves = total;
+    *layers = (int)nb;
+    if (!W) return CWX_OK;
+    CWX_CHECK(z_layers && waves_begin && waves_end, "NULL argument");
+    int64_t idx = 0;
+    for (auto& L : lists)
+      for (uint64_t i = 0; i < L.n; ++i) {
+        const double* r = (const double*)&buf[L.off + i * rec_c * 16];
+        cwx_wave_f64& w = W[idx++];
+        if (rec_c == 6) {
+          for (int c = 0; c < 3; ++c) {
+            w.E0[c] = cwx_complex{r[2 * c], r[2 * c + 1]};
+            w.k[c] = cwx_complex{r[6 + 2 * c], r[6 + 2 * c + 1]};
+          }
+        } else {
+          w.E0[0] = cwx_complex{r[0], r[1]};
+          w.E0[1] = cwx_complex{r[2], r[3]};
+          w.E0[2] = cwx_complex{0.0, 0.0};
+          for (int c = 0; c < 3; ++c) w.k[c] = cwx_complex{r[4 + 2 * c], r[4 + 2 * c + 1]};
+        }
+      }
+    // region map of cw_allocate (cpuEvaluator.cpp:34-53)
+    int64_t t = (int64_t)lists[0].n;
+    waves_begin[0] = 0;
+    for (uint64_t l = 0; l < nb; ++l) {
+      z_layers[l] = zs[l];
+      t += (int64_t)lists[1 + 2 * l].n;
+      waves_end[l] = (int)t;
+      waves_begin[l + 1] = (int)t;
+      t += (int64_t)lists[2 + 2 * l].n;
+    }
+    waves_end[nb] = (int)t;
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+static int field_init(cwx_context* ctx, const void* W, size_t recsize, int prec, int64_t nwaves,
+                      const float* z_layers, int layers, const int* wb, const int* we) {
+  CWX_CHECK(ctx && W && wb && we && (layers == 0 || z_layers), "NULL argument");
+  CWX_CHECK(nwaves >= 1 && layers >= 0, "empty wave table");
+  for (int l = 0; l <= layers; ++l)
+    CWX_CHECK(wb[l] >= 0 && we[l] >= wb[l] && we[l] <= nwaves, "region bounds outside the wave table");
+  set_device(ctx);
+  auto& c = ctx->c;
+  hipStream_t s = c.stream;
+  c.ftab.alloc(recsize * nwaves);
+  CWX_HIP(hipMemcpyAsync(c.ftab.p, W, recsize * nwaves, hipMemcpyHostToDevice, s));
+  c.fz.alloc(std::max(1, layers));
+  if (layers) CWX_HIP(hipMemcpyAsync(c.fz.p, z_layers, sizeof(float) * layers, hipMemcpyHostToDevice, s));
+  c.fwb.alloc(layers + 1);
+  c.fwe.alloc(layers + 1);
+  CWX_HIP(hipMemcpyAsync(c.fwb.p, wb, sizeof(int) * (layers + 1), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(c.fwe.p, we, sizeof(int) * (layers + 1), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipStreamSynchronize(s));
+  c.table = cwx::FieldTable{c.ftab.p, prec, (long)nwaves, c.fz.p, c.fwb.p, c.fwe.p, layers};
+  return CWX_OK;
+}
+
+int cwx_field_init_f32(cwx_context* ctx, const cwx_wave_f32* W, int64_t nwaves, const float* z_layers, int layers,
+                       const int* waves_begin, const int* waves_end) {
+  return guard([&]() -> int {
+    return field_init(ctx, W, sizeof(cwx_wave_f32), 4, nwaves, z_layers, layers, waves_begin, waves_end);
+  });
+}
+
+int cwx_field_init_f64(cwx_context* ctx, const cwx_wave_f64* W, int64_t nwaves, const float* z_layers, int layers,
+                       const int* waves_begin, const int* waves_end) {
+  return guard([&]() -> int {
+    return field_init(ctx, W, sizeof(cwx_wave_f64), 8, nwaves, z_layers, layers, waves_begin, waves_end);
+  });
+}
+
+int cwx_field_init_solution(cwx_context* ctx, const cwx_solution* sol) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && sol, "NULL argument");
+    const auto& s = sol->s;
+    CWX_CHECK(s.nrec > 0, "solution has no records");
+    set_device(ctx);
+    auto& c = ctx->c;
+    const int nb = s.st.L + 1;
+    std::vector<float> zl(nb);
+    std::vector<int> wb(nb + 1), we(nb + 1);
+    long t = s.seg_cnt[0];
+    wb[0] = 0;
+    for (int l = 0; l < nb; ++l) {
+      zl[l] = (float)((s.opt.external && l > 0 && l < nb - 1) ? 0.0 : s.st.z[l]);
+      t += s.seg_cnt[1 + 2 * l];
+      we[l] = (int)t;
+      wb[l + 1] = (int)t;
+      t += s.seg_cnt[2 + 2 * l];
+    }
+    we[nb] = (int)t;
+    hipStream_t st = c.stream;
+    c.ftab.release();
+    c.fz.alloc(nb);
+    c.fwb.alloc(nb + 1);
+    c.fwe.alloc(nb + 1);
+    CWX_HIP(hipMemcpyAsync(c.fz.p, zl.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st));
+    CWX_HIP(hipMemcpyAsync(c.fwb.p, wb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    CWX_HIP(hipMemcpyAsync(c.fwe.p, we.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    CWX_HIP(hipStreamSynchronize(st));
+    c.table = cwx::FieldTable{s.rec.p, 8, s.nrec, c.fz.p, c.fwb.p, c.fwe.p, nb};
+    return CWX_OK;
+  });
+}
+
+int cwx_field_grid_dev(cwx_context* ctx, void* d_out, int out_f64, int tol_mode, float x0, float y0, float z0,
+                       float dx, float dy, float dz, int nx, int ny, int nz) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && d_out, "NULL argument");
+    if (ctx->c.table.nwaves <= 0) return fail(CWX_ERR_STATE, "no wave table: call cwx_field_init_* first");
+    CWX_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && nz <= 65535, "bad grid dimensions");
+    CWX_CHECK(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+    set_device(ctx);
+    cwx::GridSpec g{x0, y0, z0, dx, dy, dz, nx, ny, nz};
+    cwx::launch_field_direct(ctx->c.table, g, tol_mode, out_f64, d_out, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_field_evaluate(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz, float x_start,
+                       float y_start, float z_start, float x, float y, float z, float d, int N) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && E_xy && E_xz && E_yz, "NULL argument");
+    CWX_CHECK(N >= 1 && N <= 65535, "bad slice resolution");
+    if (ctx->c.table.nwaves <= 0) return fail(CWX_ERR_STATE, "no wave table: call cwx_field_init_* first");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    const size_t bytes = sizeof(cwx_complexf) * 3 * (size_t)N * N;
+    cwx::DevBuf<unsigned char> dE(bytes);
+    // kernel_xy: plane z, rows along y (gpuEvaluator.cu:70-89)
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y_start, z, d, d, 0.f, N, N, 1}, 0, 0, dE.p, s);
+    CWX_HIP(hipMemcpyAsync(E_xy, dE.p, bytes, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    // kernel_xz: plane y, rows along z
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y, z_start, d, 0.f, d, N, 1, N}, 0, 0, dE.p, s);
+    CWX_HIP(hipMemcpyAsync(E_xz, dE.p, bytes, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    // kernel_yz: plane x, columns along y, rows along z
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x, y_start, z_start, 0.f, d, d, 1, N, N}, 0, 0, dE.p, s);
+    CWX_HIP(hipMemcpyAsync(E_yz, dE.p, bytes, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    return CWX_OK;
+  });
+}
+
+}  // extern "C"

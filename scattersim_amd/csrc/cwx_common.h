@@ -1,0 +1,115 @@
+// Shared definitions for the cwx HIP library: complex arithmetic, error handling, launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace cwx {
+
+// Complex double, binary compatible with std::complex<double> and the C-ABI cwx_complex.
+struct zc {
+  double re, im;
+};
+
+__host__ __device__ inline zc mk(double r, double i = 0.0) { return zc{r, i}; }
+__host__ __device__ inline zc operator+(zc a, zc b) { return zc{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline zc operator-(zc a, zc b) { return zc{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline zc operator-(zc a) { return zc{-a.re, -a.im}; }
+__host__ __device__ inline zc operator*(zc a, zc b) {
+  return zc{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline zc operator*(double s, zc a) { return zc{s * a.re, s * a.im}; }
+__host__ __device__ inline zc operator*(zc a, double s) { return zc{s * a.re, s * a.im}; }
+__host__ __device__ inline zc& operator+=(zc& a, zc b) { a.re += b.re; a.im += b.im; return a; }
+__host__ __device__ inline zc& operator-=(zc& a, zc b) { a.re -= b.re; a.im -= b.im; return a; }
+__host__ __device__ inline zc conj(zc a) { return zc{a.re, -a.im}; }
+__host__ __device__ inline double abs1(zc a) { return fabs(a.re) + fabs(a.im); }
+__host__ __device__ inline double absz(zc a) { return hypot(a.re, a.im); }
+__host__ __device__ inline double norm2(zc a) { return a.re * a.re + a.im * a.im; }
+// a += b * c
+__host__ __device__ inline void fma_acc(zc& a, zc b, zc c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
+}
+// Smith's algorithm for a / b.
+__host__ __device__ inline zc zdiv(zc a, zc b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    if (b.re == 0.0 && b.im == 0.0) return zc{a.re / b.re, a.im / b.re};
+    double r = b.im / b.re, den = b.re + b.im * r;
+    return zc{(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  }
+  double r = b.re / b.im, den = b.im + b.re * r;
+  return zc{(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+}
+__host__ __device__ inline zc operator/(zc a, zc b) { return zdiv(a, b); }
+// Principal square root (branch cut on the negative real axis, sign of a zero imaginary part kept).
+__host__ __device__ inline zc zsqrt(zc a) {
+  if (a.re == 0.0 && a.im == 0.0) return zc{0.0, a.im};
+  double m = hypot(a.re, a.im);
+  double t = sqrt(0.5 * (m + fabs(a.re)));
+  if (a.re >= 0.0) return zc{t, 0.5 * a.im / t};
+  return zc{0.5 * fabs(a.im) / t, copysign(t, a.im)};
+}
+__host__ __device__ inline zc zexp(zc a) {
+  double e = exp(a.re);
+  double s, c;
+  sincos(a.im, &s, &c);
+  return zc{e * c, e * s};
+}
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CWX_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (call);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      throw ::cwx::Error(-2, std::string(#call " failed: ") + hipGetErrorString(_e) + " at " + \
+                                 __FILE__ + ":" + std::to_string(__LINE__));                   \
+  } while (0)
+
+#define CWX_CHECK(cond, msg)                                  \
+  do {                                                        \
+    if (!(cond)) throw ::cwx::Error(-1, std::string(msg));    \
+  } while (0)
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Device buffer owned by the library (freed on destruction).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) CWX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) { if (count > n) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace cwx

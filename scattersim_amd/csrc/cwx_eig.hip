@@ -1,0 +1,603 @@
+// Batched non-symmetric complex eigensolver with the LAPACK zgeev('N','V') contract.
+//
+// Replaces MKL_eigensolve (third_Lapack.h:38-45, LAPACKE_zgeev with jobvl='N', jobvr='V'), which the
+// reference calls once per layer (ScatterVolume.cpp:255-262).  Steps, one workgroup per matrix:
+//   1. Householder reduction to upper Hessenberg form          (zgehd2 algorithm)
+//   2. explicit formation of the unitary factor Q                (zung2r algorithm)
+//   3. complex single-shift QR with Wilkinson / exceptional shifts and the Ahues-Kahan deflation test
+//      (zlahqr algorithm) giving the Schur form T and Schur vectors Z = Q * (QR rotations).
+//      The bulge is chased through a (W+2) x (W+3) window of H held in LDS; the W rotations of a
+//      window are then applied to the rest of H and to Z from registers, one matrix line per lane.
+//   4. eigenvectors of T by back-substitution (ztrevc algorithm), one wavefront per eigenvector
+//   5. V = Z X on the GEMM kernel, then zgeev's normalisation (unit 2-norm, largest component real).
+#include "cwx_common.h"
+#include "cwx_kernels.h"
+
+namespace cwx {
+
+namespace {
+
+constexpr int EIG_THREADS = 256;
+constexpr int QR_W = 32;  // rotations per window
+constexpr double kUlp = 2.220446049250313e-16;      // dlamch('P')
+constexpr double kSafmin = 2.2250738585072014e-308;  // dlamch('S')
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// block-wide sum of a complex value; every lane receives the result
+__device__ inline zc block_sum(zc v, zc* red) {
+  v.re = wave_sum(v.re);
+  v.im = wave_sum(v.im);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  zc s = mk(0.0);
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+__device__ inline double dlapy3(double x, double y, double z) {
+  double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+  if (w == 0.0) return fabs(x) + fabs(y) + fabs(z);
+  return w * sqrt((x / w) * (x / w) + (y / w) * (y / w) + (z / w) * (z / w));
+}
+
+// ------------------------------------------------------------------------------------------------
+// 1. Hessenberg reduction (zgehd2).  A(j+1:n, j) -> beta e1, v stored below, tau[j].
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, long sA, zc* tau) {
+  extern __shared__ zc smem[];
+  zc* v = smem;           // n
+  zc* w = smem + n;       // n
+  __shared__ zc red[EIG_THREADS / 64];
+  __shared__ zc s_tau;
+  const int b = blockIdx.x;
+  A += sA * b;
+  tau += (long)n * b;
+  const int tid = threadIdx.x;
+  for (int j = 0; j < n - 2; ++j) {
+    const int m = n - j - 1;  // length of x = A(j+1:n, j)
+    zc* x = A + (long)j * lda + j + 1;
+    // ||x(1:)||^2
+    double ss = 0.0;
+    for (int r = 1 + tid; r < m; r += blockDim.x) ss += norm2(x[r]);
+    zc tot = block_sum(mk(ss), red);
+    const double xnorm = sqrt(tot.re);
+    const zc alpha = x[0];
+    zc t = mk(0.0);
+    double beta = 0.0;
+    if (!(xnorm == 0.0 && alpha.im == 0.0)) {
+      beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
+      t = mk((beta - alpha.re) / beta, -alpha.im / beta);
+    }
+    const bool trivial = (t.re == 0.0 && t.im == 0.0);
+    if (!trivial) {
+      const zc scal = zdiv(mk(1.0), alpha - mk(beta));
+      for (int r = 1 + tid; r < m; r += blockDim.x) x[r] = x[r] * scal;
+    }
+    __syncthreads();
+    // v = [1; x(1:)] into LDS
+    for (int r = tid; r < m; r += blockDim.x) v[r] = (r == 0) ? mk(1.0) : x[r];
+    if (tid == 0) {
+      tau[j] = t;
+      if (!trivial) x[0] = mk(beta);
+    }
+    __syncthreads();
+    if (!trivial) {
+      // right: A(0:n, j+1:n) -= t (A v) v^H
+      for (int r = tid; r < n; r += blockDim.x) {
+        zc acc = mk(0.0);
+        for (int c = 0; c < m; ++c) fma_acc(acc, A[(long)(j + 1 + c) * lda + r], v[c]);
+        w[r] = acc * t;
+      }
+      __syncthreads();
+      for (int r = tid; r < n; r += blockDim.x) {
+        const zc wr = w[r];
+        for (int c = 0; c < m; ++c) {
+          zc& a = A[(long)(j + 1 + c) * lda + r];
+          a = a - wr * conj(v[c]);
+        }
+      }
+      __syncthreads();
+      // left: A(j+1:n, j+1:n) -= conj(t) v (v^H A); wavefront per column, lanes down the rows
+      const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+      for (int c = wid; c < m; c += nw) {
+        const zc* col = A + (long)(j + 1 + c) * lda + j + 1;
+        zc acc = mk(0.0);
+        for (int r = lane; r < m; r += 64) fma_acc(acc, conj(v[r]), col[r]);
+        acc.re = wave_sum(acc.re);
+        acc.im = wave_sum(acc.im);
+        if (lane == 0) w[c] = acc * conj(t);
+      }
+      __syncthreads();
+      for (int r = tid; r < m; r += blockDim.x) {
+        const zc vr = v[r];
+        for (int c = 0; c < m; ++c) {
+          zc& a = A[(long)(j + 1 + c) * lda + j + 1 + r];
+          a = a - vr * w[c];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 2. Q = H(0) H(1) ... H(n-3) applied to the identity (zung2r on the trailing block), then the
+//    Householder vectors below the subdiagonal of A are cleared so that A holds H.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(EIG_THREADS) void k_unghr(int n, zc* A, int lda, long sA, const zc* tau,
+                                                       zc* Z, int ldz, long sZ) {
+  extern __shared__ zc smem[];
+  zc* v = smem;      // n
+  zc* w = smem + n;  // n
+  const int b = blockIdx.x;
+  A += sA * b;
+  tau += (long)n * b;
+  Z += sZ * b;
+  const int tid = threadIdx.x;
+  for (long e = tid; e < (long)n * n; e += blockDim.x) {
+    int r = (int)(e % n), c = (int)(e / n);
+    Z[(long)c * ldz + r] = mk(r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  for (int j = n - 3; j >= 0; --j) {
+    const zc t = tau[j];
+    if (t.re == 0.0 && t.im == 0.0) continue;
+    const int m = n - j - 1;
+    for (int r = tid; r < m; r += blockDim.x) v[r] = (r == 0) ? mk(1.0) : A[(long)j * lda + j + 1 + r];
+    __syncthreads();
+    // Z(j+1:n, j+1:n) = (I - t v v^H) Z(j+1:n, j+1:n)
+    for (int c = wid; c < m; c += nw) {
+      const zc* col = Z + (long)(j + 1 + c) * ldz + j + 1;
+      zc acc = mk(0.0);
+      for (int r = lane; r < m; r += 64) fma_acc(acc, conj(v[r]), col[r]);
+      acc.re = wave_sum(acc.re);
+      acc.im = wave_sum(acc.im);
+      if (lane == 0) w[c] = acc * t;
+    }
+    __syncthreads();
+    for (int r = tid; r < m; r += blockDim.x) {
+      const zc vr = v[r];
+      for (int c = 0; c < m; ++c) {
+        zc& z = Z[(long)(j + 1 + c) * ldz + j + 1 + r];
+        z = z - vr * w[c];
+      }
+    }
+    __syncthreads();
+  }
+  for (long e = tid; e < (long)n * n; e += blockDim.x) {
+    int r = (int)(e % n), c = (int)(e / n);
+    if (r > c + 1) A[(long)c * lda + r] = mk(0.0);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 3. Schur form by single-shift QR (zlahqr), windowed bulge chase.
+// ------------------------------------------------------------------------------------------------
+struct Rot {
+  zc t1;
+  zc v2;
+  double t2;
+};
+
+// 2-element Householder (zlarfg with n = 2): returns (t1, v2, t2 = Re(t1 v2)) and beta in alpha.
+__device__ inline Rot make_reflector(zc& alpha, zc x) {
+  Rot g;
+  const double xnorm = absz(x);
+  if (xnorm == 0.0 && alpha.im == 0.0) {
+    g.t1 = mk(0.0);
+    g.v2 = x;
+    g.t2 = 0.0;
+    return g;
+  }
+  const double beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
+  g.t1 = mk((beta - alpha.re) / beta, -alpha.im / beta);
+  g.v2 = x * zdiv(mk(1.0), alpha - mk(beta));
+  alpha = mk(beta);
+  g.t2 = (g.t1 * g.v2).re;
+  return g;
+}
+
+__device__ inline void rot_left(const Rot& g, zc& a, zc& b) {
+  // rows k, k+1:  sum = conj(t1) a + t2 b; a -= sum; b -= sum v2
+  zc s = conj(g.t1) * a + g.t2 * b;
+  a = a - s;
+  b = b - s * g.v2;
+}
+__device__ inline void rot_right(const Rot& g, zc& a, zc& b) {
+  // cols k, k+1:  sum = t1 a + t2 b; a -= sum; b -= sum conj(v2)
+  zc s = g.t1 * a + g.t2 * b;
+  a = a - s;
+  b = b - s * conj(g.v2);
+}
+
+__device__ inline bool subdiag_negligible(const zc* H, int ldh, int k, int ilo, int ihi, double smlnum) {
+  auto h = [&](int r, int c) { return H[(long)c * ldh + r]; };
+  const zc hkk1 = h(k, k - 1);
+  if (abs1(hkk1) <= smlnum) return true;
+  double tst = abs1(h(k - 1, k - 1)) + abs1(h(k, k));
+  if (tst == 0.0) {
+    if (k - 2 >= ilo) tst += fabs(h(k - 1, k - 2).re);
+    if (k + 1 <= ihi) tst += fabs(h(k + 1, k).re);
+  }
+  if (fabs(hkk1.re) <= kUlp * tst) {
+    const double ab = fmax(abs1(hkk1), abs1(h(k - 1, k)));
+    const double ba = fmin(abs1(hkk1), abs1(h(k - 1, k)));
+    const double aa = fmax(abs1(h(k, k)), abs1(h(k - 1, k - 1) - h(k, k)));
+    const double bb = fmin(abs1(h(k, k)), abs1(h(k - 1, k - 1) - h(k, k)));
+    const double s = aa + ab;
+    if (ba * (ab / s) <= fmax(smlnum, kUlp * (bb * (aa / s)))) return true;
+  }
+  return false;
+}
+
+__device__ inline bool two_small(const zc* H, int ldh, int m, zc t) {
+  auto h = [&](int r, int c) { return H[(long)c * ldh + r]; };
+  const zc h11 = h(m, m), h22 = h(m + 1, m + 1);
+  zc h11s = h11 - t;
+  double h21 = h(m + 1, m).re;
+  const double s = abs1(h11s) + fabs(h21);
+  h11s = h11s * (1.0 / s);
+  h21 = h21 / s;
+  const double h10 = h(m, m - 1).re;
+  return fabs(h10) * fabs(h21) <= kUlp * (abs1(h11s) * (abs1(h11) + abs1(h22)));
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
+                                                     long sZ, zc* W, long sW, int* info) {
+  constexpr int BR = QR_W + 2, BC = QR_W + 3;
+  __shared__ zc B[BR * BC];   // window block, column-major with leading dimension BR
+  __shared__ Rot rots[QR_W];
+  __shared__ int s_k, s_m;
+  __shared__ zc s_t, s_v1;
+  __shared__ double s_v2;
+  const int b = blockIdx.x;
+  H += sH * b;
+  Z += sZ * b;
+  W += sW * b;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  auto Hp = [&](int r, int c) -> zc& { return H[(long)c * ldh + r]; };
+  auto Zp = [&](int r, int c) -> zc& { return Z[(long)c * ldz + r]; };
+  if (tid == 0) info[b] = 0;
+  if (n == 1) {
+    if (tid == 0) W[0] = Hp(0, 0);
+    return;
+  }
+  const int ilo = 0, ihi = n - 1;
+  const double smlnum = kSafmin * ((double)n / kUlp);
+
+  // make the subdiagonal real (sequential in i: each scaling feeds the next test)
+  for (int i = ilo + 1; i <= ihi; ++i) {
+    __syncthreads();
+    const zc hi = Hp(i, i - 1);
+    if (hi.im != 0.0) {
+      zc sc = hi * (1.0 / abs1(hi));
+      sc = conj(sc) * (1.0 / absz(sc));
+      const double ah = absz(hi);
+      __syncthreads();
+      // row i, cols i..n-1 ; col i, rows 0..min(n-1, i+1) ; Z col i
+      for (int c = i + tid; c < n; c += nt) Hp(i, c) = Hp(i, c) * sc;
+      __syncthreads();
+      for (int r = tid; r <= min(n - 1, i + 1); r += nt) Hp(r, i) = Hp(r, i) * conj(sc);
+      for (int r = tid; r < n; r += nt) Zp(r, i) = Zp(r, i) * conj(sc);
+      __syncthreads();
+      if (tid == 0) Hp(i, i - 1) = mk(ah);
+    }
+  }
+  __syncthreads();
+
+  const int itmax = 30 * max(10, ihi - ilo + 1);
+  int kdefl = 0;
+  int I = ihi;
+  while (I >= ilo) {
+    int L = ilo;
+    bool converged = false;
+    for (int its = 0; its <= itmax; ++its) {
+      // ---- look for a single small subdiagonal element (largest k in [L+1, I]) ----
+      if (tid == 0) s_k = L;
+      __syncthreads();
+      int found = L;
+      for (int k = L + 1 + tid; k <= I; k += nt)
+        if (subdiag_negligible(H, ldh, k, ilo, ihi, smlnum)) found = max(found, k);
+      if (found > L) atomicMax(&s_k, found);
+      __syncthreads();
+      L = s_k;
+      if (L > ilo && tid == 0) Hp(L, L - 1) = mk(0.0);
+      __syncthreads();
+      if (L >= I) { converged = true; break; }
+      ++kdefl;
+      // ---- shift ----
+      if (tid == 0) {
+        zc t;
+        if (kdefl % 20 == 0) {
+          t = mk(0.75 * fabs(Hp(I, I - 1).re)) + Hp(I, I);
+        } else if (kdefl % 10 == 0) {
+          t = mk(0.75 * fabs(Hp(L + 1, L).re)) + Hp(L, L);
+        } else {
+          t = Hp(I, I);
+          zc u = zsqrt(Hp(I - 1, I)) * zsqrt(Hp(I, I - 1));
+          double s = abs1(u);
+          if (s != 0.0) {
+            zc x = 0.5 * (Hp(I - 1, I - 1) - t);
+            const double sx = abs1(x);
+            s = fmax(s, abs1(x));
+            zc xs = x * (1.0 / s), us = u * (1.0 / s);
+            zc y = s * zsqrt(xs * xs + us * us);
+            if (sx > 0.0) {
+              zc xn = x * (1.0 / sx);
+              if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
+            }
+            t = t - u * zdiv(u, x + y);
+          }
+        }
+        s_t = t;
+        s_m = L;
+      }
+      __syncthreads();
+      const zc tshift = s_t;
+      // ---- look for two consecutive small subdiagonal elements (largest m in [L+1, I-1]) ----
+      int mf = L;
+      for (int m = L + 1 + tid; m <= I - 1; m += nt)
+        if (two_small(H, ldh, m, tshift)) mf = max(mf, m);
+      if (mf > L) atomicMax(&s_m, mf);
+      __syncthreads();
+      const int M = s_m;
+      if (tid == 0) {
+        const zc h11 = Hp(M, M);
+        zc h11s = h11 - tshift;
+        double h21 = Hp(M + 1, M).re;
+        const double s = abs1(h11s) + fabs(h21);
+        s_v1 = h11s * (1.0 / s);
+        s_v2 = h21 / s;
+      }
+      __syncthreads();
+
+      // ---- single-shift QR sweep k = M .. I-1, in windows ----
+      int ks = M;
+      while (ks < I) {
+        // the first rotation of a sweep started at M > L is followed by a diagonal scaling;
+        // it gets a window of its own
+        const int wlen = (ks == M && M > L) ? 1 : min(QR_W, I - ks);
+        const int ke = ks + wlen;                     // rotations ks .. ke-1
+        const int re = min(ke + 1, I);                // last block row / column
+        const int cs = (ks > M) ? ks - 1 : ks;        // first block column
+        const int nr = re - ks + 1, nc = re - cs + 1;
+        for (int e = tid; e < nr * nc; e += nt) {
+          int r = e % nr, c = e / nr;
+          B[c * BR + r] = Hp(ks + r, cs + c);
+        }
+        __syncthreads();
+        auto Bp = [&](int r, int c) -> zc& { return B[(c - cs) * BR + (r - ks)]; };
+        for (int k = ks; k < ke; ++k) {
+          zc v1;
+          zc v2c;
+          if (k == M) {
+            v1 = s_v1;
+            v2c = mk(s_v2);
+          } else {
+            v1 = Bp(k, k - 1);
+            v2c = Bp(k + 1, k - 1);
+          }
+          // every lane forms the same reflector from the same LDS values; column k-1 is rewritten
+          // only in the right phase below, after the barrier
+          const Rot g = make_reflector(v1, v2c);
+          // left: rows k, k+1, cols k..re
+          for (int c = k + tid; c <= re; c += nt) rot_left(g, Bp(k, c), Bp(k + 1, c));
+          __syncthreads();
+          // right: cols k, k+1, rows ks..min(k+2, I)
+          const int rmax = min(k + 2, I);
+          for (int r = ks + tid; r <= rmax; r += nt) rot_right(g, Bp(r, k), Bp(r, k + 1));
+          if (tid == 0) {
+            if (k > M) {
+              Bp(k, k - 1) = v1;
+              Bp(k + 1, k - 1) = mk(0.0);
+            }
+            rots[k - ks] = g;
+          }
+          __syncthreads();
+        }
+        for (int e = tid; e < nr * nc; e += nt) {
+          int r = e % nr, c = e / nr;
+          Hp(ks + r, cs + c) = B[c * BR + r];
+        }
+        // deferred: far-right columns (re, n) on rows ks..ke; far-above rows [0, ks) on cols ks..ke;
+        // Schur vectors Z(:, ks..ke)
+        const int nfr = n - 1 - re, nfa = ks, nz = n;
+        const int lines = nfr + nfa + nz;
+        for (int ln = tid; ln < lines; ln += nt) {
+          zc x[QR_W + 1];
+          if (ln < nfr) {
+            const int c = re + 1 + ln;
+#pragma unroll
+            for (int j = 0; j <= QR_W; ++j)
+              if (j <= wlen) x[j] = Hp(ks + j, c);
+#pragma unroll
+            for (int j = 0; j < QR_W; ++j)
+              if (j < wlen) rot_left(rots[j], x[j], x[j + 1]);
+#pragma unroll
+            for (int j = 0; j <= QR_W; ++j)
+              if (j <= wlen) Hp(ks + j, c) = x[j];
+          } else {
+            const bool isz = ln >= nfr + nfa;
+            const int r = isz ? ln - nfr - nfa : ln - nfr;
+            zc* base = isz ? (Z + (long)ks * ldz + r) : (H + (long)ks * ldh + r);
+            const int ld = isz ? ldz : ldh;
+#pragma unroll
+            for (int j = 0; j <= QR_W; ++j)
+              if (j <= wlen) x[j] = base[(long)j * ld];
+#pragma unroll
+            for (int j = 0; j < QR_W; ++j)
+              if (j < wlen) rot_right(rots[j], x[j], x[j + 1]);
+#pragma unroll
+            for (int j = 0; j <= QR_W; ++j)
+              if (j <= wlen) base[(long)j * ld] = x[j];
+          }
+        }
+        __syncthreads();
+        if (ks == M && M > L) {
+          // extra scaling keeping H(M, M-1) real (zlahqr)
+          const Rot g = rots[0];
+          zc temp = mk(1.0) - g.t1;
+          temp = temp * (1.0 / absz(temp));
+          if (tid == 0) {
+            Hp(M + 1, M) = Hp(M + 1, M) * conj(temp);
+            if (M + 2 <= I) Hp(M + 2, M + 1) = Hp(M + 2, M + 1) * temp;
+          }
+          __syncthreads();
+          for (int J = M; J <= I; ++J) {
+            if (J == M + 1) continue;
+            for (int c = J + 1 + tid; c < n; c += nt) Hp(J, c) = Hp(J, c) * temp;
+            __syncthreads();
+            for (int r = tid; r < J; r += nt) Hp(r, J) = Hp(r, J) * conj(temp);
+            for (int r = tid; r < n; r += nt) Zp(r, J) = Zp(r, J) * conj(temp);
+            __syncthreads();
+          }
+        }
+        ks = ke;
+      }
+      // ---- ensure H(I, I-1) is real ----
+      const zc tmp = Hp(I, I - 1);
+      __syncthreads();
+      if (tmp.im != 0.0) {
+        const double rt = absz(tmp);
+        const zc tt = tmp * (1.0 / rt);
+        for (int c = I + 1 + tid; c < n; c += nt) Hp(I, c) = Hp(I, c) * conj(tt);
+        for (int r = tid; r < I; r += nt) Hp(r, I) = Hp(r, I) * tt;
+        for (int r = tid; r < n; r += nt) Zp(r, I) = Zp(r, I) * tt;
+        __syncthreads();
+        if (tid == 0) Hp(I, I - 1) = mk(rt);
+      }
+      __syncthreads();
+    }
+    if (!converged) {
+      if (tid == 0) info[b] = I + 1;
+      return;
+    }
+    if (tid == 0) W[I] = Hp(I, I);
+    kdefl = 0;
+    I = L - 1;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 4. Eigenvectors of the upper triangular T (ztrevc, side 'R'): column ki of X solves
+//    (T(0:ki,0:ki) - T(ki,ki)) x = -T(0:ki, ki), x(ki) = 1, x(ki+1:) = 0.  One wavefront per ki.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trevc(int n, const zc* T, int ldt, long sT, zc* X, int ldx,
+                                               long sX, double smlnum) {
+  extern __shared__ zc xs[];  // (blockDim.x / 64) wavefronts x n
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int ki = blockIdx.x * (blockDim.x >> 6) + wid;
+  const int b = blockIdx.y;
+  T += sT * b;
+  X += sX * b;
+  if (ki >= n) return;
+  zc* x = xs + wid * n;
+  const zc lam = T[(long)ki * ldt + ki];
+  const double smin = fmax(kUlp * abs1(lam), smlnum);
+  for (int r = lane; r < ki; r += 64) x[r] = -T[(long)ki * ldt + r];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int j = ki - 1; j >= 0; --j) {
+    zc d = T[(long)j * ldt + j] - lam;
+    if (abs1(d) < smin) d = mk(smin);
+    const zc xj = zdiv(x[j], d);
+    // every lane computed xj identically from the same LDS value; rows 0..j-1 updated in parallel
+    const zc* tcol = T + (long)j * ldt;
+    for (int r = lane; r < j; r += 64) x[r] = x[r] - xj * tcol[r];
+    if (lane == 0) x[j] = xj;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  zc* xc = X + (long)ki * ldx;
+  for (int r = lane; r < n; r += 64) xc[r] = (r < ki) ? x[r] : mk(r == ki ? 1.0 : 0.0);
+}
+
+// ------------------------------------------------------------------------------------------------
+// 5. zgeev normalisation: unit 2-norm, then rotate so that the largest |v_k|^2 is real.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_normalize(int n, zc* V, int ldv, long sV) {
+  __shared__ zc red[4];
+  __shared__ double sv[4];
+  __shared__ int si[4];
+  const int col = blockIdx.x, b = blockIdx.y;
+  zc* v = V + sV * b + (long)col * ldv;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double ss = 0.0;
+  for (int r = tid; r < n; r += blockDim.x) ss += norm2(v[r]);
+  const double nrm = sqrt(block_sum(mk(ss), red).re);
+  const double scl = 1.0 / nrm;
+  double best = -1.0;
+  int bi = n;
+  for (int r = tid; r < n; r += blockDim.x) {
+    zc e = v[r] * scl;
+    double m2 = e.re * e.re + e.im * e.im;
+    if (m2 > best) { best = m2; bi = r; }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(best, off);
+    int oi = __shfl_xor(bi, off);
+    if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+  }
+  if (lane == 0) { sv[wid] = best; si[wid] = bi; }
+  __syncthreads();
+  double bv = sv[0];
+  int k = si[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+    if (sv[w] > bv || (sv[w] == bv && si[w] < k)) { bv = sv[w]; k = si[w]; }
+  const zc vk = v[k] * scl;
+  const zc rot = conj(vk) * (1.0 / sqrt(bv));
+  __syncthreads();
+  for (int r = tid; r < n; r += blockDim.x) {
+    zc e = (v[r] * scl) * rot;
+    if (r == k) e.im = 0.0;
+    v[r] = e;
+  }
+}
+
+}  // namespace
+
+void EigWork::ensure(int n, int batch) {
+  const size_t nn = (size_t)n * n * batch;
+  Z.ensure(nn);
+  X.ensure(nn);
+  tau.ensure((size_t)n * batch);
+  info.ensure((size_t)batch);
+}
+
+void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
+                   EigWork& work, int* info, hipStream_t s) {
+  if (n <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "zgeev: batch too large");
+  work.ensure(n, batch);
+  const long nn = (long)n * n;
+  const size_t sh2 = sizeof(zc) * 2 * (size_t)n;
+  CWX_CHECK(sh2 <= 64 * 1024, "zgeev: n too large for the LDS work vectors");
+  hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(EIG_THREADS), sh2, s, n, A, lda, sA, work.tau.p);
+  hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(EIG_THREADS), sh2, s, n, A, lda, sA, work.tau.p,
+                     work.Z.p, n, nn);
+  hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n, nn, W,
+                     sW, info);
+  const double smlnum = kSafmin * ((double)n / kUlp);
+  // one wavefront per eigenvector, as many per workgroup as fit 64 KiB of LDS
+  const int waves = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / (sizeof(zc) * n)));
+  CWX_CHECK(sizeof(zc) * (size_t)n <= 64 * 1024, "zgeev: n too large for the LDS eigenvector buffer");
+  hipLaunchKernelGGL(k_trevc, dim3(cdiv(n, waves), batch), dim3(64 * waves),
+                     sizeof(zc) * waves * (size_t)n, s, n, A, lda, sA, work.X.p, n, nn, smlnum);
+  zgemm_batched(OpN, OpN, n, n, n, mk(1.0), work.Z.p, n, nn, work.X.p, n, nn, mk(0.0), V, ldv, sV,
+                batch, s);
+  hipLaunchKernelGGL(k_normalize, dim3(n, batch), dim3(256), 0, s, n, V, ldv, sV);
+  CWX_HIP(hipGetLastError());
+}
+
+}  // namespace cwx

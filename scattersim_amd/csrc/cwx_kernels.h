@@ -1,0 +1,72 @@
+// Internal launch interface of the cwx HIP kernels (not part of the C ABI).
+#pragma once
+
+#include "cwx_common.h"
+
+namespace cwx {
+
+// ---------------------------------------------------------------------------------------------
+// field evaluation
+// ---------------------------------------------------------------------------------------------
+struct GridSpec {
+  float x0, y0, z0;   // coordinates of grid point (0, 0, 0)
+  float dx, dy, dz;   // steps; coordinate i is x0 + dx * (float)i as in gpuEvaluator.cu:78-79
+  int nx, ny, nz;     // output is [nz][ny][nx][3]
+};
+
+struct FieldTable {
+  const void* waves = nullptr;  // nwaves records {E0[3], k[3]}, complex<float> (48 B) or complex<double> (96 B)
+  int precision = 0;            // 4 or 8
+  long nwaves = 0;
+  const float* z_layers = nullptr;  // device, layers
+  const int* wb = nullptr;          // device, layers + 1
+  const int* we = nullptr;
+  int layers = 0;
+};
+
+void launch_field_direct(const FieldTable& t, const GridSpec& g, int tol_mode, int out_f64, void* out,
+                         hipStream_t s);
+
+// Separable evaluation of a solved structure (one homogeneous region per z): see cwx_fieldsep.hip.
+struct SepRegion {
+  // E(x,y,z) = sum_{a<nkx, b<nky} X[a](x) Y[b](y) C[a][b](z), C = A (nkx*nky*3 x nmodes) . e^{i kz z}
+  int nmodes;
+};
+
+// ---------------------------------------------------------------------------------------------
+// dense complex linear algebra (column-major, batched with element strides)
+// ---------------------------------------------------------------------------------------------
+enum Op { OpN = 0, OpT = 1, OpC = 2 };
+
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
+void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const zc* A, int lda, long sA,
+                   const zc* B, int ldb, long sB, zc beta, zc* C, int ldc, long sC, int batch,
+                   hipStream_t s);
+
+// LU with partial pivoting, in place (LAPACK zgetrf contract, 0-based pivots).  info[b] = 0 or
+// the 1-based index of the first exactly-zero pivot.
+void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, int* info, int batch,
+                    hipStream_t s);
+
+// Solve A X = B with the factors of zgetrf (LAPACK zgetrs, no transpose).
+void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,
+                    int ldb, long sB, int batch, hipStream_t s);
+
+// ---------------------------------------------------------------------------------------------
+// non-symmetric eigensolver (LAPACK zgeev('N','V') contract)
+// ---------------------------------------------------------------------------------------------
+struct EigWork {
+  DevBuf<zc> Z;      // n*n*batch Schur vectors
+  DevBuf<zc> X;      // n*n*batch triangular eigenvectors of T
+  DevBuf<zc> tau;    // n*batch
+  DevBuf<int> info;  // batch
+  void ensure(int n, int batch);
+};
+
+// A is destroyed (overwritten by the Schur form).  W receives eigenvalues, V the unit 2-norm right
+// eigenvectors with their largest component real.  info (device, batch) is 0 or the LAPACK-style
+// failure index.
+void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
+                   EigWork& work, int* info, hipStream_t s);
+
+}  // namespace cwx

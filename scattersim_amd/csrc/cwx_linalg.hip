@@ -1,0 +1,351 @@
+// Dense complex FP64 linear algebra for the coupled-wave solve: batched GEMM, blocked LU with
+// partial pivoting and the matching triangular solves.
+//
+// Replaces the LAPACK/BLAS seam of third_Lapack.h:47-85 (LAPACKE_zgesv, the zgesv-with-identity
+// inverse and cblas_zgemm).  Storage is column-major complex<double> with explicit leading
+// dimensions, as LAPACK; every entry point is batched over matrices placed at a fixed element stride.
+#include "cwx_common.h"
+#include "cwx_kernels.h"
+
+namespace cwx {
+
+namespace {
+
+// ------------------------------------------------------------------------------------------------
+// GEMM: 64x64 output tile per 256-lane workgroup, K staged through LDS in steps of 16, each lane
+// owning a 4x4 block of outputs at stride 16 (conflict-free broadcast reads from LDS).
+// ------------------------------------------------------------------------------------------------
+constexpr int GM = 64, GN = 64, GK = 16;
+
+__device__ inline zc load_op(const zc* __restrict__ A, int ld, Op op, int r, int c) {
+  // element (r, c) of op(A)
+  if (op == OpN) return A[(long)c * ld + r];
+  zc v = A[(long)r * ld + c];
+  return op == OpC ? conj(v) : v;
+}
+
+__global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int k, zc alpha,
+                                               const zc* __restrict__ A, int lda, long sA,
+                                               const zc* __restrict__ B, int ldb, long sB, zc beta,
+                                               zc* __restrict__ C, int ldc, long sC) {
+  __shared__ zc As[GK][GM];
+  __shared__ zc Bs[GK][GN];
+  const int b = blockIdx.z;
+  A += sA * b;
+  B += sB * b;
+  C += sC * b;
+  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+  const int t = threadIdx.x;
+  const int tx = t & 15, ty = t >> 4;
+  zc acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0);
+
+  for (int k0 = 0; k0 < k; k0 += GK) {
+    // A tile: GM x GK = 1024 elements, 4 per lane; lanes run along rows for op N (coalesced)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int e = t + 256 * u;
+      int r, kk;
+      if (opa == OpN) { r = e & (GM - 1); kk = e >> 6; }
+      else { kk = e & (GK - 1); r = e >> 4; }
+      int gr = m0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < m && gk < k) ? load_op(A, lda, opa, gr, gk) : mk(0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int e = t + 256 * u;
+      int kk, c;
+      if (opb == OpN) { kk = e & (GK - 1); c = e >> 4; }
+      else { c = e & (GN - 1); kk = e >> 6; }
+      int gk = k0 + kk, gc = n0 + c;
+      Bs[kk][c] = (gk < k && gc < n) ? load_op(B, ldb, opb, gk, gc) : mk(0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < GK; ++kk) {
+      zc a[4], bb[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fma_acc(acc[i][j], a[i], bb[j]);
+    }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int gc = n0 + ty + 16 * j;
+    if (gc >= n) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int gr = m0 + tx + 16 * i;
+      if (gr >= m) continue;
+      zc* cp = C + (long)gc * ldc + gr;
+      zc v = alpha * acc[i][j];
+      if (!beta0) v = v + beta * (*cp);
+      *cp = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// LU panel: one workgroup per matrix factors columns [j0, j0+nb) over rows [j0, n) with partial
+// pivoting (pivot = first maximum of |re|+|im|, as izamax).  Swaps are applied to the panel columns
+// only; the other columns are swapped by k_lu_swap_trsm.
+// ------------------------------------------------------------------------------------------------
+constexpr int LU_NB = 32;
+constexpr int LU_THREADS = 256;
+
+__device__ inline void block_argmax(double v, int idx, double* sv, int* si, double* outv, int* outi) {
+  // reduce (value, index) pairs: larger value wins, ties -> smaller index
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_xor(v, off);
+    int oi = __shfl_xor(idx, off);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  if (lane == 0) { sv[wid] = v; si[wid] = idx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = sv[0];
+    int bi = si[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) { bv = sv[w]; bi = si[w]; }
+    *outv = bv;
+    *outi = bi;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, zc* A, int lda,
+                                                         long sA, int* ipiv, long sP, int* info) {
+  __shared__ double sv[LU_THREADS / 64];
+  __shared__ int si[LU_THREADS / 64];
+  __shared__ double pv;
+  __shared__ int pi;
+  const int b = blockIdx.x;
+  A += sA * b;
+  ipiv += sP * b;
+  for (int jj = 0; jj < nb; ++jj) {
+    const int j = j0 + jj;
+    // pivot search over rows j..n-1 of column j
+    double best = -1.0;
+    int bidx = n;
+    for (int r = j + threadIdx.x; r < n; r += blockDim.x) {
+      double v = abs1(A[(long)j * lda + r]);
+      if (v > best) { best = v; bidx = r; }
+    }
+    block_argmax(best, bidx, sv, si, &pv, &pi);
+    const int p = pi;
+    const double pval = pv;
+    if (threadIdx.x == 0) {
+      ipiv[j] = p;
+      if (pval == 0.0 && info[b] == 0) info[b] = j + 1;
+    }
+    // swap rows j and p inside the panel columns
+    if (p != j) {
+      for (int c = j0 + threadIdx.x; c < j0 + nb; c += blockDim.x) {
+        zc t = A[(long)c * lda + j];
+        A[(long)c * lda + j] = A[(long)c * lda + p];
+        A[(long)c * lda + p] = t;
+      }
+    }
+    __syncthreads();
+    const zc piv = A[(long)j * lda + j];
+    if (pval != 0.0) {
+      const zc rinv = zdiv(mk(1.0), piv);
+      for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) A[(long)j * lda + r] = A[(long)j * lda + r] * rinv;
+    }
+    __syncthreads();
+    // rank-1 update of the remaining panel columns
+    const int ncols = j0 + nb - (j + 1);
+    if (ncols > 0) {
+      for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
+        const zc l = A[(long)j * lda + r];
+        for (int c = j + 1; c < j0 + nb; ++c) {
+          const zc u = A[(long)c * lda + j];
+          zc& a = A[(long)c * lda + r];
+          a = a - l * u;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
+// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.
+__global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, zc* A, int lda, long sA,
+                                                     const int* ipiv, long sP) {
+  __shared__ zc L11[LU_NB][LU_NB];
+  __shared__ int piv[LU_NB];
+  const int b = blockIdx.y;
+  A += sA * b;
+  ipiv += sP * b;
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    int r = e % nb, c = e / nb;
+    L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
+  }
+  for (int e = threadIdx.x; e < nb; e += blockDim.x) piv[e] = ipiv[j0 + e];
+  __syncthreads();
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  // columns [0, j0) and [j0+nb, n)
+  if (c >= j0) c += nb;
+  if (c >= n) return;
+  zc* col = A + (long)c * lda;
+  for (int jj = 0; jj < nb; ++jj) {
+    int p = piv[jj], j = j0 + jj;
+    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
+  }
+  if (c < j0) return;
+  zc x[LU_NB];
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[j0 + r] : mk(0.0);
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r) {
+#pragma unroll
+    for (int q = r + 1; q < LU_NB; ++q)
+      if (q < nb) x[q] = x[q] - L11[q][r] * x[r];
+  }
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r)
+    if (r < nb) col[j0 + r] = x[r];
+}
+
+// ------------------------------------------------------------------------------------------------
+// getrs pieces: row interchanges on B, and small diagonal-block triangular solves.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_laswp(int n, int nrhs, zc* B, int ldb, long sB, const int* ipiv, long sP) {
+  const int b = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  B += sB * b;
+  ipiv += sP * b;
+  zc* col = B + (long)c * ldb;
+  for (int j = 0; j < n; ++j) {
+    int p = ipiv[j];
+    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
+  }
+}
+
+// Solve T x = b for the nb x nb diagonal block starting at j0 (unit lower if lower, else upper),
+// for every right-hand-side column; one lane per column.
+__global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, const zc* LU, int lda,
+                                                   long sA, int nrhs, zc* B, int ldb, long sB) {
+  __shared__ zc T[LU_NB][LU_NB];
+  const int b = blockIdx.y;
+  LU += sA * b;
+  B += sB * b;
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    int r = e % nb, c = e / nb;
+    T[r][c] = LU[(long)(j0 + c) * lda + j0 + r];
+  }
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  zc* col = B + (long)c * ldb + j0;
+  zc x[LU_NB];
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[r] : mk(0.0);
+  if (lower) {
+#pragma unroll
+    for (int r = 0; r < LU_NB; ++r) {
+#pragma unroll
+      for (int q = r + 1; q < LU_NB; ++q)
+        if (q < nb) x[q] = x[q] - T[q][r] * x[r];
+    }
+  } else {
+#pragma unroll
+    for (int r = LU_NB - 1; r >= 0; --r) {
+      if (r < nb) {
+        x[r] = zdiv(x[r], T[r][r]);
+#pragma unroll
+        for (int q = 0; q < r; ++q) x[q] = x[q] - T[q][r] * x[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r)
+    if (r < nb) col[r] = x[r];
+}
+
+__global__ void k_fill_int(int n, int* p, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace
+
+void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const zc* A, int lda, long sA,
+                   const zc* B, int ldb, long sB, zc beta, zc* C, int ldc, long sC, int batch,
+                   hipStream_t s) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "zgemm: batch too large");
+  if (k <= 0) {
+    // C = beta * C
+    CWX_CHECK(false, "zgemm: k must be positive");
+  }
+  dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
+  hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, sA, B, ldb, sB,
+                     beta, C, ldc, sC);
+  CWX_HIP(hipGetLastError());
+}
+
+void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, int* info, int batch,
+                    hipStream_t s) {
+  if (n <= 0 || batch <= 0) return;
+  // a fill kernel rather than hipMemsetAsync keeps the sequence capturable in a HIP graph
+  hipLaunchKernelGGL(k_fill_int, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, info, 0);
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int nb = std::min(LU_NB, n - j0);
+    hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, sA, ipiv,
+                       sP, info);
+    const int others = n - nb;
+    if (others > 0)
+      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A,
+                         lda, sA, ipiv, sP);
+    const int rest = n - j0 - nb;
+    if (rest > 0)
+      zgemm_batched(OpN, OpN, rest, rest, nb, mk(-1.0), A + (long)j0 * lda + j0 + nb, lda, sA,
+                    A + (long)(j0 + nb) * lda + j0, lda, sA, mk(1.0),
+                    A + (long)(j0 + nb) * lda + j0 + nb, lda, sA, batch, s);
+  }
+  CWX_HIP(hipGetLastError());
+}
+
+void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,
+                    int ldb, long sB, int batch, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_laswp, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, n, nrhs, B, ldb, sB, ipiv,
+                     sP);
+  // forward: L (unit lower)
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int nb = std::min(LU_NB, n - j0);
+    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 1, LU, lda,
+                       sA, nrhs, B, ldb, sB);
+    const int rest = n - j0 - nb;
+    if (rest > 0)
+      zgemm_batched(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU + (long)j0 * lda + j0 + nb, lda, sA,
+                    B + j0, ldb, sB, mk(1.0), B + j0 + nb, ldb, sB, batch, s);
+  }
+  // backward: U
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int j0 = bi * LU_NB;
+    const int nb = std::min(LU_NB, n - j0);
+    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 0, LU, lda,
+                       sA, nrhs, B, ldb, sB);
+    if (j0 > 0)
+      zgemm_batched(OpN, OpN, j0, nrhs, nb, mk(-1.0), LU + (long)j0 * lda, lda, sA, B + j0, ldb, sB,
+                    mk(1.0), B, ldb, sB, batch, s);
+  }
+  CWX_HIP(hipGetLastError());
+}
+
+}  // namespace cwx

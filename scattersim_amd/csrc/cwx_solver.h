@@ -1,0 +1,94 @@
+// Host-side coupled-wave solver: option handling, device pipeline and solution storage.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "cwx_common.h"
+#include "cwx_kernels.h"
+
+namespace cwx {
+
+// Options of scattervolume (ScatterVolume.cpp:510-530).
+struct VolumeOptions {
+  double lambda = 1.0;
+  double direction[3] = {0.0, 0.0, 1.0};
+  double ex[2] = {0.0, 0.0}, ey[2] = {1.0, 0.0}, ez[2] = {0.0, 0.0};
+  double n[2] = {1.0, 1.0};
+  double kappa = 0.0;
+  double size[3] = {10.0, 10.0, 2.0};
+  double z = 0.0;
+  int ncoef = 0;
+  int coef[2] = {0, 0};
+  bool psf = false;
+  bool external = false;
+  bool npz = false;
+  bool no_cutoff = false;  // library callers may bypass the silent Fourier-order cut-off
+  int eig_mode = 0;        // 0: zgeev of the full 4MF Phi; 1: 2MF reduction through P Q
+};
+
+// Everything computed on the host before the per-layer work (ScatterVolume.cpp:574-685).
+struct Setup {
+  int M0 = 0, M1 = 0, MF = 0, L = 0, s1 = 0, s2 = 0;
+  double k = 0.0;
+  double dir_n[3] = {0, 0, 0};
+  zc e[3];
+  zc ni[2];
+  std::vector<double> z;            // L + 1 boundary positions
+  std::vector<double> up, wq;       // M0, M1
+  std::vector<zc> Sx, Sy, Sz0, Sz1; // MF each, column-major flattening of the (M1 x M0) meshes
+  std::vector<zc> EF;               // 3 MF
+  std::vector<int> rows, cols;      // DFT frequency index kept at each cropped row / column
+  bool cutoff = false;              // the order cut-off heuristic fired
+};
+
+Setup make_setup(const VolumeOptions& o, int L, int s1, int s2);
+
+struct Context;
+
+// Device-resident solution of one sample.
+struct Solution {
+  VolumeOptions opt;
+  Setup st;
+  int n4 = 0;                 // 4 MF
+  DevBuf<zc> gam;             // L x 4MF sorted eigenvalues (even: down, odd: up)
+  DevBuf<zc> Gd, Gc;          // L x (4MF)^2
+  DevBuf<zc> x;               // 6 MF boundary solution [R; T]
+  DevBuf<zc> beta;            // L x 4MF
+  DevBuf<zc> Qch;             // L x (3MF x 4MF) = [Q_check | Q_hat] per layer
+  DevBuf<zc> NIF;             // L x MF (M1 x M0 column-major)
+  DevBuf<zc> rec;             // .cw records, nrec x 6 (E0xyz, kxyz)
+  std::vector<long> seg_off;  // per list: Pi, then Pr_0, Pt_0, Pr_1, ...
+  std::vector<long> seg_cnt;
+  long nrec = 0;
+  int info = 0;               // >0: numerical failure (eig non-convergence or singular pivot)
+  std::string info_msg;
+};
+
+// Solve one sample given as (L, s1, s2) complex128 refractive indices in C order on the device.
+void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
+                  Solution& sol);
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  EigWork eig;
+  // scratch for the solver
+  DevBuf<zc> w0, w1, w2, w3, w4;
+  DevBuf<int> ipiv, info;
+  // uploaded field table
+  DevBuf<unsigned char> ftab;
+  DevBuf<float> fz;
+  DevBuf<int> fwb, fwe;
+  FieldTable table;
+  std::vector<float> h_z;
+};
+
+// Phi of every layer (blocks = 0, L x 4MF x 4MF) or its P, Q blocks (blocks = 1) into device memory.
+Setup build_phi_dev(const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2, int blocks, zc* out,
+                    hipStream_t s);
+
+void cw_write_file(const Solution& sol, const std::string& path, hipStream_t s);
+
+}  // namespace cwx

@@ -1,0 +1,61 @@
+"""Seeded synthetic samples and option sets of the benchmark configurations.
+
+Each sample is a complex128 grid of refractive index in C order, (L, Ny, Nx), as scattervolume reads
+it (the solver squares n itself, FourierWave.h:341).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _wrap(d, period):
+    return (d + period / 2) % period - period / 2
+
+
+def disks(N: int, cell: float, ndisk: int, n_bg, n_disk, rng, rmin=0.1, rmax=0.3):
+    """One N x N slice: background plus periodically wrapped disks of radius U[rmin, rmax]."""
+    xs = (np.arange(N) + 0.5) * cell / N
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    out = np.full((N, N), n_bg, dtype=np.complex128)
+    for _ in range(ndisk):
+        r = rng.uniform(rmin, rmax)
+        cx, cy = rng.uniform(0, cell, 2)
+        nd = n_disk(rng) if callable(n_disk) else n_disk
+        out[_wrap(X - cx, cell) ** 2 + _wrap(Y - cy, cell) ** 2 <= r * r] = nd
+    return out
+
+
+def plain_small(seed: int = 0):
+    """One 128 x 128 layer of 8 disks (r 0.1-0.3 um, n = 1.4) in n = 1.0, 4 x 4 x 1 um cell,
+    lambda 0.5 um, normal incidence, x-polarised, 7 x 7 orders (a 196 x 196 eigenproblem)."""
+    rng = np.random.default_rng(seed)
+    sample = disks(128, 4.0, 8, 1.0, 1.4 + 0j, rng)[None]
+    opts = dict(lam=0.5, direction=(0, 0, 1), ex=(1, 0), ey=(0, 0), n=(1.0, 1.0), size=(4.0, 4.0, 1.0),
+                coef=(7, 7))
+    grid = dict(x0=0.0, y0=0.0, z0=-0.5 + 0.5 / 32, dx=4.0 / 128, dy=4.0 / 128, dz=1.0 / 32, nx=128, ny=128, nz=32)
+    return sample, opts, grid
+
+
+def volume32(seed: int = 0):
+    """64 x 64 x 32 voxels sliced into 32 layers: random spheres with n ~ U[1.4, 1.6] and
+    kappa ~ U[0, 0.01] in an n = 1.33 background, 8 x 8 x 4 um cell, lambda 0.6 um, 15 x 15 orders
+    (900 x 900 eigenproblems), field on a 256 x 256 x 64 grid through the volume."""
+    rng = np.random.default_rng(seed)
+    L, N = 32, 64
+    cell = (8.0, 8.0, 4.0)
+    xs = (np.arange(N) + 0.5) * cell[0] / N
+    zs = (np.arange(L) + 0.5) * cell[2] / L
+    Z, Y, X = np.meshgrid(zs, xs, xs, indexing="ij")
+    sample = np.full((L, N, N), 1.33 + 0j, dtype=np.complex128)
+    for _ in range(14):
+        r = rng.uniform(0.6, 1.4)
+        c = rng.uniform(0, 1, 3) * np.asarray(cell)
+        n = rng.uniform(1.4, 1.6) + 1j * rng.uniform(0.0, 0.01)
+        d2 = _wrap(X - c[0], cell[0]) ** 2 + _wrap(Y - c[1], cell[1]) ** 2 + (Z - c[2]) ** 2
+        sample[d2 <= r * r] = n
+    opts = dict(lam=0.6, direction=(0, 0, 1), ex=(1, 0), ey=(0, 0), n=(1.33, 1.33), size=cell, coef=(15, 15))
+    grid = dict(x0=0.0, y0=0.0, z0=-2.0 + 2.0 / 64, dx=8.0 / 256, dy=8.0 / 256, dz=4.0 / 64, nx=256, ny=256, nz=64)
+    return sample, opts, grid
+
+
+CONFIGS = {"volume32": volume32, "plain_small": plain_small}

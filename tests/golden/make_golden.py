@@ -1,0 +1,62 @@
+"""Writes the committed fixtures of this directory from data files shipped with the reference.
+
+    python tests/golden/make_golden.py /path/to/reference
+
+* acw_te.npz  -- the 63 TE incidences of x64/a.cw (scatterlayer output for the stack
+                 n = 1 | 1.4+0.05i | 1.4 | 1, boundaries z = -3, 0, 3, lambda = 1): for each incident
+                 wave its record and the reflected/transmitted record at every boundary, as
+                 (E0.x, E0.y, E0.z = 0, k.x, k.y, k.z) complex128 rows.
+* samples.npz -- the input grids data/layers.npy, data/circle.npy and 3_npy.npy.
+The reference's records hold only (E0.x, E0.y, k); for ky = 0 TE waves E0.y is the global Ey.
+"""
+import os
+import struct
+import sys
+
+import numpy as np
+
+
+def read_cw80(path):
+    buf = open(path, "rb").read()
+    off = 0
+    prec, = struct.unpack_from("<Q", buf, off)
+    assert prec == 8
+    off += 8
+
+    def recs(off):
+        n, = struct.unpack_from("<Q", buf, off)
+        off += 8
+        a = np.frombuffer(buf, np.complex128, n * 5, off).reshape(n, 5)
+        off += n * 80
+        return np.concatenate([a[:, :2], np.zeros((n, 1), np.complex128), a[:, 2:]], axis=1), off
+
+    Pi, off = recs(off)
+    nb, = struct.unpack_from("<Q", buf, off)
+    off += 8
+    z, Pr, Pt = [], [], []
+    for _ in range(nb):
+        zz, = struct.unpack_from("<d", buf, off)
+        off += 8
+        r, off = recs(off)
+        t, off = recs(off)
+        z.append(zz)
+        Pr.append(r)
+        Pt.append(t)
+    assert off == len(buf), "trailing bytes"
+    return Pi, np.array(z), Pr, Pt
+
+
+def main(ref):
+    here = os.path.dirname(os.path.abspath(__file__))
+    Pi, z, Pr, Pt = read_cw80(os.path.join(ref, "x64", "a.cw"))
+    te = np.nonzero((np.abs(Pi[:, 4]) == 0) & (np.abs(Pi[:, 1]) > 0))[0]
+    np.savez_compressed(os.path.join(here, "acw_te.npz"), index=te, z=z, Pi=Pi[te],
+                        Pr=np.stack([p[te] for p in Pr], 1), Pt=np.stack([p[te] for p in Pt], 1))
+    np.savez_compressed(os.path.join(here, "samples.npz"),
+                        layers=np.load(os.path.join(ref, "data", "layers.npy")),
+                        circle=np.load(os.path.join(ref, "data", "circle.npy")),
+                        npy3=np.load(os.path.join(ref, "3_npy.npy")))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

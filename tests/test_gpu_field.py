@@ -1,0 +1,101 @@
+"""GPU parity of the plane-wave field evaluator (gpuEvaluator.cu:28-165) against the FP64 oracle sum."""
+import numpy as np
+import pytest
+
+import scattersim_amd as sa
+from oracle import cw_oracle as o
+from tests.helpers import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def random_table(seed, nwaves=(40, 120, 90, 60), z_layers=(-1.0, 0.0, 1.5), kscale=12.0, lossy=True):
+    """Random waves per region: Pi+Pr0 | Pt0+Pr1 | Pt1+Pr2 | Pt2, as cw_allocate lays them out."""
+    rng = np.random.default_rng(seed)
+    n = sum(nwaves)
+    W = np.zeros((n, 6), np.complex128)
+    W[:, 0:3] = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    W[:, 3:5] = kscale * rng.uniform(-0.7, 0.7, (n, 2))
+    W[:, 5] = kscale * rng.uniform(0.3, 1.0, n) * np.where(rng.random(n) < 0.5, 1, -1)
+    if lossy:
+        W[:, 3:6] += 1j * rng.uniform(0, 0.05, (n, 3))
+    b = np.zeros(len(nwaves), np.int64)
+    b[1:] = np.cumsum(nwaves)[:-1]
+    e = np.cumsum(nwaves)
+    return W, np.asarray(z_layers, np.float32), b, e
+
+
+def to_struct(W, dtype):
+    out = np.zeros(len(W), dtype)
+    out["E0"] = W[:, 0:3]
+    out["k"] = W[:, 3:6]
+    return out
+
+
+def grid_points(x0, y0, z0, dx, dy, dz, nx, ny, nz):
+    ix = np.arange(nx, dtype=np.float32)
+    iy = np.arange(ny, dtype=np.float32)
+    iz = np.arange(nz, dtype=np.float32)
+    Z, Y, X = np.meshgrid(np.float32(z0) + np.float32(dz) * iz, np.float32(y0) + np.float32(dy) * iy,
+                          np.float32(x0) + np.float32(dx) * ix, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+@pytest.mark.parametrize("tol_mode", [0, 1])
+def test_grid_f64_table(ctx, tol_mode):
+    torch = pytest.importorskip("torch")
+    W, zl, b, e = random_table(1)
+    ctx.field_init_f64(to_struct(W, sa.WAVE_F64), zl, b, e)
+    g = (-0.4, -0.3, -2.0, 0.05, 0.07, 0.125, 17, 9, 33)  # z crosses every boundary, incl. exactly z_l
+    out = torch.zeros((g[8], g[7], g[6], 3), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    ctx.field_grid_dev(out.data_ptr(), True, tol_mode, *g)
+    ctx.synchronize()
+    pts = grid_points(*g)
+    ref = o.field_points(W, zl, b, e, pts, tol=0.001 if tol_mode == 0 else 0.0)
+    got = out.cpu().numpy().reshape(-1, 3)
+    assert rel_err(got, ref) < 1e-6
+
+
+def test_grid_f32_table_reference_arithmetic(ctx):
+    torch = pytest.importorskip("torch")
+    W, zl, b, e = random_table(2, kscale=6.0)
+    W32 = W.astype(np.complex64)
+    ctx.field_init_f32(to_struct(W32, sa.WAVE_F32), zl, b, e)
+    g = (0.1, 0.2, -2.0, 0.1, 0.1, 0.2, 16, 8, 20)
+    out = torch.zeros((g[8], g[7], g[6], 3), dtype=torch.complex64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.field_grid_dev(out.data_ptr(), False, 0, *g)
+    ctx.synchronize()
+    ref = o.field_points(W32, zl, b, e, grid_points(*g), tol=0.001)
+    # FP32 phases with |k.r| <~ 30: relative error of a few 1e-6 per term
+    assert rel_err(out.cpu().numpy().reshape(-1, 3), ref) < 5e-5
+
+
+def test_slices_match_oracle(ctx):
+    W, zl, b, e = random_table(3)
+    ctx.field_init_f64(to_struct(W, sa.WAVE_F64), zl, b, e)
+    N, extent = 32, 3.0
+    center = (0.1, -0.2, 0.05)
+    plane = (0.3, -0.1, 0.4)
+    d = np.float32(extent / (N - 1))
+    start = np.asarray(center, np.float32) - np.float32(extent / 2)
+    Exy, Exz, Eyz = ctx.field_evaluate(float(start[0]), float(start[1]), float(start[2]), *plane, float(d), N)
+    for axis, got in ((2, Exy), (1, Exz), (0, Eyz)):
+        pts = o.slice_points(axis, N, extent, center, plane)
+        ref = o.field_points(W, zl, b, e, pts, tol=0.001).reshape(N, N, 3)
+        assert rel_err(got, ref) < 1e-6, axis
+
+
+def test_acw_fixture_field(ctx):
+    """Evaluate the TE subset of the reference's a.cw (scatterlayer output) on the GPU."""
+    g = golden("acw_te.npz")
+    Pi, Pr, Pt, z = g["Pi"], g["Pr"], g["Pt"], g["z"]
+    cw = o.CW(Pi=Pi, z=list(z), Pr=[Pr[:, l] for l in range(3)], Pt=[Pt[:, l] for l in range(3)])
+    W, zl, b, e = o.unpack(cw, np.complex128)
+    ctx.field_init_f64(to_struct(W, sa.WAVE_F64), zl, b, e)
+    N = 64
+    Exy, Exz, Eyz = ctx.field_evaluate(-2.0, -2.0, -5.0, 0.0, 0.0, 0.0, 10.0 / (N - 1), N)
+    pts = o.slice_points(1, N, 10.0, (3.0, 3.0, 0.0), (0.0, 0.0, 0.0))
+    ref = o.field_points(W, zl, b, e, pts, tol=0.001).reshape(N, N, 3)
+    assert rel_err(Exz, ref) < 1e-6

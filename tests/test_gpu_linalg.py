@@ -1,0 +1,120 @@
+"""GPU parity of the dense complex linear algebra (third_Lapack.h:38-85 seam) against numpy LAPACK."""
+import numpy as np
+import pytest
+
+import scattersim_amd as sa
+from oracle import cw_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def crand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (64, 64, 64), (100, 37, 129), (130, 65, 17)])
+def test_zgemm(ctx, m, n, k):
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    A, B = crand(rng, m, k), crand(rng, k, n)
+    C = sa.zgemm(ctx, A, B, alpha=0.5 - 0.25j)
+    ref = (0.5 - 0.25j) * (A @ B)
+    assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * np.sqrt(k)
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 3), (31, 1), (32, 4), (33, 2), (100, 1), (300, 7)])
+def test_zgesv(ctx, n, nrhs):
+    rng = np.random.default_rng(n * 7 + nrhs)
+    A = crand(rng, n, n) + 2 * np.eye(n)
+    B = crand(rng, n, nrhs)
+    X = sa.zgesv(ctx, A, B)
+    res = np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X))
+    assert res < 1e-14 * max(10, n)
+    assert np.max(np.abs(X - np.linalg.solve(A, B))) <= 1e-10 * np.max(np.abs(X))
+
+
+def test_zgesv_needs_pivoting(ctx):
+    # zero leading pivot: solvable only with row interchanges
+    A = np.array([[0, 1, 2], [1, 0, 3], [4, -3, 8]], dtype=np.complex128)
+    b = np.array([1, 2, 3], dtype=np.complex128)
+    x = sa.zgesv(ctx, A, b)
+    assert np.allclose(A @ x, b, atol=1e-14)
+
+
+def test_zinverse(ctx):
+    rng = np.random.default_rng(3)
+    A = crand(rng, 50, 50) + 3 * np.eye(50)
+    Ai = sa.zinverse(ctx, A)
+    assert np.max(np.abs(A @ Ai - np.eye(50))) < 1e-12
+
+
+def check_eig(A, w, V, tol_res=None):
+    n = A.shape[0]
+    nA = np.linalg.norm(A)
+    res = np.linalg.norm(A @ V - V * w[None, :]) / (max(nA, 1e-300) * np.sqrt(n))
+    assert res < (tol_res or 1e-14 * max(10, n)), res
+    # zgeev normalisation: unit 2-norm, largest component real and non-negative
+    assert np.allclose(np.linalg.norm(V, axis=0), 1.0, atol=1e-12)
+    k = np.argmax(np.abs(V) ** 2, axis=0)
+    vk = V[k, np.arange(n)]
+    assert np.all(vk.imag == 0.0)
+    # eigenvalue multiset against LAPACK
+    wr = np.linalg.eigvals(A)
+    for x in w:
+        assert np.min(np.abs(wr - x)) <= 1e-10 * max(nA, 1.0)
+    for x in wr:
+        assert np.min(np.abs(w - x)) <= 1e-10 * max(nA, 1.0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 33, 50, 98, 196])
+def test_zgeev_random(ctx, n):
+    rng = np.random.default_rng(n)
+    A = crand(rng, n, n)
+    w, V = sa.zgeev(ctx, A)
+    check_eig(A, w, V)
+
+
+def test_zgeev_structured(ctx):
+    # already triangular, diagonal with repeated values, and a Jordan-free degenerate case
+    T = np.triu(np.arange(1, 26).reshape(5, 5)).astype(np.complex128)
+    w, V = sa.zgeev(ctx, T)
+    check_eig(T, w, V)
+    D = np.diag([2, 2, 1j, 1j, -3]).astype(np.complex128)
+    w, V = sa.zgeev(ctx, D)
+    check_eig(D, w, V)
+    assert np.linalg.matrix_rank(V) == 5
+
+
+def test_zgeev_uniform_layer_phi(ctx):
+    # Phi of a uniform layer: eigenvalues +-sqrt(n^2 - Sx^2 - Sy^2), each doubly degenerate
+    p = o.VolumeParams(lam=0.5, size=(4, 4, 1), coef=(3, 3), n=(1.0, 1.0))
+    s = o.setup(np.full((1, 8, 8), 1.4 + 0j), p)
+    Nf, Nif = o.layer_fourier(s.layers[0], s.M0, s.M1)
+    Phi = o.phi_matrix(Nf, Nif, s.up, s.wq, s.k)
+    w, V = sa.zgeev(ctx, Phi)
+    check_eig(Phi, w, V)
+    assert np.linalg.matrix_rank(V, tol=1e-8) == Phi.shape[0]
+    sx = np.tile(s.up / s.k, s.M1)
+    sy = np.repeat(s.wq / s.k, s.M0)
+    g = np.sqrt((1.4 ** 2 - sx ** 2 - sy ** 2).astype(np.complex128))
+    expect = np.sort_complex(np.concatenate([g, g, -g, -g]))
+    assert np.max(np.abs(np.sort_complex(w) - expect)) < 1e-12
+
+
+def test_batched_dev_entry_points(ctx):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    n, b = 40, 5
+    A = crand(rng, b, n, n)
+    # column-major per matrix: store transposes in row-major
+    dA = torch.tensor(np.ascontiguousarray(A.transpose(0, 2, 1)), dtype=torch.complex128, device="cuda")
+    dW = torch.zeros((b, n), dtype=torch.complex128, device="cuda")
+    dV = torch.zeros((b, n, n), dtype=torch.complex128, device="cuda")
+    lib = sa.lib()
+    info = np.zeros(b, np.int32)
+    torch.cuda.synchronize()
+    sa.check(lib.cwx_zgeev_batched_dev(ctx.h, n, b, dA.data_ptr(), n * n, dW.data_ptr(), n, dV.data_ptr(), n * n,
+                                       info.ctypes.data))
+    W = dW.cpu().numpy()
+    V = dV.cpu().numpy().transpose(0, 2, 1)
+    for i in range(b):
+        check_eig(A[i], W[i], V[i])

@@ -1,0 +1,113 @@
+"""GPU parity of the scattervolume solve (ScatterVolume.cpp:574-987) against the FP64 oracle.
+
+Compared quantities are the ones every correct solve shares (eigenvector scaling and the basis of
+degenerate modes are free, A10 of the survey): the boundary solution x = [R; T], the sorted eigenvalue
+multiset per layer, and the fields the .cw records produce at points inside every region.  Bar:
+1e-9 relative for the FP64 path.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import scattersim_amd as sa
+from oracle import cw_oracle as o
+from tests.helpers import cw_from_records, disks_sample, fields, golden, probe_points, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def gpu_solve(ctx, sample, p: o.VolumeParams, eig_mode=0):
+    return sa.scattervolume(ctx, sample, lam=p.lam, direction=p.direction, ex=p.ex, ey=p.ey, ez=p.ez, n=p.n,
+                            kappa=p.kappa, size=p.size, z=p.z, coef=p.coef, psf=p.psf, external=p.external,
+                            npz=p.npz, no_cutoff=p.no_cutoff, eig_mode=eig_mode)
+
+
+def compare(ctx, sample, p, eig_mode=0, tol=TOL, check_gamma=True):
+    ref = o.solve(sample, p)
+    sol = gpu_solve(ctx, sample, p, eig_mode)
+    s = ref.setup
+    assert (sol.M0, sol.M1, sol.L) == (s.M0, s.M1, s.L)
+    assert rel_err(sol.x, ref.x) < tol
+    if check_gamma:
+        G = sol.gammas
+        for l in range(s.L):
+            assert rel_err(np.sort_complex(G[l]), np.sort_complex(ref.gammas[l])) < tol
+    # fields of the .cw records inside every region
+    off, cnt = sol.lists()
+    mine = cw_from_records(sol.records, cnt, sol.z)
+    theirs = o.build_cw(ref)
+    pts = probe_points(s.z, p.size)
+    assert rel_err(fields(mine, pts), fields(theirs, pts)) < tol
+    return sol, ref
+
+
+@pytest.mark.parametrize("eig_mode", [0, 1])
+def test_acw_known_answer(ctx, eig_mode):
+    """layers.npy through the M = 1 path against the reference-produced x64/a.cw (TE waves)."""
+    g = golden("acw_te.npz")
+    sample = golden("samples.npz")["layers"]
+    Pi, Pr, Pt, z = g["Pi"], g["Pr"], g["Pt"], g["z"]
+    worst = 0.0
+    for j in range(0, len(Pi), 9):
+        kv = Pi[j, 3:6].real
+        km = np.linalg.norm(kv)
+        sol = sa.scattervolume(ctx, sample, lam=2 * o.PI_TRUNC / km, direction=tuple(kv / km), ex=(0, 0), ey=(1, 0),
+                               n=(1.0, 1.0), size=(10, 10, 6), z=0.0, eig_mode=eig_mode)
+        off, cnt = sol.lists()
+        mine = cw_from_records(sol.records, cnt, sol.z)
+        ref = o.CW(Pi=Pi[j:j + 1], z=list(z), Pr=[Pr[j:j + 1, l] for l in range(3)],
+                   Pt=[Pt[j:j + 1, l] for l in range(3)])
+        pts = np.array([[0.3, 0.0, zz] for zz in (-4.0, -2.0, -0.5, 0.5, 2.5, 4.0)])
+        Em = fields(mine, pts)[:, 1] * Pi[j, 1]
+        Ea = fields(ref, pts)[:, 1]
+        worst = max(worst, rel_err(Em, Ea))
+    assert worst < 1e-12
+
+
+@pytest.mark.parametrize("coef", [(1, 1), (3, 3), (5, 5), (4, 3), (5, 1)])
+def test_circle_single_layer(ctx, coef):
+    sample = golden("samples.npz")["circle"]
+    p = o.VolumeParams(lam=0.6, size=(4.0, 4.0, 1.0), coef=coef, ex=(1, 0), ey=(0, 0), n=(1.0, 1.2), kappa=(0.01,))
+    compare(ctx, sample, p)
+
+
+@pytest.mark.parametrize("eig_mode", [0, 1])
+def test_three_layer_oblique(ctx, eig_mode):
+    sample = disks_sample(16, 3, 4.0, 4, 1.0, lambda r: 1.4 + 0.02j * r.random(), seed=5)
+    p = o.VolumeParams(lam=0.7, direction=(0.3, 0.1, 1.0), ex=(1, 0.5), ey=(0.2, 0), size=(4.0, 4.0, 1.5),
+                       coef=(5, 5), n=(1.0, 1.33))
+    compare(ctx, sample, p, eig_mode=eig_mode)
+
+
+def test_external_and_npz(ctx):
+    sample = disks_sample(12, 2, 3.0, 3, 1.0, 1.5, seed=9)
+    p = o.VolumeParams(lam=0.8, size=(3.0, 3.0, 1.0), coef=(3, 3), external=True)
+    ref = o.solve(sample, p)
+    sol = gpu_solve(ctx, sample, p)
+    assert rel_err(sol.x, ref.x) < TOL
+    off, cnt = sol.lists()
+    assert list(cnt) == [len(ref.setup.EF) // 3] + [9, 0, 0, 0, 0, 9]
+    p2 = o.VolumeParams(lam=0.8, size=(3.0, 3.0, 1.0), coef=(3, 3), npz=True)
+    sol2 = gpu_solve(ctx, sample, p2)
+    Q = sol2.Q_check_hat
+    assert np.all(Q[:, 18:27, :] == 0)
+
+
+def test_cw_file_roundtrip(ctx):
+    sample = disks_sample(8, 1, 2.0, 2, 1.0, 1.3, seed=1)
+    p = o.VolumeParams(lam=0.9, size=(2.0, 2.0, 0.5), coef=(3, 3))
+    sol = gpu_solve(ctx, sample, p)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c.cw")
+        sol.write_cw(path)
+        cw = o.cw_read(path)
+        W, zl, b, e = sa.cw_read_unpacked(path)
+    recs = sol.records
+    flat = np.concatenate([cw.Pi] + [x for pair in zip(cw.Pr, cw.Pt) for x in pair])
+    assert np.array_equal(flat, recs)
+    assert np.array_equal(W["E0"], recs[:, 0:3]) and np.array_equal(W["k"], recs[:, 3:6])
+    assert np.allclose(zl, sol.z)
