@@ -57,6 +57,13 @@ int cwx_destroy(cwx_context* ctx);
 int cwx_set_stream(cwx_context* ctx, void* hip_stream);
 int cwx_synchronize(cwx_context* ctx);
 
+/* Device memory of the context's device for callers without their own allocator (no reference
+ * counterpart).  cwx_memcpy kind: 0 host to device, 1 device to host, 2 device to device; it
+ * synchronises the context stream. */
+int cwx_dev_alloc(cwx_context* ctx, int64_t bytes, void** p);
+int cwx_dev_free(cwx_context* ctx, void* p);
+int cwx_memcpy(cwx_context* ctx, void* dst, const void* src, int64_t bytes, int kind);
+
 /* ---- dense complex FP64 linear algebra: the LAPACK seam (third_Lapack.h:38-85) --------------- */
 /* Replaces MKL_eigensolve (third_Lapack.h:38-45, LAPACKE_zgeev 'N','V').  A (n x n, host) is
  * overwritten by its Schur form; w (n) eigenvalues; VR (n x n) right eigenvectors of unit 2-norm
@@ -187,6 +194,14 @@ int cwx_field_evaluate(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz,
  * region test z >= z_l - 0.001 (gpuEvaluator.cu:34), 1 the CPU one z >= z_l (cpuEvaluator.cpp:303). */
 int cwx_field_grid_dev(cwx_context* ctx, void* d_out, int out_f64, int tol_mode, float x0, float y0,
                        float z0, float dx, float dy, float dz, int nx, int ny, int nz);
+
+/* Field of a solution on a regular grid without a wave table: the records of every region share
+ * the per-order transverse wavevectors, so the sum of gpuEvaluator.cu:53-66 is regrouped as
+ * exp(i kx x) . C(z) . exp(i ky y) with C(z) from the mode amplitudes (FP64 throughout).  Same
+ * output layout, region test and tol_mode as cwx_field_grid_dev; no counterpart in the reference. */
+int cwx_field_solution_grid_dev(cwx_context* ctx, const cwx_solution* sol, void* d_out, int out_f64,
+                                int tol_mode, float x0, float y0, float z0, float dx, float dy, float dz,
+                                int nx, int ny, int nz);
 
 #ifdef __cplusplus
 }

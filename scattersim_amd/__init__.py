@@ -21,7 +21,7 @@ import numpy as np
 from . import _capi
 from ._capi import check, lib
 
-__all__ = ["Context", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
+__all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
            "cw_read_unpacked", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
@@ -89,10 +89,54 @@ class Context:
                                        z_start, x, y, z, d, N))
         return tuple(out)
 
+    def field_solution_grid_dev(self, sol: "Solution", d_out: int, out_f64: bool, tol_mode: int, x0, y0, z0, dx,
+                                dy, dz, nx, ny, nz):
+        """Separable evaluation of a solution on a grid into device memory ([nz][ny][nx][3] complex)."""
+        check(lib().cwx_field_solution_grid_dev(self.h, sol.h, C.c_void_p(d_out), int(out_f64), tol_mode, x0, y0,
+                                                z0, dx, dy, dz, nx, ny, nz))
+
     def field_grid_dev(self, d_out: int, out_f64: bool, tol_mode: int, x0, y0, z0, dx, dy, dz, nx, ny, nz):
         """Evaluate a grid into device memory at address d_out ([nz][ny][nx][3] complex)."""
         check(lib().cwx_field_grid_dev(self.h, C.c_void_p(d_out), int(out_f64), tol_mode, x0, y0, z0, dx, dy,
                                        dz, nx, ny, nz))
+
+
+class DeviceArray:
+    """A device allocation of the context's device holding a numpy-shaped array (cwx_dev_alloc)."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if not isinstance(shape, int) else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().cwx_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        if d.nbytes:
+            check(lib().cwx_memcpy(ctx.h, C.c_void_p(d.ptr), _ptr(a), d.nbytes, 0))
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().cwx_memcpy(self.ctx.h, _ptr(out), C.c_void_p(self.ptr), self.nbytes, 1))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cwx_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _params(lam=1.0, direction=(0, 0, 1), ex=(0, 0), ey=(1, 0), ez=(0, 0), n=(1, 1), kappa=0.0,

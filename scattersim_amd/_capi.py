@@ -59,6 +59,9 @@ SIGNATURES = {
     "cwx_destroy": (I, [P]),
     "cwx_set_stream": (I, [P, P]),
     "cwx_synchronize": (I, [P]),
+    "cwx_dev_alloc": (I, [P, I64, C.POINTER(P)]),
+    "cwx_dev_free": (I, [P, P]),
+    "cwx_memcpy": (I, [P, P, P, I64, I]),
     "cwx_zgeev": (I, [P, I, P, I, P, P, I]),
     "cwx_zgesv": (I, [P, I, I, P, I, P, P, I]),
     "cwx_zinverse": (I, [P, I, P, I, P, I]),
@@ -85,6 +88,7 @@ SIGNATURES = {
     "cwx_field_init_solution": (I, [P, P]),
     "cwx_field_evaluate": (I, [P, P, P, P, F, F, F, F, F, F, F, I]),
     "cwx_field_grid_dev": (I, [P, P, I, I, F, F, F, F, F, F, I, I, I]),
+    "cwx_field_solution_grid_dev": (I, [P, P, P, I, I, F, F, F, F, F, F, I, I, I]),
 }
 
 _lib = None

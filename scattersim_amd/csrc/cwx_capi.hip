@@ -150,6 +150,41 @@ int cwx_synchronize(cwx_context* ctx) {
 }
 
 // ---------------------------------------------------------------------------------------------
+int cwx_dev_alloc(cwx_context* ctx, int64_t bytes, void** p) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && p && bytes >= 0, "bad argument");
+    set_device(ctx);
+    *p = nullptr;
+    if (bytes) CWX_HIP(hipMalloc(p, (size_t)bytes));
+    return CWX_OK;
+  });
+}
+
+int cwx_dev_free(cwx_context* ctx, void* p) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx != nullptr, "ctx must not be NULL");
+    set_device(ctx);
+    if (p) {
+      CWX_HIP(hipStreamSynchronize(ctx->c.stream));
+      CWX_HIP(hipFree(p));
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_memcpy(cwx_context* ctx, void* dst, const void* src, int64_t bytes, int kind) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dst && src && bytes >= 0, "bad argument");
+    CWX_CHECK(kind >= 0 && kind <= 2, "kind must be 0 (host to device), 1 (device to host) or 2 (device to device)");
+    set_device(ctx);
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    CWX_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, ctx->c.stream));
+    CWX_HIP(hipStreamSynchronize(ctx->c.stream));
+    return CWX_OK;
+  });
+}
+
 int cwx_zgeev(cwx_context* ctx, int n, cwx_complex* A, int lda, cwx_complex* w, cwx_complex* VR, int ldvr) {
   return guard([&]() -> int {
     CWX_CHECK(ctx && A && w && VR, "NULL argument");
@@ -634,6 +669,20 @@ int cwx_field_grid_dev(cwx_context* ctx, void* d_out, int out_f64, int tol_mode,
     set_device(ctx);
     cwx::GridSpec g{x0, y0, z0, dx, dy, dz, nx, ny, nz};
     cwx::launch_field_direct(ctx->c.table, g, tol_mode, out_f64, d_out, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_field_solution_grid_dev(cwx_context* ctx, const cwx_solution* sol, void* d_out, int out_f64, int tol_mode,
+                                float x0, float y0, float z0, float dx, float dy, float dz, int nx, int ny,
+                                int nz) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && sol && d_out, "NULL argument");
+    CWX_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && nz <= 65535, "bad grid dimensions");
+    CWX_CHECK(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 or 1");
+    set_device(ctx);
+    cwx::field_separable(sol->s, cwx::GridSpec{x0, y0, z0, dx, dy, dz, nx, ny, nz}, tol_mode, out_f64, d_out,
+                         ctx->c.sep, ctx->c.stream);
     return CWX_OK;
   });
 }

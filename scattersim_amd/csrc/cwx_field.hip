@@ -20,6 +20,10 @@ namespace cwx {
 
 namespace {
 
+// origin + step * i rounded as two separate float operations (no fused multiply-add), so that grid
+// coordinates are reproducible bit for bit on the host
+__device__ inline float coord(float o, float d, int i) { return __fadd_rn(o, __fmul_rn(d, (float)i)); }
+
 constexpr int kFieldThreads = 256;
 
 __device__ inline int region_of(float z, const float* __restrict__ zl, int layers, int tol_mode) {
@@ -50,7 +54,7 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_f32(
   __shared__ WaveF32 sw[kFieldThreads];
   const int iz = blockIdx.y;
   const long plane = (long)g.nx * g.ny;
-  const float z = g.z0 + g.dz * (float)iz;
+  const float z = coord(g.z0, g.dz, iz);
   const int l = region_of(z, zl, layers, tol_mode);
   const int b = wb[l], e = we[l];
 
@@ -62,8 +66,8 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_f32(
     idx[p] = (long)blockIdx.x * kFieldThreads * PTS + p * kFieldThreads + threadIdx.x;
     long q = idx[p] < plane ? idx[p] : 0;
     int ix = (int)(q % g.nx), iy = (int)(q / g.nx);
-    px[p] = g.x0 + g.dx * (float)ix;
-    py[p] = g.y0 + g.dy * (float)iy;
+    px[p] = coord(g.x0, g.dx, ix);
+    py[p] = coord(g.y0, g.dy, iy);
 #pragma unroll
     for (int c = 0; c < 3; ++c) E[p][c] = make_float2(0.f, 0.f);
   }
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_f64r(
   __shared__ WaveF64 sw[kFieldThreads];
   const int iz = blockIdx.y;
   const long plane = (long)g.nx * g.ny;
-  const float zf = g.z0 + g.dz * (float)iz;
+  const float zf = coord(g.z0, g.dz, iz);
   const double z = (double)zf;
   const int l = region_of(zf, zl, layers, tol_mode);
   const int b = wb[l], e = we[l];
@@ -139,8 +143,8 @@ __global__ __launch_bounds__(kFieldThreads) void k_field_f64r(
     idx[p] = (long)blockIdx.x * kFieldThreads * PTS + p * kFieldThreads + threadIdx.x;
     long q = idx[p] < plane ? idx[p] : 0;
     int ix = (int)(q % g.nx), iy = (int)(q / g.nx);
-    px[p] = (double)(g.x0 + g.dx * (float)ix);
-    py[p] = (double)(g.y0 + g.dy * (float)iy);
+    px[p] = (double)(coord(g.x0, g.dx, ix));
+    py[p] = (double)(coord(g.y0, g.dy, iy));
 #pragma unroll
     for (int c = 0; c < 3; ++c) E[p][c] = make_double2(0.0, 0.0);
   }

@@ -57,6 +57,7 @@ struct Solution {
   DevBuf<zc> beta;            // L x 4MF
   DevBuf<zc> Qch;             // L x (3MF x 4MF) = [Q_check | Q_hat] per layer
   DevBuf<zc> NIF;             // L x MF (M1 x M0 column-major)
+  DevBuf<zc> tabs;            // Sx | Sy | Sz0 | Sz1 (MF each) | EF (3MF)
   DevBuf<zc> rec;             // .cw records, nrec x 6 (E0xyz, kxyz)
   std::vector<long> seg_off;  // per list: Pi, then Pr_0, Pt_0, Pr_1, ...
   std::vector<long> seg_cnt;
@@ -68,6 +69,17 @@ struct Solution {
 // Solve one sample given as (L, s1, s2) complex128 refractive indices in C order on the device.
 void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
                   Solution& sol);
+
+// Work buffers of the separable field evaluation.
+struct SepWork {
+  DevBuf<zc> C, X, Y;
+  DevBuf<float> zf;
+  DevBuf<double> zd;
+};
+
+// Field of a solution on a regular grid by order-separable synthesis (see cwx_fieldsep.hip).
+void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int out_f64, void* out,
+                     SepWork& w, hipStream_t s);
 
 struct Context {
   int device = 0;
@@ -83,6 +95,7 @@ struct Context {
   DevBuf<int> fwb, fwe;
   FieldTable table;
   std::vector<float> h_z;
+  SepWork sep;
 };
 
 // Phi of every layer (blocks = 0, L x 4MF x 4MF) or its P, Q blocks (blocks = 1) into device memory.

@@ -706,7 +706,6 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
 
   // small host tables -> device
   DevBuf<double> d_up, d_wq, d_dz, d_upz, d_wqz;
-  DevBuf<zc> d_Sx, d_Sy, d_Sz0, d_Sz1, d_EF;
   upload(d_up, st.up, s);
   upload(d_wq, st.wq, s);
   std::vector<double> dz(L);
@@ -721,11 +720,18 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
     }
   upload(d_upz, upz, s);
   upload(d_wqz, wqz, s);
-  upload(d_Sx, st.Sx, s);
-  upload(d_Sy, st.Sy, s);
-  upload(d_Sz0, st.Sz0, s);
-  upload(d_Sz1, st.Sz1, s);
-  upload(d_EF, st.EF, s);
+  // Sx | Sy | Sz0 | Sz1 | EF (3MF), kept with the solution for the separable field evaluation
+  {
+    std::vector<zc> tabs;
+    tabs.reserve(7 * (size_t)MF);
+    for (auto* v : {&st.Sx, &st.Sy, &st.Sz0, &st.Sz1, &st.EF}) tabs.insert(tabs.end(), v->begin(), v->end());
+    upload(sol.tabs, tabs, s);
+  }
+  const zc* dSx = sol.tabs.p;
+  const zc* dSy = sol.tabs.p + MF;
+  const zc* dSz0 = sol.tabs.p + 2 * MF;
+  const zc* dSz1 = sol.tabs.p + 3 * MF;
+  const zc* dEF = sol.tabs.p + 4 * MF;
   DevBuf<zc> F((size_t)2 * L * MF);
   fourier_stage(st, d_sample, F.p, s);
   const zc* NF = F.p;
@@ -802,7 +808,7 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
   }
 
   // ---- 5. boundary system: A = [Gauss; f2 | Gd0 S^-1 f3], b = [0; -f1 EF] ----
-  BoundaryArgs ba{MF, k, st.z[0], d_Sx.p, d_Sy.p, d_Sz0.p, d_Sz1.p, d_EF.p};
+  BoundaryArgs ba{MF, k, st.z[0], dSx, dSy, dSz0, dSz1, dEF};
   DevBuf<zc> Y((size_t)n4 * n3), Abr((size_t)n4 * n3), A((size_t)n6 * n6);
   sol.x.alloc(n6);
   int* pivS = piv + (long)L * n4;
@@ -869,7 +875,7 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
   upload(d_segs, segs, s);
   long maxcnt = 1;
   for (auto& g : segs) maxcnt = std::max(maxcnt, g.cnt);
-  RecArgs ra{MF, k, o.n[0], o.n[1], d_Sx.p, d_Sy.p, d_Sz0.p, d_Sz1.p, d_EF.p, sol.x.p, sol.gam.p,
+  RecArgs ra{MF, k, o.n[0], o.n[1], dSx, dSy, dSz0, dSz1, dEF, sol.x.p, sol.gam.p,
              o.external ? nullptr : sol.Qch.p};
   hipLaunchKernelGGL(k_records, dim3(grid_for(maxcnt), (unsigned)segs.size()), dim3(256), 0, s, ra, d_segs.p,
                      sol.rec.p);

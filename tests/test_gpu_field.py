@@ -43,33 +43,29 @@ def grid_points(x0, y0, z0, dx, dy, dz, nx, ny, nz):
 
 @pytest.mark.parametrize("tol_mode", [0, 1])
 def test_grid_f64_table(ctx, tol_mode):
-    torch = pytest.importorskip("torch")
     W, zl, b, e = random_table(1)
     ctx.field_init_f64(to_struct(W, sa.WAVE_F64), zl, b, e)
     g = (-0.4, -0.3, -2.0, 0.05, 0.07, 0.125, 17, 9, 33)  # z crosses every boundary, incl. exactly z_l
-    out = torch.zeros((g[8], g[7], g[6], 3), dtype=torch.complex128, device="cuda")
-    torch.cuda.synchronize()
-    ctx.field_grid_dev(out.data_ptr(), True, tol_mode, *g)
+    out = sa.DeviceArray(ctx, (g[8], g[7], g[6], 3), np.complex128)
+    ctx.field_grid_dev(out.ptr, True, tol_mode, *g)
     ctx.synchronize()
     pts = grid_points(*g)
     ref = o.field_points(W, zl, b, e, pts, tol=0.001 if tol_mode == 0 else 0.0)
-    got = out.cpu().numpy().reshape(-1, 3)
+    got = out.numpy().reshape(-1, 3)
     assert rel_err(got, ref) < 1e-6
 
 
 def test_grid_f32_table_reference_arithmetic(ctx):
-    torch = pytest.importorskip("torch")
     W, zl, b, e = random_table(2, kscale=6.0)
     W32 = W.astype(np.complex64)
     ctx.field_init_f32(to_struct(W32, sa.WAVE_F32), zl, b, e)
     g = (0.1, 0.2, -2.0, 0.1, 0.1, 0.2, 16, 8, 20)
-    out = torch.zeros((g[8], g[7], g[6], 3), dtype=torch.complex64, device="cuda")
-    torch.cuda.synchronize()
-    ctx.field_grid_dev(out.data_ptr(), False, 0, *g)
+    out = sa.DeviceArray(ctx, (g[8], g[7], g[6], 3), np.complex64)
+    ctx.field_grid_dev(out.ptr, False, 0, *g)
     ctx.synchronize()
     ref = o.field_points(W32, zl, b, e, grid_points(*g), tol=0.001)
     # FP32 phases with |k.r| <~ 30: relative error of a few 1e-6 per term
-    assert rel_err(out.cpu().numpy().reshape(-1, 3), ref) < 5e-5
+    assert rel_err(out.numpy().reshape(-1, 3), ref) < 5e-5
 
 
 def test_slices_match_oracle(ctx):
@@ -99,3 +95,30 @@ def test_acw_fixture_field(ctx):
     pts = o.slice_points(1, N, 10.0, (3.0, 3.0, 0.0), (0.0, 0.0, 0.0))
     ref = o.field_points(W, zl, b, e, pts, tol=0.001).reshape(N, N, 3)
     assert rel_err(Exz, ref) < 1e-6
+
+
+@pytest.mark.parametrize("external", [False, True])
+def test_separable_matches_direct_and_oracle(ctx, external):
+    """Order-separable synthesis of a solution's field equals the direct sum over its .cw records."""
+    from tests.helpers import disks_sample
+    sample = disks_sample(12, 3, 3.0, 3, 1.0, lambda r: 1.45 + 0.02j * r.random(), seed=4)
+    opts = dict(lam=0.7, direction=(0.2, 0.1, 1.0), ex=(1, 0), ey=(0.3, 0), size=(3.0, 3.0, 1.2), coef=(5, 3),
+                n=(1.0, 1.2), external=external)
+    sol = sa.scattervolume(ctx, sample, **opts)
+    g = (-0.3, 0.1, -1.4, 0.11, 0.13, 0.1, 37, 21, 29)      # planes above, inside every layer and below
+    sep = sa.DeviceArray(ctx, (g[8], g[7], g[6], 3), np.complex128)
+    direct = sa.DeviceArray(ctx, (g[8], g[7], g[6], 3), np.complex128)
+    ctx.field_solution_grid_dev(sol, sep.ptr, True, 0, *g)
+    ctx.field_init_solution(sol)
+    ctx.field_grid_dev(direct.ptr, True, 0, *g)
+    ctx.synchronize()
+    a, b = sep.numpy(), direct.numpy()
+    assert rel_err(a, b) < 1e-6           # the direct FP64 kernel reduces phases to FP32 sincos
+    p = o.VolumeParams(lam=0.7, direction=(0.2, 0.1, 1.0), ex=(1, 0), ey=(0.3, 0), size=(3.0, 3.0, 1.2),
+                       coef=(5, 3), n=(1.0, 1.2), external=external)
+    ref = o.build_cw(o.solve(sample, p))
+    W, zl, bb, ee = o.unpack(ref, np.complex128)
+    pts = grid_points(*g)
+    E = o.field_points(W, zl, bb, ee, pts, tol=0.001).reshape(a.shape)
+    assert rel_err(a, E) < 1e-9
+    sol.free()

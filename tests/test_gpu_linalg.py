@@ -101,20 +101,37 @@ def test_zgeev_uniform_layer_phi(ctx):
 
 
 def test_batched_dev_entry_points(ctx):
-    torch = pytest.importorskip("torch")
     rng = np.random.default_rng(11)
     n, b = 40, 5
     A = crand(rng, b, n, n)
-    # column-major per matrix: store transposes in row-major
-    dA = torch.tensor(np.ascontiguousarray(A.transpose(0, 2, 1)), dtype=torch.complex128, device="cuda")
-    dW = torch.zeros((b, n), dtype=torch.complex128, device="cuda")
-    dV = torch.zeros((b, n, n), dtype=torch.complex128, device="cuda")
     lib = sa.lib()
+    # column-major per matrix: store transposes in row-major
+    dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(A.transpose(0, 2, 1)))
+    dW = sa.DeviceArray(ctx, (b, n), np.complex128)
+    dV = sa.DeviceArray(ctx, (b, n, n), np.complex128)
     info = np.zeros(b, np.int32)
-    torch.cuda.synchronize()
-    sa.check(lib.cwx_zgeev_batched_dev(ctx.h, n, b, dA.data_ptr(), n * n, dW.data_ptr(), n, dV.data_ptr(), n * n,
-                                       info.ctypes.data))
-    W = dW.cpu().numpy()
-    V = dV.cpu().numpy().transpose(0, 2, 1)
+    sa.check(lib.cwx_zgeev_batched_dev(ctx.h, n, b, dA.ptr, n * n, dW.ptr, n, dV.ptr, n * n, info.ctypes.data))
+    W = dW.numpy()
+    V = dV.numpy().transpose(0, 2, 1)
     for i in range(b):
         check_eig(A[i], W[i], V[i])
+    # LU + solve on the device
+    M = crand(rng, b, n, n) + 3 * np.eye(n)
+    B = crand(rng, b, n, 3)
+    dM = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(M.transpose(0, 2, 1)))
+    dB = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(B.transpose(0, 2, 1)))
+    dP = sa.DeviceArray(ctx, (b, n), np.int32)
+    sa.check(lib.cwx_zgetrf_batched_dev(ctx.h, n, b, dM.ptr, n * n, dP.ptr, n, info.ctypes.data))
+    sa.check(lib.cwx_zgetrs_batched_dev(ctx.h, n, 3, b, dM.ptr, n * n, dP.ptr, n, dB.ptr, n * 3))
+    X = dB.numpy().transpose(0, 2, 1)
+    for i in range(b):
+        assert np.max(np.abs(M[i] @ X[i] - B[i])) < 1e-12
+    # GEMM with a conjugate-transposed operand
+    G = crand(rng, b, n, n)
+    dG = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(G.transpose(0, 2, 1)))
+    dC = sa.DeviceArray(ctx, (b, n, n), np.complex128)
+    sa.check(lib.cwx_zgemm_batched_dev(ctx.h, 2, 0, n, n, n, sa._capi.cwx_complex(1, 0), dG.ptr, n, n * n, dG.ptr, n,
+                                       n * n, sa._capi.cwx_complex(0, 0), dC.ptr, n, n * n, b))
+    Cm = dC.numpy().transpose(0, 2, 1)
+    for i in range(b):
+        assert np.max(np.abs(Cm[i] - G[i].conj().T @ G[i])) < 1e-12 * np.abs(Cm[i]).max()
