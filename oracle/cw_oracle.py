@@ -716,6 +716,14 @@ def field_points(W, z_layers, begin, end, pts: np.ndarray, tol: float = 0.001) -
     return out
 
 
+def grid_coord(origin, step, i) -> np.ndarray:
+    """float32 coordinate origin + step * i rounded once, as a fused multiply-add computes it
+    (gpuEvaluator.cu:78-79 under contraction).  The exact value is formed in extended precision."""
+    o = np.longdouble(np.float32(origin))
+    d = np.longdouble(np.float32(step))
+    return (o + d * np.asarray(i, dtype=np.longdouble)).astype(np.float32)
+
+
 def slice_points(axis: int, N: int, extent: float, center, plane_position):
     """ScatterView.cpp:354-376 with gpuEvaluator.cu:70-129: points of the xy / xz / yz slice.
 
@@ -723,22 +731,33 @@ def slice_points(axis: int, N: int, extent: float, center, plane_position):
     """
     d = np.float32(extent / (N - 1))
     start = np.asarray(center, np.float32) - np.float32(extent / 2)
-    i = np.arange(N, dtype=np.float32)
-    a = (i * d).astype(np.float32)
+    i = np.arange(N)
     out = np.zeros((N, N, 3), dtype=np.float32)
     if axis == 2:      # xy: E_xy[yi, xi] at (x_start + d xi, y_start + d yi, z)
-        out[..., 0] = (start[0] + a)[None, :]
-        out[..., 1] = (start[1] + a)[:, None]
+        out[..., 0] = grid_coord(start[0], d, i)[None, :]
+        out[..., 1] = grid_coord(start[1], d, i)[:, None]
         out[..., 2] = np.float32(plane_position[2])
     elif axis == 1:    # xz: E_xz[zi, xi] at (x_start + d xi, y, z_start + d zi)
-        out[..., 0] = (start[0] + a)[None, :]
+        out[..., 0] = grid_coord(start[0], d, i)[None, :]
         out[..., 1] = np.float32(plane_position[1])
-        out[..., 2] = (start[2] + a)[:, None]
+        out[..., 2] = grid_coord(start[2], d, i)[:, None]
     else:              # yz: E_yz[zi, yi] at (x, y_start + d yi, z_start + d zi)
         out[..., 0] = np.float32(plane_position[0])
-        out[..., 1] = (start[1] + a)[None, :]
-        out[..., 2] = (start[2] + a)[:, None]
+        out[..., 1] = grid_coord(start[1], d, i)[None, :]
+        out[..., 2] = grid_coord(start[2], d, i)[:, None]
     return out.reshape(N * N, 3)
+
+
+def grid_points(x0, y0, z0, dx, dy, dz, nx, ny, nz) -> np.ndarray:
+    """(nz*ny*nx, 3) float32 points of a regular grid in [iz][iy][ix] order."""
+    X = grid_coord(x0, dx, np.arange(nx))
+    Y = grid_coord(y0, dy, np.arange(ny))
+    Z = grid_coord(z0, dz, np.arange(nz))
+    P = np.zeros((nz, ny, nx, 3), np.float32)
+    P[..., 0] = X[None, None, :]
+    P[..., 1] = Y[None, :, None]
+    P[..., 2] = Z[:, None, None]
+    return P.reshape(-1, 3)
 
 
 def far_field_flux(sol: Solution):

@@ -6,6 +6,7 @@
  *                             waves of the region of r, complex<float> arithmetic;
  *   gpuEvaluator.cu:32-37     the region test z >= z_l - 0.001 (tol_mode 0) or z >= z_l (tol_mode 1).
  *
+ * Grid coordinates are origin + step * i rounded once (fmaf), as the GPU kernels compute them.
  * cw_eval_grid_f32 keeps the reference's FP32 arithmetic; cw_eval_grid_f64 evaluates the same sum in
  * FP64 on FP64 waves.  Both are OpenMP-parallel over points.
  *
@@ -37,7 +38,7 @@ void cw_eval_grid_f32(const float complex* W, const float* zl, const int* wb, co
   for (long p = 0; p < plane * nz; ++p) {
     const int iz = (int)(p / plane);
     const long q = p % plane;
-    const float x = x0 + dx * (float)(q % nx), y = y0 + dy * (float)(q / nx), z = z0 + dz * (float)iz;
+    const float x = fmaf(dx, (float)(q % nx), x0), y = fmaf(dy, (float)(q / nx), y0), z = fmaf(dz, (float)iz, z0);
     const int l = region(z, zl, layers, tol_mode);
     float complex E0 = 0, E1 = 0, E2 = 0;
     for (int w = wb[l]; w < we[l]; ++w) {
@@ -62,7 +63,7 @@ void cw_eval_grid_f64(const double complex* W, const float* zl, const int* wb, c
   for (long p = 0; p < plane * nz; ++p) {
     const int iz = (int)(p / plane);
     const long q = p % plane;
-    const float xf = x0 + dx * (float)(q % nx), yf = y0 + dy * (float)(q / nx), zf = z0 + dz * (float)iz;
+    const float xf = fmaf(dx, (float)(q % nx), x0), yf = fmaf(dy, (float)(q / nx), y0), zf = fmaf(dz, (float)iz, z0);
     const double x = xf, y = yf, z = zf;
     const int l = region(zf, zl, layers, tol_mode);
     double complex E0 = 0, E1 = 0, E2 = 0;

@@ -48,14 +48,105 @@ __device__ inline double dlapy3(double x, double y, double z) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Level-2 building blocks for one workgroup operating on a column-major block in global memory.
+// Loads are unrolled with independent accumulators so that several are in flight per lane: the
+// block lives in L2 and latency, not bandwidth, limits a single workgroup.
+// ------------------------------------------------------------------------------------------------
+constexpr int HH_THREADS = 1024;
+
+// w[r] = scale * sum_c A(r, c) v[c], r < rows, c < cols.  Lanes run down the rows; when the rows do
+// not fill the workgroup, G groups of lanes take interleaved columns and their partial sums are
+// combined through `part` (LDS, at least blockDim.x entries).  Ends with a barrier.
+__device__ inline void gemv_rows(const zc* __restrict__ A, int lda, int rows, int cols, const zc* v, zc scale,
+                                 zc* w, zc* part) {
+  const int R = (rows + 63) & ~63;
+  const int G = rows >= (int)blockDim.x ? 1 : min(8, (int)blockDim.x / R);
+  const int g = threadIdx.x / R;
+  for (int r = threadIdx.x % R; r < rows; r += (G == 1 ? blockDim.x : R)) {
+    if (g >= G) break;
+    zc a0 = mk(0.0), a1 = mk(0.0), a2 = mk(0.0), a3 = mk(0.0);
+    const zc* p = A + r;
+    int c = g;
+    for (; c + 3 * G < cols; c += 4 * G) {
+      const zc x0 = p[(long)c * lda], x1 = p[(long)(c + G) * lda];
+      const zc x2 = p[(long)(c + 2 * G) * lda], x3 = p[(long)(c + 3 * G) * lda];
+      fma_acc(a0, x0, v[c]);
+      fma_acc(a1, x1, v[c + G]);
+      fma_acc(a2, x2, v[c + 2 * G]);
+      fma_acc(a3, x3, v[c + 3 * G]);
+    }
+    for (; c < cols; c += G) fma_acc(a0, p[(long)c * lda], v[c]);
+    const zc acc = (a0 + a1) + (a2 + a3);
+    if (G == 1) w[r] = acc * scale;
+    else part[g * rows + r] = acc;
+  }
+  __syncthreads();
+  if (G > 1) {
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+      zc acc = part[r];
+      for (int q = 1; q < G; ++q) acc += part[q * rows + r];
+      w[r] = acc * scale;
+    }
+    __syncthreads();
+  }
+}
+
+// u[c] = scale * sum_r conj(v[r]) A(r, c) (wavefront per column, lanes down the rows)
+__device__ inline void gemv_cols_conj(const zc* __restrict__ A, int lda, int rows, int cols, const zc* v,
+                                      zc scale, zc* u) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int c = wid; c < cols; c += nw) {
+    const zc* col = A + (long)c * lda;
+    zc a0 = mk(0.0), a1 = mk(0.0);
+    int r = lane;
+    for (; r + 64 < rows; r += 128) {
+      const zc x0 = col[r], x1 = col[r + 64];
+      fma_acc(a0, conj(v[r]), x0);
+      fma_acc(a1, conj(v[r + 64]), x1);
+    }
+    if (r < rows) fma_acc(a0, conj(v[r]), col[r]);
+    zc acc = a0 + a1;
+    acc.re = wave_sum(acc.re);
+    acc.im = wave_sum(acc.im);
+    if (lane == 0) u[c] = acc * scale;
+  }
+}
+
+// A(r, c) -= x[r] * y[c] for r < rows, c < cols; y_conj conjugates y (flat, four elements per lane)
+__device__ inline void rank1_sub(zc* __restrict__ A, int lda, int rows, int cols, const zc* x, const zc* y,
+                                 bool y_conj) {
+  const long total = (long)rows * cols;
+  for (long base = (long)threadIdx.x * 4; base < total; base += (long)blockDim.x * 4) {
+    zc val[4];
+    long off[4];
+    int rr[4], cc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long e = base + u;
+      rr[u] = (int)(e % rows);
+      cc[u] = (int)(e / rows);
+      off[u] = (long)cc[u] * lda + rr[u];
+      if (e < total) val[u] = A[off[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (base + u < total) {
+        const zc yc = y_conj ? conj(y[cc[u]]) : y[cc[u]];
+        A[off[u]] = val[u] - x[rr[u]] * yc;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // 1. Hessenberg reduction (zgehd2).  A(j+1:n, j) -> beta e1, v stored below, tau[j].
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, long sA, zc* tau) {
+__global__ __launch_bounds__(HH_THREADS) void k_gehrd(int n, zc* A, int lda, long sA, zc* tau) {
   extern __shared__ zc smem[];
   zc* v = smem;           // n
   zc* w = smem + n;       // n
-  __shared__ zc red[EIG_THREADS / 64];
-  __shared__ zc s_tau;
+  zc* part = smem + 2 * n;  // blockDim.x
+  __shared__ zc red[HH_THREADS / 64];
   const int b = blockIdx.x;
   A += sA * b;
   tau += (long)n * b;
@@ -63,7 +154,6 @@ __global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, lo
   for (int j = 0; j < n - 2; ++j) {
     const int m = n - j - 1;  // length of x = A(j+1:n, j)
     zc* x = A + (long)j * lda + j + 1;
-    // ||x(1:)||^2
     double ss = 0.0;
     for (int r = 1 + tid; r < m; r += blockDim.x) ss += norm2(x[r]);
     zc tot = block_sum(mk(ss), red);
@@ -81,7 +171,6 @@ __global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, lo
       for (int r = 1 + tid; r < m; r += blockDim.x) x[r] = x[r] * scal;
     }
     __syncthreads();
-    // v = [1; x(1:)] into LDS
     for (int r = tid; r < m; r += blockDim.x) v[r] = (r == 0) ? mk(1.0) : x[r];
     if (tid == 0) {
       tau[j] = t;
@@ -89,39 +178,14 @@ __global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, lo
     }
     __syncthreads();
     if (!trivial) {
-      // right: A(0:n, j+1:n) -= t (A v) v^H
-      for (int r = tid; r < n; r += blockDim.x) {
-        zc acc = mk(0.0);
-        for (int c = 0; c < m; ++c) fma_acc(acc, A[(long)(j + 1 + c) * lda + r], v[c]);
-        w[r] = acc * t;
-      }
+      // right: A(0:n, j+1:n) -= (t A v) v^H
+      gemv_rows(A + (long)(j + 1) * lda, lda, n, m, v, t, w, part);
+      rank1_sub(A + (long)(j + 1) * lda, lda, n, m, w, v, true);
       __syncthreads();
-      for (int r = tid; r < n; r += blockDim.x) {
-        const zc wr = w[r];
-        for (int c = 0; c < m; ++c) {
-          zc& a = A[(long)(j + 1 + c) * lda + r];
-          a = a - wr * conj(v[c]);
-        }
-      }
+      // left: A(j+1:n, j+1:n) -= v (conj(t) v^H A)
+      gemv_cols_conj(A + (long)(j + 1) * lda + j + 1, lda, m, m, v, conj(t), w);
       __syncthreads();
-      // left: A(j+1:n, j+1:n) -= conj(t) v (v^H A); wavefront per column, lanes down the rows
-      const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
-      for (int c = wid; c < m; c += nw) {
-        const zc* col = A + (long)(j + 1 + c) * lda + j + 1;
-        zc acc = mk(0.0);
-        for (int r = lane; r < m; r += 64) fma_acc(acc, conj(v[r]), col[r]);
-        acc.re = wave_sum(acc.re);
-        acc.im = wave_sum(acc.im);
-        if (lane == 0) w[c] = acc * conj(t);
-      }
-      __syncthreads();
-      for (int r = tid; r < m; r += blockDim.x) {
-        const zc vr = v[r];
-        for (int c = 0; c < m; ++c) {
-          zc& a = A[(long)(j + 1 + c) * lda + j + 1 + r];
-          a = a - vr * w[c];
-        }
-      }
+      rank1_sub(A + (long)(j + 1) * lda + j + 1, lda, m, m, v, w, false);
       __syncthreads();
     }
   }
@@ -131,8 +195,8 @@ __global__ __launch_bounds__(EIG_THREADS) void k_gehrd(int n, zc* A, int lda, lo
 // 2. Q = H(0) H(1) ... H(n-3) applied to the identity (zung2r on the trailing block), then the
 //    Householder vectors below the subdiagonal of A are cleared so that A holds H.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(EIG_THREADS) void k_unghr(int n, zc* A, int lda, long sA, const zc* tau,
-                                                       zc* Z, int ldz, long sZ) {
+__global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, zc* A, int lda, long sA, const zc* tau,
+                                                      zc* Z, int ldz, long sZ) {
   extern __shared__ zc smem[];
   zc* v = smem;      // n
   zc* w = smem + n;  // n
@@ -146,7 +210,6 @@ __global__ __launch_bounds__(EIG_THREADS) void k_unghr(int n, zc* A, int lda, lo
     Z[(long)c * ldz + r] = mk(r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   for (int j = n - 3; j >= 0; --j) {
     const zc t = tau[j];
     if (t.re == 0.0 && t.im == 0.0) continue;
@@ -154,22 +217,10 @@ __global__ __launch_bounds__(EIG_THREADS) void k_unghr(int n, zc* A, int lda, lo
     for (int r = tid; r < m; r += blockDim.x) v[r] = (r == 0) ? mk(1.0) : A[(long)j * lda + j + 1 + r];
     __syncthreads();
     // Z(j+1:n, j+1:n) = (I - t v v^H) Z(j+1:n, j+1:n)
-    for (int c = wid; c < m; c += nw) {
-      const zc* col = Z + (long)(j + 1 + c) * ldz + j + 1;
-      zc acc = mk(0.0);
-      for (int r = lane; r < m; r += 64) fma_acc(acc, conj(v[r]), col[r]);
-      acc.re = wave_sum(acc.re);
-      acc.im = wave_sum(acc.im);
-      if (lane == 0) w[c] = acc * t;
-    }
+    zc* Zb = Z + (long)(j + 1) * ldz + j + 1;
+    gemv_cols_conj(Zb, ldz, m, m, v, t, w);
     __syncthreads();
-    for (int r = tid; r < m; r += blockDim.x) {
-      const zc vr = v[r];
-      for (int c = 0; c < m; ++c) {
-        zc& z = Z[(long)(j + 1 + c) * ldz + j + 1 + r];
-        z = z - vr * w[c];
-      }
-    }
+    rank1_sub(Zb, ldz, m, m, v, w, false);
     __syncthreads();
   }
   for (long e = tid; e < (long)n * n; e += blockDim.x) {
@@ -581,10 +632,10 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   CWX_CHECK(batch <= 65535, "zgeev: batch too large");
   work.ensure(n, batch);
   const long nn = (long)n * n;
-  const size_t sh2 = sizeof(zc) * 2 * (size_t)n;
+  const size_t sh2 = sizeof(zc) * (2 * (size_t)n + HH_THREADS);
   CWX_CHECK(sh2 <= 64 * 1024, "zgeev: n too large for the LDS work vectors");
-  hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(EIG_THREADS), sh2, s, n, A, lda, sA, work.tau.p);
-  hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(EIG_THREADS), sh2, s, n, A, lda, sA, work.tau.p,
+  hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(HH_THREADS), sh2, s, n, A, lda, sA, work.tau.p);
+  hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(HH_THREADS), sh2, s, n, A, lda, sA, work.tau.p,
                      work.Z.p, n, nn);
   hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n, nn, W,
                      sW, info);

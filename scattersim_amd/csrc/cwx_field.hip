@@ -20,9 +20,9 @@ namespace cwx {
 
 namespace {
 
-// origin + step * i rounded as two separate float operations (no fused multiply-add), so that grid
-// coordinates are reproducible bit for bit on the host
-__device__ inline float coord(float o, float d, int i) { return __fadd_rn(o, __fmul_rn(d, (float)i)); }
+// grid coordinate origin + step * i with a single rounding (a fused multiply-add, the form a compiler
+// with contraction gives x_start + d * xi of gpuEvaluator.cu:78-79), reproducible bit for bit on the host
+__device__ inline float coord(float o, float d, int i) { return fmaf(d, (float)i, o); }
 
 constexpr int kFieldThreads = 256;
 

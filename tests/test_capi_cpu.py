@@ -132,10 +132,7 @@ def test_oracle_c_field_restatement():
     g = (-0.2, 0.1, -1.0, 0.05, 0.04, 0.1, 8, 6, 21)
     out64 = np.zeros((g[8], g[7], g[6], 3), np.complex128)
     lib.cw_eval_grid_f64(W.ctypes.data, zl.ctypes.data, b.ctypes.data, e.ctypes.data, 2, 0, *g, out64.ctypes.data)
-    ix, iy, iz = np.meshgrid(np.arange(g[6], dtype=np.float32), np.arange(g[7], dtype=np.float32),
-                             np.arange(g[8], dtype=np.float32), indexing="ij")
-    pts = np.stack([np.float32(g[0]) + np.float32(g[3]) * ix, np.float32(g[1]) + np.float32(g[4]) * iy,
-                    np.float32(g[2]) + np.float32(g[5]) * iz], -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    pts = o.grid_points(*g)
     ref = o.field_points(W, zl, b, e, pts, tol=0.001)
     assert rel_err(out64.reshape(-1, 3), ref) < 1e-12
     W32 = W.astype(np.complex64)

@@ -32,13 +32,7 @@ def to_struct(W, dtype):
     return out
 
 
-def grid_points(x0, y0, z0, dx, dy, dz, nx, ny, nz):
-    ix = np.arange(nx, dtype=np.float32)
-    iy = np.arange(ny, dtype=np.float32)
-    iz = np.arange(nz, dtype=np.float32)
-    Z, Y, X = np.meshgrid(np.float32(z0) + np.float32(dz) * iz, np.float32(y0) + np.float32(dy) * iy,
-                          np.float32(x0) + np.float32(dx) * ix, indexing="ij")
-    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+grid_points = o.grid_points
 
 
 @pytest.mark.parametrize("tol_mode", [0, 1])
