@@ -301,14 +301,212 @@ __device__ inline bool two_small(const zc* H, int ldh, int m, zc t) {
   return fabs(h10) * fabs(h21) <= kUlp * (abs1(h11s) * (abs1(h11) + abs1(h22)));
 }
 
+// ------------------------------------------------------------------------------------------------
+// Multishift sweeps.  A chain of up to MS_NB single-shift bulges, introduced two rows apart at the top
+// of the active block and chased down in lockstep: bulges two rows apart touch disjoint row pairs
+// and column pairs, so within a step every left transformation commutes with every other left one,
+// right ones likewise, and a left with a right; each bulge reads only entries no trailing bulge has
+// written yet.  Each step is therefore three phases (reflectors; left products plus the bulge
+// column; right products), and the result is the sequence of MS_NB single-shift sweeps with shifts
+// fixed before the first one.  The chain moves through windows of MS_S steps held in LDS; the
+// window's rotations are then applied to the rest of H and to Z from registers.
+// ------------------------------------------------------------------------------------------------
+constexpr int MS_NB = 16;                               // bulges (shifts) per sweep
+constexpr int MS_S = 16;                                // chase steps per window
+constexpr int MS_LEN = MS_S + 2 * (MS_NB - 1) + 2;      // rows spanned by one window (48)
+constexpr int MS_NMIN = 72;                             // smallest active block swept this way
+
+struct Rot2 {
+  zc t1, v2, t2;  // reflector I - t1 v v^H with v = (1, v2); t2 = t1 v2
+};
+
+// 2-element Householder (zlarfg, n = 2) for a general complex second entry.
+__device__ inline Rot2 make_reflector2(zc& alpha, zc x) {
+  Rot2 g;
+  const double xnorm = absz(x);
+  if (xnorm == 0.0 && alpha.im == 0.0) {
+    g.t1 = g.v2 = g.t2 = mk(0.0);
+    return g;
+  }
+  const double beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
+  g.t1 = mk((beta - alpha.re) / beta, -alpha.im / beta);
+  g.v2 = x * zdiv(mk(1.0), alpha - mk(beta));
+  alpha = mk(beta);
+  g.t2 = g.t1 * g.v2;
+  return g;
+}
+__device__ inline void rot2_left(const Rot2& g, zc& a, zc& b) {
+  const zc s = conj(g.t1) * a + conj(g.t2) * b;
+  a = a - s;
+  b = b - s * g.v2;
+}
+__device__ inline void rot2_right(const Rot2& g, zc& a, zc& b) {
+  const zc s = g.t1 * a + g.t2 * b;
+  a = a - s;
+  b = b - s * conj(g.v2);
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Deflation test of zlahqr with |h| = |re| + |im| for the subdiagonal entries (they may be complex).
+__device__ inline bool negligible_c(const zc* S, int ld, int k, int lo, int hi, double smlnum) {
+  auto h = [&](int r, int c) { return S[(long)c * ld + r]; };
+  const zc hkk1 = h(k, k - 1);
+  if (abs1(hkk1) <= smlnum) return true;
+  double tst = abs1(h(k - 1, k - 1)) + abs1(h(k, k));
+  if (tst == 0.0) {
+    if (k - 2 >= lo) tst += abs1(h(k - 1, k - 2));
+    if (k + 1 <= hi) tst += abs1(h(k + 1, k));
+  }
+  if (abs1(hkk1) <= kUlp * tst) {
+    const double ab = fmax(abs1(hkk1), abs1(h(k - 1, k)));
+    const double ba = fmin(abs1(hkk1), abs1(h(k - 1, k)));
+    const double aa = fmax(abs1(h(k, k)), abs1(h(k - 1, k - 1) - h(k, k)));
+    const double bb = fmin(abs1(h(k, k)), abs1(h(k - 1, k - 1) - h(k, k)));
+    const double s = aa + ab;
+    if (ba * (ab / s) <= fmax(smlnum, kUlp * (bb * (aa / s)))) return true;
+  }
+  return false;
+}
+
+// Eigenvalues of the m x m upper Hessenberg S (LDS, column-major, m <= 64) by single-shift QR
+// without Schur form or vectors (zlahqr with wantt = wantz = false), run by one wavefront.
+__device__ bool small_eigvals_wave(zc* S, int m, zc* w, double smlnum) {
+  const int lane = threadIdx.x & 63;
+  auto Sp = [&](int r, int c) -> zc& { return S[c * m + r]; };
+  int i = m - 1, kdefl = 0, its = 0;
+  const int itmax = 30 * max(10, m) * m;
+  while (i >= 0) {
+    if (++its > itmax) return false;
+    const int k = lane + 1;
+    const bool neg = (k <= i) && negligible_c(S, m, k, 0, m - 1, smlnum);
+    const unsigned long long mask = __ballot(neg);
+    const int l = mask ? (64 - __clzll(mask)) : 0;  // largest k with a negligible subdiagonal
+    wave_sync();
+    if (l > 0 && lane == 0) Sp(l, l - 1) = mk(0.0);
+    wave_sync();
+    if (l >= i) {
+      if (lane == 0) w[i] = Sp(i, i);
+      i = l - 1;
+      kdefl = 0;
+      wave_sync();
+      continue;
+    }
+    ++kdefl;
+    zc t;
+    if (kdefl % 20 == 0) {
+      t = mk(0.75 * abs1(Sp(i, i - 1))) + Sp(i, i);
+    } else if (kdefl % 10 == 0) {
+      t = mk(0.75 * abs1(Sp(l + 1, l))) + Sp(l, l);
+    } else {
+      t = Sp(i, i);
+      const zc u = zsqrt(Sp(i - 1, i)) * zsqrt(Sp(i, i - 1));
+      double s = abs1(u);
+      if (s != 0.0) {
+        const zc x = 0.5 * (Sp(i - 1, i - 1) - t);
+        const double sx = abs1(x);
+        s = fmax(s, sx);
+        const zc xs = x * (1.0 / s), us = u * (1.0 / s);
+        zc y = s * zsqrt(xs * xs + us * us);
+        if (sx > 0.0) {
+          const zc xn = x * (1.0 / sx);
+          if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
+        }
+        t = t - u * zdiv(u, x + y);
+      }
+    }
+    for (int kk = l; kk < i; ++kk) {
+      zc v1, v2;
+      if (kk == l) {
+        const zc h11s = Sp(l, l) - t, h21 = Sp(l + 1, l);
+        const double sc = abs1(h11s) + abs1(h21);
+        v1 = h11s * (1.0 / sc);
+        v2 = h21 * (1.0 / sc);
+      } else {
+        v1 = Sp(kk, kk - 1);
+        v2 = Sp(kk + 1, kk - 1);
+      }
+      const Rot2 g = make_reflector2(v1, v2);
+      wave_sync();
+      for (int c = kk + lane; c <= i; c += 64) rot2_left(g, Sp(kk, c), Sp(kk + 1, c));
+      if (kk > l && lane == 0) {
+        Sp(kk, kk - 1) = v1;
+        Sp(kk + 1, kk - 1) = mk(0.0);
+      }
+      wave_sync();
+      const int rmax = min(kk + 2, i);
+      for (int r = l + lane; r <= rmax; r += 64) rot2_right(g, Sp(r, kk), Sp(r, kk + 1));
+      wave_sync();
+    }
+  }
+  return true;
+}
+
+// Make the subdiagonal H(L+1:I, L:I-1) real by the diagonal unitary similarity D = diag(d_L..d_I),
+// d_i = d_{i-1} h_i / |h_i| for complex h_i (d_i = d_{i-1} for real h_i), applied to H (rows L..I
+// from the subdiagonal rightwards, rows above L in columns L..I) and to Z.  Real subdiagonal entries
+// keep their value, as in zlahqr's preparatory loop (a sequential scan whose scalings compound the
+// same way).  d: LDS scratch of I - L + 1 entries.  Called by the whole workgroup.
+__device__ void real_subdiag(zc* H, int ldh, zc* Z, int ldz, int n, int L, int I, zc* d) {
+  __shared__ int s_first;
+  auto Hp = [&](int r, int c) -> zc& { return H[(long)c * ldh + r]; };
+  if (threadIdx.x == 0) s_first = I + 1;
+  __syncthreads();
+  for (int i = L + 1 + threadIdx.x; i <= I; i += blockDim.x)
+    if (Hp(i, i - 1).im != 0.0) atomicMin(&s_first, i);
+  __syncthreads();
+  const int f = s_first;
+  __syncthreads();
+  if (f > I) return;
+  const int m = I - f + 1;
+  if (threadIdx.x == 0) {
+    zc acc = mk(1.0);
+    for (int i = f; i <= I; ++i) {
+      const zc h = Hp(i, i - 1);
+      if (h.im != 0.0) acc = acc * (h * (1.0 / absz(h)));
+      d[i - f] = acc;
+    }
+  }
+  __syncthreads();
+  for (long e = threadIdx.x; e < (long)m * (n - f + 1); e += blockDim.x) {
+    const int r = f + (int)(e % m), c = f - 1 + (int)(e / m);
+    if (c < r - 1) continue;
+    zc& h = Hp(r, c);
+    if (c == r - 1) {
+      if (h.im != 0.0) h = mk(absz(h));
+      continue;
+    }
+    const zc dc = (c <= I) ? d[c - f] : mk(1.0);
+    h = conj(d[r - f]) * h * dc;
+  }
+  for (long e = threadIdx.x; e < (long)f * m; e += blockDim.x) {
+    const int r = (int)(e % f), c = f + (int)(e / f);
+    Hp(r, c) = Hp(r, c) * d[c - f];
+  }
+  for (long e = threadIdx.x; e < (long)n * m; e += blockDim.x) {
+    const int r = (int)(e % n), c = f + (int)(e / n);
+    zc& z = Z[(long)c * ldz + r];
+    z = z * d[c - f];
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
                                                      long sZ, zc* W, long sW, int* info) {
   constexpr int BR = QR_W + 2, BC = QR_W + 3;
-  __shared__ zc B[BR * BC];   // window block, column-major with leading dimension BR
+  static_assert(BR * BC <= MS_LEN * (MS_LEN + 1), "window buffers share one LDS array");
+  __shared__ zc MB[MS_LEN * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
   __shared__ Rot rots[QR_W];
+  __shared__ Rot2 rots2[MS_S * MS_NB];
+  __shared__ zc shifts[MS_NB], betas[MS_NB];
   __shared__ int s_k, s_m;
   __shared__ zc s_t, s_v1;
   __shared__ double s_v2;
+  zc* B = MB;
   const int b = blockIdx.x;
   H += sH * b;
   Z += sZ * b;
@@ -323,217 +521,370 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
   }
   const int ilo = 0, ihi = n - 1;
   const double smlnum = kSafmin * ((double)n / kUlp);
-
-  // make the subdiagonal real (sequential in i: each scaling feeds the next test)
-  for (int i = ilo + 1; i <= ihi; ++i) {
-    __syncthreads();
-    const zc hi = Hp(i, i - 1);
-    if (hi.im != 0.0) {
-      zc sc = hi * (1.0 / abs1(hi));
-      sc = conj(sc) * (1.0 / absz(sc));
-      const double ah = absz(hi);
+  constexpr int kScratch = MS_LEN * (MS_LEN + 1);
+  if (n <= kScratch) {
+    real_subdiag(H, ldh, Z, ldz, n, ilo, ihi, MB);
+  } else {
+    // the prefix-product scratch does not fit: zlahqr's sequential scan
+    for (int i = ilo + 1; i <= ihi; ++i) {
       __syncthreads();
-      // row i, cols i..n-1 ; col i, rows 0..min(n-1, i+1) ; Z col i
-      for (int c = i + tid; c < n; c += nt) Hp(i, c) = Hp(i, c) * sc;
-      __syncthreads();
-      for (int r = tid; r <= min(n - 1, i + 1); r += nt) Hp(r, i) = Hp(r, i) * conj(sc);
-      for (int r = tid; r < n; r += nt) Zp(r, i) = Zp(r, i) * conj(sc);
-      __syncthreads();
-      if (tid == 0) Hp(i, i - 1) = mk(ah);
+      const zc hi = Hp(i, i - 1);
+      if (hi.im != 0.0) {
+        zc sc = hi * (1.0 / abs1(hi));
+        sc = conj(sc) * (1.0 / absz(sc));
+        const double ah = absz(hi);
+        __syncthreads();
+        for (int c = i + tid; c < n; c += nt) Hp(i, c) = Hp(i, c) * sc;
+        __syncthreads();
+        for (int r = tid; r <= min(n - 1, i + 1); r += nt) Hp(r, i) = Hp(r, i) * conj(sc);
+        for (int r = tid; r < n; r += nt) Zp(r, i) = Zp(r, i) * conj(sc);
+        __syncthreads();
+        if (tid == 0) Hp(i, i - 1) = mk(ah);
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   const int itmax = 30 * max(10, ihi - ilo + 1);
-  int kdefl = 0;
-  int I = ihi;
+  int kdefl = 0, its = 0, ms_fail = 0, lastL = -1, lastI = -1;
+  int I = ihi, L = ilo;
   while (I >= ilo) {
-    int L = ilo;
-    bool converged = false;
-    for (int its = 0; its <= itmax; ++its) {
-      // ---- look for a single small subdiagonal element (largest k in [L+1, I]) ----
-      if (tid == 0) s_k = L;
+    // ---- look for a single small subdiagonal element (largest k in [L+1, I]) ----
+    if (tid == 0) s_k = L;
+    __syncthreads();
+    int found = L;
+    for (int k = L + 1 + tid; k <= I; k += nt)
+      if (subdiag_negligible(H, ldh, k, ilo, ihi, smlnum)) found = max(found, k);
+    if (found > L) atomicMax(&s_k, found);
+    __syncthreads();
+    L = s_k;
+    if (L > ilo && tid == 0) Hp(L, L - 1) = mk(0.0);
+    __syncthreads();
+    if (L >= I) {
+      // H(I, I-1) negligible: one eigenvalue converged
+      if (tid == 0) W[I] = Hp(I, I);
+      kdefl = 0;
+      its = 0;
+      I = L - 1;
+      L = ilo;
       __syncthreads();
-      int found = L;
-      for (int k = L + 1 + tid; k <= I; k += nt)
-        if (subdiag_negligible(H, ldh, k, ilo, ihi, smlnum)) found = max(found, k);
-      if (found > L) atomicMax(&s_k, found);
-      __syncthreads();
-      L = s_k;
-      if (L > ilo && tid == 0) Hp(L, L - 1) = mk(0.0);
-      __syncthreads();
-      if (L >= I) { converged = true; break; }
-      ++kdefl;
-      // ---- shift ----
-      if (tid == 0) {
-        zc t;
-        if (kdefl % 20 == 0) {
-          t = mk(0.75 * fabs(Hp(I, I - 1).re)) + Hp(I, I);
-        } else if (kdefl % 10 == 0) {
-          t = mk(0.75 * fabs(Hp(L + 1, L).re)) + Hp(L, L);
-        } else {
-          t = Hp(I, I);
-          zc u = zsqrt(Hp(I - 1, I)) * zsqrt(Hp(I, I - 1));
-          double s = abs1(u);
-          if (s != 0.0) {
-            zc x = 0.5 * (Hp(I - 1, I - 1) - t);
-            const double sx = abs1(x);
-            s = fmax(s, abs1(x));
-            zc xs = x * (1.0 / s), us = u * (1.0 / s);
-            zc y = s * zsqrt(xs * xs + us * us);
-            if (sx > 0.0) {
-              zc xn = x * (1.0 / sx);
-              if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
-            }
-            t = t - u * zdiv(u, x + y);
-          }
-        }
-        s_t = t;
-        s_m = L;
-      }
-      __syncthreads();
-      const zc tshift = s_t;
-      // ---- look for two consecutive small subdiagonal elements (largest m in [L+1, I-1]) ----
-      int mf = L;
-      for (int m = L + 1 + tid; m <= I - 1; m += nt)
-        if (two_small(H, ldh, m, tshift)) mf = max(mf, m);
-      if (mf > L) atomicMax(&s_m, mf);
-      __syncthreads();
-      const int M = s_m;
-      if (tid == 0) {
-        const zc h11 = Hp(M, M);
-        zc h11s = h11 - tshift;
-        double h21 = Hp(M + 1, M).re;
-        const double s = abs1(h11s) + fabs(h21);
-        s_v1 = h11s * (1.0 / s);
-        s_v2 = h21 / s;
-      }
-      __syncthreads();
-
-      // ---- single-shift QR sweep k = M .. I-1, in windows ----
-      int ks = M;
-      while (ks < I) {
-        // the first rotation of a sweep started at M > L is followed by a diagonal scaling;
-        // it gets a window of its own
-        const int wlen = (ks == M && M > L) ? 1 : min(QR_W, I - ks);
-        const int ke = ks + wlen;                     // rotations ks .. ke-1
-        const int re = min(ke + 1, I);                // last block row / column
-        const int cs = (ks > M) ? ks - 1 : ks;        // first block column
-        const int nr = re - ks + 1, nc = re - cs + 1;
-        for (int e = tid; e < nr * nc; e += nt) {
-          int r = e % nr, c = e / nr;
-          B[c * BR + r] = Hp(ks + r, cs + c);
-        }
-        __syncthreads();
-        auto Bp = [&](int r, int c) -> zc& { return B[(c - cs) * BR + (r - ks)]; };
-        for (int k = ks; k < ke; ++k) {
-          zc v1;
-          zc v2c;
-          if (k == M) {
-            v1 = s_v1;
-            v2c = mk(s_v2);
-          } else {
-            v1 = Bp(k, k - 1);
-            v2c = Bp(k + 1, k - 1);
-          }
-          // every lane forms the same reflector from the same LDS values; column k-1 is rewritten
-          // only in the right phase below, after the barrier
-          const Rot g = make_reflector(v1, v2c);
-          // left: rows k, k+1, cols k..re
-          for (int c = k + tid; c <= re; c += nt) rot_left(g, Bp(k, c), Bp(k + 1, c));
-          __syncthreads();
-          // right: cols k, k+1, rows ks..min(k+2, I)
-          const int rmax = min(k + 2, I);
-          for (int r = ks + tid; r <= rmax; r += nt) rot_right(g, Bp(r, k), Bp(r, k + 1));
-          if (tid == 0) {
-            if (k > M) {
-              Bp(k, k - 1) = v1;
-              Bp(k + 1, k - 1) = mk(0.0);
-            }
-            rots[k - ks] = g;
-          }
-          __syncthreads();
-        }
-        for (int e = tid; e < nr * nc; e += nt) {
-          int r = e % nr, c = e / nr;
-          Hp(ks + r, cs + c) = B[c * BR + r];
-        }
-        // deferred: far-right columns (re, n) on rows ks..ke; far-above rows [0, ks) on cols ks..ke;
-        // Schur vectors Z(:, ks..ke)
-        const int nfr = n - 1 - re, nfa = ks, nz = n;
-        const int lines = nfr + nfa + nz;
-        for (int ln = tid; ln < lines; ln += nt) {
-          zc x[QR_W + 1];
-          if (ln < nfr) {
-            const int c = re + 1 + ln;
-#pragma unroll
-            for (int j = 0; j <= QR_W; ++j)
-              if (j <= wlen) x[j] = Hp(ks + j, c);
-#pragma unroll
-            for (int j = 0; j < QR_W; ++j)
-              if (j < wlen) rot_left(rots[j], x[j], x[j + 1]);
-#pragma unroll
-            for (int j = 0; j <= QR_W; ++j)
-              if (j <= wlen) Hp(ks + j, c) = x[j];
-          } else {
-            const bool isz = ln >= nfr + nfa;
-            const int r = isz ? ln - nfr - nfa : ln - nfr;
-            zc* base = isz ? (Z + (long)ks * ldz + r) : (H + (long)ks * ldh + r);
-            const int ld = isz ? ldz : ldh;
-#pragma unroll
-            for (int j = 0; j <= QR_W; ++j)
-              if (j <= wlen) x[j] = base[(long)j * ld];
-#pragma unroll
-            for (int j = 0; j < QR_W; ++j)
-              if (j < wlen) rot_right(rots[j], x[j], x[j + 1]);
-#pragma unroll
-            for (int j = 0; j <= QR_W; ++j)
-              if (j <= wlen) base[(long)j * ld] = x[j];
-          }
-        }
-        __syncthreads();
-        if (ks == M && M > L) {
-          // extra scaling keeping H(M, M-1) real (zlahqr)
-          const Rot g = rots[0];
-          zc temp = mk(1.0) - g.t1;
-          temp = temp * (1.0 / absz(temp));
-          if (tid == 0) {
-            Hp(M + 1, M) = Hp(M + 1, M) * conj(temp);
-            if (M + 2 <= I) Hp(M + 2, M + 1) = Hp(M + 2, M + 1) * temp;
-          }
-          __syncthreads();
-          for (int J = M; J <= I; ++J) {
-            if (J == M + 1) continue;
-            for (int c = J + 1 + tid; c < n; c += nt) Hp(J, c) = Hp(J, c) * temp;
-            __syncthreads();
-            for (int r = tid; r < J; r += nt) Hp(r, J) = Hp(r, J) * conj(temp);
-            for (int r = tid; r < n; r += nt) Zp(r, J) = Zp(r, J) * conj(temp);
-            __syncthreads();
-          }
-        }
-        ks = ke;
-      }
-      // ---- ensure H(I, I-1) is real ----
-      const zc tmp = Hp(I, I - 1);
-      __syncthreads();
-      if (tmp.im != 0.0) {
-        const double rt = absz(tmp);
-        const zc tt = tmp * (1.0 / rt);
-        for (int c = I + 1 + tid; c < n; c += nt) Hp(I, c) = Hp(I, c) * conj(tt);
-        for (int r = tid; r < I; r += nt) Hp(r, I) = Hp(r, I) * tt;
-        for (int r = tid; r < n; r += nt) Zp(r, I) = Zp(r, I) * tt;
-        __syncthreads();
-        if (tid == 0) Hp(I, I - 1) = mk(rt);
-      }
-      __syncthreads();
+      continue;
     }
-    if (!converged) {
+    if (++its > itmax) {
       if (tid == 0) info[b] = I + 1;
       return;
     }
-    if (tid == 0) W[I] = Hp(I, I);
-    kdefl = 0;
-    I = L - 1;
+    const int m = I - L + 1;
+    if (L != lastL || I != lastI) ms_fail = 0;
+    if (m >= MS_NMIN && n <= kScratch && ms_fail < 3) {
+      // ================= multishift sweep =================
+      lastL = L;
+      lastI = I;
+      ++ms_fail;
+      const int nb = MS_NB;
+      if (tid < 64) {
+        zc* S = MB;
+        for (int e = tid; e < nb * nb; e += 64) {
+          const int r = e % nb, c = e / nb;
+          S[e] = (r <= c + 1) ? Hp(I - nb + 1 + r, I - nb + 1 + c) : mk(0.0);
+        }
+        wave_sync();
+        if (!small_eigvals_wave(S, nb, shifts, smlnum)) {
+          wave_sync();
+          for (int j = tid; j < nb; j += 64) shifts[j] = Hp(I - nb + 1 + j, I - nb + 1 + j);
+        }
+      }
+      __syncthreads();
+      const int T_total = (I - 1 - L) + 2 * (nb - 1) + 1;  // steps t = 0 .. T_total - 1
+      for (int t0 = 0; t0 < T_total; t0 += MS_S) {
+        const int kbase = L + t0 - 2 * (MS_NB - 1);
+        const int rlo = max(kbase, L), rhi = min(kbase + MS_LEN - 1, I);
+        const int clo = max(kbase - 1, L);
+        auto MBp = [&](int r, int c) -> zc& { return MB[(c - kbase + 1) * MS_LEN + (r - kbase)]; };
+        const int nr = rhi - rlo + 1, nc = rhi - clo + 1;
+        for (int e = tid; e < nr * nc; e += nt) {
+          const int r = rlo + e % nr, c = clo + e / nr;
+          MBp(r, c) = Hp(r, c);
+        }
+        __syncthreads();
+        const int nsteps = min(MS_S, T_total - t0);
+        for (int s = 0; s < nsteps; ++s) {
+          const int t = t0 + s;
+          if (tid < MS_NB) {
+            const int j = tid;
+            const int k = L + t - 2 * j;
+            Rot2 g;
+            g.t1 = g.v2 = g.t2 = mk(0.0);
+            zc beta = mk(0.0);
+            if (j < nb && k >= L && k <= I - 1) {
+              zc v1, v2;
+              if (k == L) {
+                const zc h11s = MBp(L, L) - shifts[j], h21 = MBp(L + 1, L);
+                const double sc = abs1(h11s) + abs1(h21);
+                v1 = h11s * (1.0 / sc);
+                v2 = h21 * (1.0 / sc);
+              } else {
+                v1 = MBp(k, k - 1);
+                v2 = MBp(k + 1, k - 1);
+              }
+              g = make_reflector2(v1, v2);
+              beta = v1;
+            }
+            rots2[s * MS_NB + j] = g;
+            betas[j] = beta;
+          }
+          __syncthreads();
+          // left products on rows k, k+1 (columns k..rhi) and the bulge column of chase steps
+          for (int e = tid; e < MS_NB * MS_LEN; e += nt) {
+            const int j = e / MS_LEN, ci = e % MS_LEN;
+            const int k = L + t - 2 * j;
+            if (j >= nb || k < L || k > I - 1) continue;
+            const int c = k + ci;
+            if (c > rhi) continue;
+            rot2_left(rots2[s * MS_NB + j], MBp(k, c), MBp(k + 1, c));
+            if (ci == 0 && k > L) {
+              MBp(k, k - 1) = betas[j];
+              MBp(k + 1, k - 1) = mk(0.0);
+            }
+          }
+          __syncthreads();
+          // right products on columns k, k+1 (rows rlo..min(k+2, I))
+          for (int e = tid; e < MS_NB * MS_LEN; e += nt) {
+            const int j = e / MS_LEN, ri = e % MS_LEN;
+            const int k = L + t - 2 * j;
+            if (j >= nb || k < L || k > I - 1) continue;
+            const int r = rlo + ri;
+            if (r > min(k + 2, I)) continue;
+            rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
+          }
+          __syncthreads();
+        }
+        for (int e = tid; e < nr * nc; e += nt) {
+          const int r = rlo + e % nr, c = clo + e / nr;
+          Hp(r, c) = MBp(r, c);
+        }
+        // deferred products: columns right of the window (left rotations on rows kbase + q),
+        // rows above it and all rows of Z (right rotations on columns kbase + q)
+        const int nfr = n - 1 - rhi, nfa = rlo, nz = n;
+        const int lines = nfr + nfa + nz;
+        for (int ln = tid; ln < lines; ln += nt) {
+          zc x[MS_LEN];
+          if (ln < nfr) {
+            const int c = rhi + 1 + ln;
+#pragma unroll
+            for (int q = 0; q < MS_LEN; ++q) {
+              const int r = kbase + q;
+              x[q] = (r >= rlo && r <= rhi) ? Hp(r, c) : mk(0.0);
+            }
+#pragma unroll
+            for (int s = 0; s < MS_S; ++s) {
+              if (s < nsteps) {
+#pragma unroll
+                for (int j = 0; j < MS_NB; ++j) {
+                  const int q = 2 * (MS_NB - 1) + s - 2 * j;
+                  rot2_left(rots2[s * MS_NB + j], x[q], x[q + 1]);
+                }
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < MS_LEN; ++q) {
+              const int r = kbase + q;
+              if (r >= rlo && r <= rhi) Hp(r, c) = x[q];
+            }
+          } else {
+            const bool isz = ln >= nfr + nfa;
+            const int r = isz ? ln - nfr - nfa : ln - nfr;
+            zc* base = isz ? (Z + r) : (H + r);
+            const int ld = isz ? ldz : ldh;
+#pragma unroll
+            for (int q = 0; q < MS_LEN; ++q) {
+              const int c = kbase + q;
+              x[q] = (c >= L && c <= I) ? base[(long)c * ld] : mk(0.0);
+            }
+#pragma unroll
+            for (int s = 0; s < MS_S; ++s) {
+              if (s < nsteps) {
+#pragma unroll
+                for (int j = 0; j < MS_NB; ++j) {
+                  const int q = 2 * (MS_NB - 1) + s - 2 * j;
+                  rot2_right(rots2[s * MS_NB + j], x[q], x[q + 1]);
+                }
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < MS_LEN; ++q) {
+              const int c = kbase + q;
+              if (c >= L && c <= I) base[(long)c * ld] = x[q];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      real_subdiag(H, ldh, Z, ldz, n, L, I, MB);
+      continue;
+    }
+    // ================= single-shift iteration (zlahqr) =================
+    ++kdefl;
+    if (tid == 0) {
+      zc t;
+      if (kdefl % 20 == 0) {
+        t = mk(0.75 * fabs(Hp(I, I - 1).re)) + Hp(I, I);
+      } else if (kdefl % 10 == 0) {
+        t = mk(0.75 * fabs(Hp(L + 1, L).re)) + Hp(L, L);
+      } else {
+        t = Hp(I, I);
+        zc u = zsqrt(Hp(I - 1, I)) * zsqrt(Hp(I, I - 1));
+        double s = abs1(u);
+        if (s != 0.0) {
+          zc x = 0.5 * (Hp(I - 1, I - 1) - t);
+          const double sx = abs1(x);
+          s = fmax(s, abs1(x));
+          zc xs = x * (1.0 / s), us = u * (1.0 / s);
+          zc y = s * zsqrt(xs * xs + us * us);
+          if (sx > 0.0) {
+            zc xn = x * (1.0 / sx);
+            if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
+          }
+          t = t - u * zdiv(u, x + y);
+        }
+      }
+      s_t = t;
+      s_m = L;
+    }
+    __syncthreads();
+    const zc tshift = s_t;
+    // ---- look for two consecutive small subdiagonal elements (largest m in [L+1, I-1]) ----
+    int mf = L;
+    for (int mm = L + 1 + tid; mm <= I - 1; mm += nt)
+      if (two_small(H, ldh, mm, tshift)) mf = max(mf, mm);
+    if (mf > L) atomicMax(&s_m, mf);
+    __syncthreads();
+    const int M = s_m;
+    if (tid == 0) {
+      const zc h11 = Hp(M, M);
+      zc h11s = h11 - tshift;
+      double h21 = Hp(M + 1, M).re;
+      const double s = abs1(h11s) + fabs(h21);
+      s_v1 = h11s * (1.0 / s);
+      s_v2 = h21 / s;
+    }
+    __syncthreads();
+
+    // ---- single-shift QR sweep k = M .. I-1, in windows ----
+    int ks = M;
+    while (ks < I) {
+      // the first rotation of a sweep started at M > L is followed by a diagonal scaling;
+      // it gets a window of its own
+      const int wlen = (ks == M && M > L) ? 1 : min(QR_W, I - ks);
+      const int ke = ks + wlen;                     // rotations ks .. ke-1
+      const int re = min(ke + 1, I);                // last block row / column
+      const int cs = (ks > M) ? ks - 1 : ks;        // first block column
+      const int nr = re - ks + 1, nc = re - cs + 1;
+      for (int e = tid; e < nr * nc; e += nt) {
+        int r = e % nr, c = e / nr;
+        B[c * BR + r] = Hp(ks + r, cs + c);
+      }
+      __syncthreads();
+      auto Bp = [&](int r, int c) -> zc& { return B[(c - cs) * BR + (r - ks)]; };
+      for (int k = ks; k < ke; ++k) {
+        zc v1;
+        zc v2c;
+        if (k == M) {
+          v1 = s_v1;
+          v2c = mk(s_v2);
+        } else {
+          v1 = Bp(k, k - 1);
+          v2c = Bp(k + 1, k - 1);
+        }
+        // every lane forms the same reflector from the same LDS values; column k-1 is rewritten
+        // only in the right phase below, after the barrier
+        const Rot g = make_reflector(v1, v2c);
+        // left: rows k, k+1, cols k..re
+        for (int c = k + tid; c <= re; c += nt) rot_left(g, Bp(k, c), Bp(k + 1, c));
+        __syncthreads();
+        // right: cols k, k+1, rows ks..min(k+2, I)
+        const int rmax = min(k + 2, I);
+        for (int r = ks + tid; r <= rmax; r += nt) rot_right(g, Bp(r, k), Bp(r, k + 1));
+        if (tid == 0) {
+          if (k > M) {
+            Bp(k, k - 1) = v1;
+            Bp(k + 1, k - 1) = mk(0.0);
+          }
+          rots[k - ks] = g;
+        }
+        __syncthreads();
+      }
+      for (int e = tid; e < nr * nc; e += nt) {
+        int r = e % nr, c = e / nr;
+        Hp(ks + r, cs + c) = B[c * BR + r];
+      }
+      // deferred: far-right columns (re, n) on rows ks..ke; far-above rows [0, ks) on cols ks..ke;
+      // Schur vectors Z(:, ks..ke)
+      const int nfr = n - 1 - re, nfa = ks, nz = n;
+      const int lines = nfr + nfa + nz;
+      for (int ln = tid; ln < lines; ln += nt) {
+        zc x[QR_W + 1];
+        if (ln < nfr) {
+          const int c = re + 1 + ln;
+#pragma unroll
+          for (int j = 0; j <= QR_W; ++j)
+            if (j <= wlen) x[j] = Hp(ks + j, c);
+#pragma unroll
+          for (int j = 0; j < QR_W; ++j)
+            if (j < wlen) rot_left(rots[j], x[j], x[j + 1]);
+#pragma unroll
+          for (int j = 0; j <= QR_W; ++j)
+            if (j <= wlen) Hp(ks + j, c) = x[j];
+        } else {
+          const bool isz = ln >= nfr + nfa;
+          const int r = isz ? ln - nfr - nfa : ln - nfr;
+          zc* base = isz ? (Z + (long)ks * ldz + r) : (H + (long)ks * ldh + r);
+          const int ld = isz ? ldz : ldh;
+#pragma unroll
+          for (int j = 0; j <= QR_W; ++j)
+            if (j <= wlen) x[j] = base[(long)j * ld];
+#pragma unroll
+          for (int j = 0; j < QR_W; ++j)
+            if (j < wlen) rot_right(rots[j], x[j], x[j + 1]);
+#pragma unroll
+          for (int j = 0; j <= QR_W; ++j)
+            if (j <= wlen) base[(long)j * ld] = x[j];
+        }
+      }
+      __syncthreads();
+      if (ks == M && M > L) {
+        // extra scaling keeping H(M, M-1) real (zlahqr)
+        const Rot g = rots[0];
+        zc temp = mk(1.0) - g.t1;
+        temp = temp * (1.0 / absz(temp));
+        if (tid == 0) {
+          Hp(M + 1, M) = Hp(M + 1, M) * conj(temp);
+          if (M + 2 <= I) Hp(M + 2, M + 1) = Hp(M + 2, M + 1) * temp;
+        }
+        __syncthreads();
+        for (int J = M; J <= I; ++J) {
+          if (J == M + 1) continue;
+          for (int c = J + 1 + tid; c < n; c += nt) Hp(J, c) = Hp(J, c) * temp;
+          __syncthreads();
+          for (int r = tid; r < J; r += nt) Hp(r, J) = Hp(r, J) * conj(temp);
+          for (int r = tid; r < n; r += nt) Zp(r, J) = Zp(r, J) * conj(temp);
+          __syncthreads();
+        }
+      }
+      ks = ke;
+    }
+    // ---- ensure H(I, I-1) is real ----
+    const zc tmp = Hp(I, I - 1);
+    __syncthreads();
+    if (tmp.im != 0.0) {
+      const double rt = absz(tmp);
+      const zc tt = tmp * (1.0 / rt);
+      for (int c = I + 1 + tid; c < n; c += nt) Hp(I, c) = Hp(I, c) * conj(tt);
+      for (int r = tid; r < I; r += nt) Hp(r, I) = Hp(r, I) * tt;
+      for (int r = tid; r < n; r += nt) Zp(r, I) = Zp(r, I) * tt;
+      __syncthreads();
+      if (tid == 0) Hp(I, I - 1) = mk(rt);
+    }
     __syncthreads();
   }
 }
