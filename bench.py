@@ -4,10 +4,12 @@
 
 One step = one scattervolume solve of the configuration's sample (already resident in HBM) through
 libcwx: Fourier coefficients, Phi, per-layer eigendecomposition, transfer chain, boundary LU, beta,
-Q/Ez rows and the .cw records on the device.  `value` is solves/s on one GPU.  The field evaluation
-of the solution on the configuration's grid is timed separately (field_gpts_per_s).  The roofline
-entry covers the eigensolver stage, timed with HIP events on the stream it runs on.  cpu_baseline
-times the oracle (numpy/MKL FP64 restatement of the reference algorithm) on a bounded sample.
+Q/Ez rows and the .cw records on the device.  `value` is solves/s on one GPU.  The field of the
+last solution on the configuration's grid is timed separately ("field": separable synthesis,
+complex64 out, with its error against the direct FP64 plane-wave sum).  The roofline entry covers
+the eigensolver stage (cwx_zgeev_batched_dev over all layers), timed with HIP events on the stream
+it runs on.  cpu_baseline times the oracle (numpy/MKL FP64 restatement of the reference algorithm)
+on a bounded sample, and the oracle's C field sum on a few planes.
 """
 from __future__ import annotations
 
@@ -24,6 +26,12 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+
+# HBM bytes per eigensolver-stage launch, (workload, eig_mode) -> bytes: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE summed over the stage's kernels, each scaled by what the counter reads for a 1 GiB device copy
+# (run with --copy-calib 1024).  Counters cannot be read inside a timed run, so the profiled figure is
+# carried here and must be refreshed whenever the stage's kernels change.
+PROFILED_STAGE_TRAFFIC = {("volume32", 1): 171.9e9}
 
 
 def eig_flops(n: int) -> float:
@@ -63,20 +71,99 @@ def cpu_baseline(sample, opts, layers: int):
     return t * L / layers, threads
 
 
+def plane_z(g, iz: int) -> float:
+    """z of grid plane iz as the kernels compute it: dz * iz + z0 rounded once to float32."""
+    return float(np.float32(float(np.float32(g["dz"])) * iz + float(np.float32(g["z0"]))))
+
+
+def field_leg(ctx, sol, g, stream, dev):
+    """Separable synthesis of the solution's field on the grid (complex64 out, FP64 inside), timed with
+    HIP events on the library's stream; its error against the direct FP64 plane-wave sum (the
+    reference's per-point algorithm) on the first planes of the grid."""
+    import torch
+    args = (g["x0"], g["y0"], g["z0"], g["dx"], g["dy"], g["dz"], g["nx"], g["ny"], g["nz"])
+    out = torch.empty((g["nz"], g["ny"], g["nx"], 3), dtype=torch.complex64, device=dev)
+    ctx.field_solution_grid_dev(sol, out.data_ptr(), False, 0, *args)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.field_solution_grid_dev(sol, out.data_ptr(), False, 0, *args)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    tf = sum(times) / len(times)
+    npts = g["nx"] * g["ny"] * g["nz"]
+    # error of the complex64 output against the direct FP64 sum over the .cw records, on planes spread
+    # through the stack (each evaluated alone at the same coordinate, fmaf(dz, iz, z0))
+    ctx.field_init_solution(sol)
+    planes = sorted({0, g["nz"] // 3, (2 * g["nz"]) // 3, g["nz"] - 1})
+    ref = torch.empty((1, g["ny"], g["nx"], 3), dtype=torch.complex128, device=dev)
+    err, scale = 0.0, 0.0
+    for iz in planes:
+        ctx.field_grid_dev(ref.data_ptr(), True, 0, g["x0"], g["y0"], plane_z(g, iz), g["dx"], g["dy"], g["dz"],
+                           g["nx"], g["ny"], 1)
+        torch.cuda.synchronize()
+        err = max(err, (out[iz].to(torch.complex128) - ref[0]).abs().max().item())
+        scale = max(scale, ref.abs().max().item())
+    err /= scale
+    bytes_out = npts * 3 * 8
+    gbs = bytes_out / tf / 1e9
+    return {"form": "separable exp(i kx x).C(z).exp(i ky y), FP64 inside, complex64 out",
+            "gpts_per_s": npts / tf / 1e9, "points": npts, "ms": tf * 1e3,
+            "rel_err_vs_fp64_direct": err, "err_sample": f"planes {planes} of {g['nz']}",
+            "hbm_write_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+
+
+def cpu_field_baseline(sol, g):
+    """The oracle's C restatement of the reference's CPU field sum (OpenMP, FP64) on a few planes."""
+    from oracle import build as ob
+    from oracle import cw_oracle as o
+    lib = ob.load()
+    rec = sol.records
+    off, cnt = sol.lists()
+    lists = [rec[a:a + c] for a, c in zip(off, cnt)]
+    cw = o.CW(Pi=lists[0], z=list(sol.z), Pr=lists[1::2], Pt=lists[2::2])
+    W, zl, b, e = o.unpack(cw, np.complex128)
+    W = np.ascontiguousarray(W)
+    zl = np.ascontiguousarray(zl, np.float32)
+    b = np.ascontiguousarray(b, np.int32)
+    e = np.ascontiguousarray(e, np.int32)
+    # one plane inside the stack, as many rows as keep the sum near 2e9 wave evaluations
+    iz = g["nz"] // 2
+    z = plane_z(g, iz)
+    l = sum(1 for zz in zl if z >= zz - 0.001)
+    nwaves = int(e[l] - b[l])
+    ny = max(1, min(g["ny"], int(2e9 / max(nwaves, 1) / g["nx"])))
+    out = np.empty((1, ny, g["nx"], 3), np.complex128)
+    t0 = time.perf_counter()
+    lib.cw_eval_grid_f64(W.ctypes.data, zl.ctypes.data, b.ctypes.data, e.ctypes.data, len(zl), 0,
+                         g["x0"], g["y0"], z, g["dx"], g["dy"], g["dz"], g["nx"], ny, 1, out.ctypes.data)
+    t = time.perf_counter() - t0
+    npts = g["nx"] * ny
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"gpts_per_s": npts / t / 1e9, "cores": threads, "kind": "port",
+            "sample": f"{ny} rows of plane {iz} ({npts} points, {nwaves} waves each), direct FP64 sum"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="volume32")
-    ap.add_argument("--eig-mode", type=int, default=0)
-    ap.add_argument("--cpu-layers", type=int, default=4)
+    ap.add_argument("--eig-mode", type=int, default=1)
+    ap.add_argument("--cpu-layers", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-field", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a blit of known traffic that scales the "
+                         "profiler's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (instances and layers "
-              "shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the solver has no multi-GPU path (layers and instances "
+              "would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -92,6 +179,12 @@ def main():
     ctx = sa.Context(0)
     ctx.set_stream(stream.cuda_stream)
     torch.cuda.synchronize()
+    if args.copy_calib > 0:
+        src_buf = torch.ones(args.copy_calib << 20, dtype=torch.uint8, device=dev)
+        dst_buf = torch.empty_like(src_buf)
+        dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        del src_buf, dst_buf
 
     def one():
         sol = sa.scattervolume(ctx, d_sample.data_ptr(), shape=(L, s1, s2), eig_mode=args.eig_mode, **opts)
@@ -150,21 +243,10 @@ def main():
     # ---- field evaluation of the solution on the configuration grid ----
     field = None
     if not args.no_field:
-        ctx.field_init_solution(sol)
-        g = grid
-        out = torch.zeros((g["nz"], g["ny"], g["nx"], 3), dtype=torch.complex64, device=dev)
-        ctx.field_grid_dev(out.data_ptr(), False, 0, g["x0"], g["y0"], g["z0"], g["dx"], g["dy"], g["dz"],
-                           g["nx"], g["ny"], g["nz"])
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        ctx.field_grid_dev(out.data_ptr(), False, 0, g["x0"], g["y0"], g["z0"], g["dx"], g["dy"], g["dz"],
-                           g["nx"], g["ny"], g["nz"])
-        e1.record(stream)
-        torch.cuda.synchronize()
-        tf = e0.elapsed_time(e1) * 1e-3
-        npts = g["nx"] * g["ny"] * g["nz"]
-        field = {"gpts_per_s": npts / tf / 1e9, "points": npts, "ms": tf * 1e3, "form": "direct FP64-reduced"}
+        field = field_leg(ctx, sol, grid, stream, dev)
+    cpu_field = None
+    if not args.no_field and not args.no_cpu:
+        cpu_field = cpu_field_baseline(sol, grid)
     sol.free()
 
     cpu = None
@@ -172,7 +254,7 @@ def main():
         t_cpu, threads = cpu_baseline(sample, opts, min(args.cpu_layers, L))
         cpu = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"oracle solve of the first {min(args.cpu_layers, L)} of {L} layers, scaled by "
-                         f"{L}/{min(args.cpu_layers, L)}"}
+                         f"{L}/{min(args.cpu_layers, L)}", "field": cpu_field}
 
     line = {
         "metric": "sample solves/sec + field Gpts/s at <=1e-6 rel err vs FP64 (plain vs batched)",
@@ -188,13 +270,15 @@ def main():
         "dtype": "c128 (f64)",
         "data": "synthetic (seeded)",
         "config": {"workload": args.config, "layers": L, "grid": [s1, s2], "orders": [sol.M0, sol.M1],
-                   "eig_n": n, "eig_mode": "full 4MF Phi" if args.eig_mode == 0 else "2MF PQ",
+                   "eig_n": n, "eig_mode": "full 4MF Phi" if args.eig_mode == 0 else "reduced 2MF PQ",
+                   "field_grid": [grid["nx"], grid["ny"], grid["nz"]],
                    "parallelism": "1 GPU"},
         "solve_tflops_nominal": solve_flops(MF, L, args.eig_mode) / (ms * 1e-3) / 1e12,
         "field": field,
         "roofline": {"kernel": "eigensolver stage (gehrd+unghr+hqr+trevc+gemm+normalize)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": PROFILED_STAGE_TRAFFIC.get((args.config, args.eig_mode)),
                      "ms_per_launch": t_eig * 1e3, "flops_per_launch": fl},
         "cpu_baseline": cpu,
     }
