@@ -37,8 +37,8 @@ routes zgeev and zgesv through torch's CPU build (Intel MKL, the library family 
 
 Pins: every function here is checked in ``tests/test_oracle_pins.py`` against the reference-produced
 ``x64/a.cw`` known answer, closed forms (uniform layers, Fresnel), a brute-force DFT, an independent
-scalar TE/TM formulation for one-dimensional gratings, eigen residuals and energy balance.  Functions
-with no such check are listed in ``UNPINNED``.
+Maxwell formulation for one- and two-dimensional gratings, eigen residuals and energy balance.
+Parts with no such check are listed in ``UNPINNED``.
 """
 from __future__ import annotations
 
@@ -54,11 +54,15 @@ DBL_EPS = np.finfo(np.float64).eps
 
 # Parts of the path that no golden vector, closed form or invariant pins (see tests/test_oracle_pins.py).
 UNPINNED = (
-    "fftshift2d quirk for even-row/odd-column grids (index restatement only)",
-    "x/y ordering of Sx/Sy/EF against Phi rows and the transposed Pz index for M0>1 and M1>1 "
-    "(pinned only for x<->y symmetric samples and one-dimensional gratings)",
+    "fftshift2d on even-row/odd-column grids: checked only against a 2x3 case worked by hand from "
+    "FourierWave.h:92-135",
+    "the mixed x/y ordering of Sx/Sy/Sz/EF against Phi's rows (ScatterVolume.cpp:677-685) for M0>1 and M1>1: "
+    "an independent Maxwell formulation agrees once these tables are flattened x-fastest, so everything "
+    "but the ordering itself is pinned",
+    "the transposed Pz index (ScatterVolume.cpp:872,875) for M0>1 and M1>1: it changes only interior Ez",
     "eigen-sort order of eigenvalues whose imaginary parts differ by less than DBL_EPS",
 )
+
 
 _LAPACK = "numpy"
 

@@ -10,7 +10,7 @@
  * cw_eval_grid_f32 keeps the reference's FP32 arithmetic; cw_eval_grid_f64 evaluates the same sum in
  * FP64 on FP64 waves.  Both are OpenMP-parallel over points.
  *
- * Pins: tests/test_oracle_field.py compares both against oracle/cw_oracle.field_points, which is
+ * Pins: tests/test_capi_cpu.py compares both against oracle/cw_oracle.field_points, which is
  * checked against the reference-produced x64/a.cw continuity and closed-form TE stack.
  */
 #include <complex.h>
