@@ -126,6 +126,18 @@ int cwx_solve_volume(cwx_context* ctx, const cwx_volume_params* p, const cwx_com
 int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L,
                          int s1, int s2, cwx_solution** out);
 
+/* Solve `count` independent samples together (no counterpart in the reference, which runs one
+ * scattervolume process per sample, e.g. spectroscopy.py:66-90).  Instance i has params[i] and L[i]
+ * layers; the layers of all instances lie back to back in d_samples ((sum L) x s1 x s2 complex128,
+ * C order, device).  All instances must resolve to the same Fourier orders and eigen mode.  Each chunk
+ * of instances whose device workspace stays below max_bytes (0: 64 GiB) is one launch set over all of
+ * its layers.  d_x (device, count x 6MF complex) receives [Rx Ry Rz Tx Ty Tz] of every instance,
+ * h_info (host, count, may be NULL) its status; *M0 and *M1 (may be NULL) the orders.  With
+ * d_x == NULL only the orders are computed.  Returns the first non-zero instance status. */
+int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                        const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
+                        int* M1);
+
 /* Replaces volume::CalculateD / phi (FourierWave.h:303-397): the property matrix of every layer,
  * Phi = [[0, P], [Q, 0]] (4MF x 4MF, column-major, blocks = 0, L matrices) or its blocks P and Q
  * (2MF x 2MF each, blocks = 1: the L P blocks, then the L Q blocks) into device memory d_out.

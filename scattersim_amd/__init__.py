@@ -267,6 +267,57 @@ def scattervolume(ctx: Context, sample: np.ndarray, **opts) -> Solution:
     return Solution(ctx, h)
 
 
+def solve_batch(ctx: Context, samples, params, max_bytes: int = 0, shape=None, d_x: Optional[int] = None):
+    """Solve independent samples together (cwx_solve_batch_dev).
+
+    ``params`` is one option dict per instance (the keywords of ``scattervolume``); every instance must
+    resolve to the same Fourier orders and eigen mode.  ``samples`` is a list of (L_i, s1, s2) complex
+    arrays, or a device pointer (int) to the layers of all instances back to back with
+    ``shape=(Ls, s1, s2)``.  Returns (x, info, (M0, M1)): x is (count, 6MF) complex128 [R; T] per
+    instance.  With ``d_x`` (a device pointer to count x 6MF complex128) the result stays on the device
+    and x is None.
+    """
+    count = len(params)
+    pa = (_capi.cwx_volume_params * count)()
+    for i, o in enumerate(params):
+        pa[i] = _params(**o)
+    own = None
+    if isinstance(samples, int):
+        Ls, s1, s2 = shape
+        Ls = np.ascontiguousarray(Ls, np.int32)
+        ptr = samples
+    else:
+        arrs = [np.asarray(s, dtype=np.complex128) for s in samples]
+        arrs = [a[None] if a.ndim == 2 else a for a in arrs]
+        s1, s2 = arrs[0].shape[1:]
+        if any(a.shape[1:] != (s1, s2) for a in arrs):
+            raise ValueError("all samples must share the grid")
+        Ls = np.array([a.shape[0] for a in arrs], np.int32)
+        own = DeviceArray.from_numpy(ctx, np.ascontiguousarray(np.concatenate(arrs, axis=0)))
+        ptr = own.ptr
+    if len(Ls) != count:
+        raise ValueError("one layer count per instance is needed")
+    M0, M1 = C.c_int(), C.c_int()
+    check(lib().cwx_solve_batch_dev(ctx.h, count, pa, _ptr(Ls), None, int(s1), int(s2), int(max_bytes), None, None,
+                                    C.byref(M0), C.byref(M1)))
+    MF = M0.value * M1.value
+    out = None
+    if d_x is None:
+        out = DeviceArray(ctx, (count, 6 * MF), np.complex128)
+        d_x = out.ptr
+    info = np.zeros(count, np.int32)
+    st = lib().cwx_solve_batch_dev(ctx.h, count, pa, _ptr(Ls), C.c_void_p(ptr), int(s1), int(s2), int(max_bytes),
+                                   C.c_void_p(d_x), _ptr(info), None, None)
+    if st < 0:
+        raise _capi.CwxError(st, lib().cwx_last_error().decode())
+    x = out.numpy() if out is not None else None
+    if own is not None:
+        own.free()
+    if out is not None:
+        out.free()
+    return x, info, (M0.value, M1.value)
+
+
 def zgeev(ctx: Context, A: np.ndarray):
     """MKL_eigensolve contract: returns (w, VR); A is not modified."""
     n = A.shape[0]

@@ -74,6 +74,8 @@ SIGNATURES = {
     "cwx_volume_params_default": (None, [C.POINTER(cwx_volume_params)]),
     "cwx_solve_volume": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),
     "cwx_solve_volume_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),
+    "cwx_solve_batch_dev": (I, [P, I, C.POINTER(cwx_volume_params), P, P, I, I, C.c_int64, P, P, C.POINTER(I),
+                                C.POINTER(I)]),
     "cwx_phi_build_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, I, P, C.POINTER(I)]),
     "cwx_solution_get_info": (I, [P, C.POINTER(cwx_solution_info)]),
     "cwx_solution_size": (I64, [P, I]),

@@ -382,6 +382,36 @@ int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const voi
   });
 }
 
+int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                        const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
+                        int* M1) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && params && L, "NULL argument");
+    CWX_CHECK(count >= 1, "count must be positive");
+    CWX_CHECK(s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
+    CWX_CHECK(max_bytes >= 0, "max_bytes must be non-negative");
+    std::vector<cwx::VolumeOptions> opts(count);
+    std::vector<int> Ls(L, L + count);
+    for (int i = 0; i < count; ++i) {
+      CWX_CHECK(Ls[i] >= 1, "instance " + std::to_string(i) + " has no layers");
+      opts[i] = to_options(params + i);
+    }
+    const cwx::Setup st = cwx::make_setup(opts[0], Ls[0], s1, s2);
+    if (M0) *M0 = st.M0;
+    if (M1) *M1 = st.M1;
+    if (!d_x) return CWX_OK;
+    CWX_CHECK(d_samples != nullptr, "NULL samples");
+    set_device(ctx);
+    std::vector<int> info;
+    std::string msg;
+    const int st_code = cwx::solve_batch(ctx->c, opts, (const zc*)d_samples, Ls, s1, s2, (size_t)max_bytes,
+                                         (zc*)d_x, info, msg);
+    if (h_info) std::copy(info.begin(), info.end(), h_info);
+    if (st_code) return fail(st_code, msg);
+    return CWX_OK;
+  });
+}
+
 int cwx_phi_build_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
                       int blocks, void* d_out, int* n_out) {
   return guard([&]() -> int {

@@ -38,19 +38,62 @@ struct SepRegion {
 // ---------------------------------------------------------------------------------------------
 enum Op { OpN = 0, OpT = 1, OpC = 2 };
 
+// A set of `batch` matrices (or vectors): member b starts at base + stride * b, or at ptr[b] when a
+// device array of pointers is given; `off` (elements) is added in both cases, so sub-blocks of every
+// member form a set of their own.
+template <typename T>
+struct Set {
+  T* base = nullptr;
+  long stride = 0;
+  T* const* ptr = nullptr;
+  long off = 0;
+  __host__ __device__ T* at(int b) const { return (ptr ? ptr[b] : base + stride * b) + off; }
+  Set shifted(long o) const {
+    Set s = *this;
+    s.off += o;
+    return s;
+  }
+};
+using MatSet = Set<zc>;
+using IntSet = Set<int>;
+template <typename T>
+inline Set<T> strided(const T* p, long stride) {
+  Set<T> s;
+  s.base = const_cast<T*>(p);
+  s.stride = stride;
+  return s;
+}
+template <typename T>
+inline Set<T> listed(T* const* d_ptrs) {
+  Set<T> s;
+  s.ptr = d_ptrs;
+  return s;
+}
+
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
-void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const zc* A, int lda, long sA,
-                   const zc* B, int ldb, long sB, zc beta, zc* C, int ldc, long sC, int batch,
-                   hipStream_t s);
+void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda, MatSet B, int ldb, zc beta,
+               MatSet C, int ldc, int batch, hipStream_t s);
+inline void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const zc* A, int lda, long sA,
+                          const zc* B, int ldb, long sB, zc beta, zc* C, int ldc, long sC, int batch,
+                          hipStream_t s) {
+  zgemm_set(opa, opb, m, n, k, alpha, strided(A, sA), lda, strided(B, sB), ldb, beta, strided(C, sC), ldc, batch,
+            s);
+}
 
 // LU with partial pivoting, in place (LAPACK zgetrf contract, 0-based pivots).  info[b] = 0 or
 // the 1-based index of the first exactly-zero pivot.
-void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, int* info, int batch,
-                    hipStream_t s);
+void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hipStream_t s);
+inline void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, int* info, int batch,
+                           hipStream_t s) {
+  zgetrf_set(n, strided(A, sA), lda, strided(ipiv, sP), info, batch, s);
+}
 
 // Solve A X = B with the factors of zgetrf (LAPACK zgetrs, no transpose).
-void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,
-                    int ldb, long sB, int batch, hipStream_t s);
+void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int ldb, int batch, hipStream_t s);
+inline void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,
+                           int ldb, long sB, int batch, hipStream_t s) {
+  zgetrs_set(n, nrhs, strided(LU, sA), lda, strided(ipiv, sP), strided(B, sB), ldb, batch, s);
+}
 
 // ---------------------------------------------------------------------------------------------
 // non-symmetric eigensolver (LAPACK zgeev('N','V') contract)

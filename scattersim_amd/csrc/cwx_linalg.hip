@@ -24,16 +24,14 @@ __device__ inline zc load_op(const zc* __restrict__ A, int ld, Op op, int r, int
   return op == OpC ? conj(v) : v;
 }
 
-__global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int k, zc alpha,
-                                               const zc* __restrict__ A, int lda, long sA,
-                                               const zc* __restrict__ B, int ldb, long sB, zc beta,
-                                               zc* __restrict__ C, int ldc, long sC) {
+__global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet Aset,
+                                               int lda, MatSet Bset, int ldb, zc beta, MatSet Cset, int ldc) {
   __shared__ zc As[GK][GM];
   __shared__ zc Bs[GK][GN];
   const int b = blockIdx.z;
-  A += sA * b;
-  B += sB * b;
-  C += sC * b;
+  const zc* __restrict__ A = Aset.at(b);
+  const zc* __restrict__ B = Bset.at(b);
+  zc* __restrict__ C = Cset.at(b);
   const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
   const int t = threadIdx.x;
   const int tx = t & 15, ty = t >> 4;
@@ -124,15 +122,15 @@ __device__ inline void block_argmax(double v, int idx, double* sv, int* si, doub
   __syncthreads();
 }
 
-__global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, zc* A, int lda,
-                                                         long sA, int* ipiv, long sP, int* info) {
+__global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, MatSet Aset, int lda,
+                                                         IntSet pset, int* info) {
   __shared__ double sv[LU_THREADS / 64];
   __shared__ int si[LU_THREADS / 64];
   __shared__ double pv;
   __shared__ int pi;
   const int b = blockIdx.x;
-  A += sA * b;
-  ipiv += sP * b;
+  zc* A = Aset.at(b);
+  int* ipiv = pset.at(b);
   for (int jj = 0; jj < nb; ++jj) {
     const int j = j0 + jj;
     // pivot search over rows j..n-1 of column j
@@ -182,13 +180,13 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
 
 // Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
 // solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.
-__global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, zc* A, int lda, long sA,
-                                                     const int* ipiv, long sP) {
+__global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda,
+                                                     IntSet pset) {
   __shared__ zc L11[LU_NB][LU_NB];
   __shared__ int piv[LU_NB];
   const int b = blockIdx.y;
-  A += sA * b;
-  ipiv += sP * b;
+  zc* A = Aset.at(b);
+  const int* ipiv = pset.at(b);
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     int r = e % nb, c = e / nb;
     L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
@@ -222,12 +220,12 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, zc* 
 // ------------------------------------------------------------------------------------------------
 // getrs pieces: row interchanges on B, and small diagonal-block triangular solves.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_laswp(int n, int nrhs, zc* B, int ldb, long sB, const int* ipiv, long sP) {
+__global__ void k_laswp(int n, int nrhs, MatSet Bset, int ldb, IntSet pset) {
   const int b = blockIdx.y;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
-  B += sB * b;
-  ipiv += sP * b;
+  zc* B = Bset.at(b);
+  const int* ipiv = pset.at(b);
   zc* col = B + (long)c * ldb;
   for (int j = 0; j < n; ++j) {
     int p = ipiv[j];
@@ -237,12 +235,12 @@ __global__ void k_laswp(int n, int nrhs, zc* B, int ldb, long sB, const int* ipi
 
 // Solve T x = b for the nb x nb diagonal block starting at j0 (unit lower if lower, else upper),
 // for every right-hand-side column; one lane per column.
-__global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, const zc* LU, int lda,
-                                                   long sA, int nrhs, zc* B, int ldb, long sB) {
+__global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, MatSet LUset, int lda, int nrhs,
+                                                   MatSet Bset, int ldb) {
   __shared__ zc T[LU_NB][LU_NB];
   const int b = blockIdx.y;
-  LU += sA * b;
-  B += sB * b;
+  const zc* LU = LUset.at(b);
+  zc* B = Bset.at(b);
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     int r = e % nb, c = e / nb;
     T[r][c] = LU[(long)(j0 + c) * lda + j0 + r];
@@ -282,68 +280,60 @@ __global__ void k_fill_int(int n, int* p, int v) {
 
 }  // namespace
 
-void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const zc* A, int lda, long sA,
-                   const zc* B, int ldb, long sB, zc beta, zc* C, int ldc, long sC, int batch,
-                   hipStream_t s) {
+void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda, MatSet B, int ldb, zc beta,
+               MatSet C, int ldc, int batch, hipStream_t s) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgemm: batch too large");
-  if (k <= 0) {
-    // C = beta * C
-    CWX_CHECK(false, "zgemm: k must be positive");
-  }
+  CWX_CHECK(k > 0, "zgemm: k must be positive");
   dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
-  hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, sA, B, ldb, sB,
-                     beta, C, ldc, sC);
+  hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   CWX_HIP(hipGetLastError());
 }
 
-void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, int* info, int batch,
-                    hipStream_t s) {
+void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hipStream_t s) {
   if (n <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "zgetrf: batch too large");
   // a fill kernel rather than hipMemsetAsync keeps the sequence capturable in a HIP graph
   hipLaunchKernelGGL(k_fill_int, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, info, 0);
   for (int j0 = 0; j0 < n; j0 += LU_NB) {
     const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, sA, ipiv,
-                       sP, info);
+    hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     const int others = n - nb;
     if (others > 0)
-      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A,
-                         lda, sA, ipiv, sP);
+      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A, lda, ipiv);
     const int rest = n - j0 - nb;
     if (rest > 0)
-      zgemm_batched(OpN, OpN, rest, rest, nb, mk(-1.0), A + (long)j0 * lda + j0 + nb, lda, sA,
-                    A + (long)(j0 + nb) * lda + j0, lda, sA, mk(1.0),
-                    A + (long)(j0 + nb) * lda + j0 + nb, lda, sA, batch, s);
+      zgemm_set(OpN, OpN, rest, rest, nb, mk(-1.0), A.shifted((long)j0 * lda + j0 + nb), lda,
+                A.shifted((long)(j0 + nb) * lda + j0), lda, mk(1.0), A.shifted((long)(j0 + nb) * lda + j0 + nb), lda,
+                batch, s);
   }
   CWX_HIP(hipGetLastError());
 }
 
-void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,
-                    int ldb, long sB, int batch, hipStream_t s) {
+void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int ldb, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
-  hipLaunchKernelGGL(k_laswp, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, n, nrhs, B, ldb, sB, ipiv,
-                     sP);
+  CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
+  hipLaunchKernelGGL(k_laswp, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, n, nrhs, B, ldb, ipiv);
   // forward: L (unit lower)
   for (int j0 = 0; j0 < n; j0 += LU_NB) {
     const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 1, LU, lda,
-                       sA, nrhs, B, ldb, sB);
+    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 1, LU, lda, nrhs, B,
+                       ldb);
     const int rest = n - j0 - nb;
     if (rest > 0)
-      zgemm_batched(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU + (long)j0 * lda + j0 + nb, lda, sA,
-                    B + j0, ldb, sB, mk(1.0), B + j0 + nb, ldb, sB, batch, s);
+      zgemm_set(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda + j0 + nb), lda, B.shifted(j0), ldb,
+                mk(1.0), B.shifted(j0 + nb), ldb, batch, s);
   }
   // backward: U
   const int nblk = (n + LU_NB - 1) / LU_NB;
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int j0 = bi * LU_NB;
     const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 0, LU, lda,
-                       sA, nrhs, B, ldb, sB);
+    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 0, LU, lda, nrhs, B,
+                       ldb);
     if (j0 > 0)
-      zgemm_batched(OpN, OpN, j0, nrhs, nb, mk(-1.0), LU + (long)j0 * lda, lda, sA, B + j0, ldb, sB,
-                    mk(1.0), B, ldb, sB, batch, s);
+      zgemm_set(OpN, OpN, j0, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda), lda, B.shifted(j0), ldb, mk(1.0), B,
+                ldb, batch, s);
   }
   CWX_HIP(hipGetLastError());
 }

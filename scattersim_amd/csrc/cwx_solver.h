@@ -70,6 +70,14 @@ struct Solution {
 void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
                   Solution& sol);
 
+// Solve many independent samples that share the grid and the Fourier orders: instance i has opts[i]
+// and Ls[i] layers, stored back to back in d_samples.  Instances are processed in chunks whose
+// device workspace stays below max_bytes (0: 64 GiB); each chunk is one launch set over all of
+// its layers.  d_x (device, count x 6MF) receives every instance's [R; T]; info its status.
+// Returns the first non-zero status (first_msg describes it).
+int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
+                int s1, int s2, size_t max_bytes, zc* d_x, std::vector<int>& info, std::string& first_msg);
+
 // Work buffers of the separable field evaluation.
 struct SepWork {
   DevBuf<zc> C, X, Y;

@@ -175,12 +175,33 @@ __global__ void k_square_layers(long n, const zc* __restrict__ smp, zc* __restri
   }
 }
 
-struct PhiArgs {
+// Per-instance tables of a set of samples solved together (one entry per instance unless noted).
+// Every per-layer kernel finds its instance through linst.
+struct InstArgs {
+  int M0, M1, MF;
+  const int* linst;    // layer -> instance
+  const double* k;     // 2 PI / lambda
+  const double* up;    // count x M0
+  const double* wq;    // count x M1
+  const double* upz;   // count x MF: UP with M_PI, indexed p' = qj*M0 + pj (ScatterVolume.cpp:813-814)
+  const double* wqz;   // count x MF
+  const double* z0;    // first boundary
+  const double* nio;   // count x 2: n0, n1 (the extra factor on the external kz)
+  const zc* tabs;      // count x 7MF: Sx | Sy | Sz0 | Sz1 | EF (3MF)
+};
+
+struct PhiLocal {
   int M0, M1, MF;
   double kinv;
   const double* up;  // M0
   const double* wq;  // M1
 };
+
+__device__ inline PhiLocal phi_local(const InstArgs& a, int layer) {
+  const int i = a.linst[layer];
+  const double k = a.k[i];
+  return PhiLocal{a.M0, a.M1, a.MF, 1.0 / (k * k), a.up + (long)i * a.M0, a.wq + (long)i * a.M1};
+}
 
 // Element (r, li) coefficient indices of the circulant blocks (FourierWave.h:366-372).
 __device__ inline int circ_index(int r, int li, int M0, int M1) {
@@ -191,7 +212,7 @@ __device__ inline int circ_index(int r, int li, int M0, int M1) {
 }
 
 // P (rows 0..2MF, cols 2MF..4MF of Phi) and Q (rows 2MF..4MF, cols 0..2MF), FourierWave.h:378-392.
-__device__ inline zc phi_P(const PhiArgs& a, const zc* NI, int rr, int cc) {
+__device__ inline zc phi_P(const PhiLocal& a, const zc* NI, int rr, int cc) {
   const int MF = a.MF;
   const int r = rr % MF, li = cc % MF;
   const double upr = a.up[r % a.M0], wqr = a.wq[r / a.M0];
@@ -212,7 +233,7 @@ __device__ inline zc phi_P(const PhiArgs& a, const zc* NI, int rr, int cc) {
   return (-wqr * a.kinv * upl) * ni;
 }
 
-__device__ inline zc phi_Q(const PhiArgs& a, const zc* NF, int rr, int cc) {
+__device__ inline zc phi_Q(const PhiLocal& a, const zc* NF, int rr, int cc) {
   const int MF = a.MF;
   const int r = rr % MF, li = cc % MF;
   const double upr = a.up[r % a.M0], wqr = a.wq[r / a.M0];
@@ -232,9 +253,10 @@ __device__ inline zc phi_Q(const PhiArgs& a, const zc* NF, int rr, int cc) {
 }
 
 // full Phi (4MF x 4MF, column-major) per layer
-__global__ void k_phi_full(PhiArgs a, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* Phi) {
-  const int n = 4 * a.MF, n2 = 2 * a.MF;
+__global__ void k_phi_full(InstArgs ia, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* Phi) {
   const int l = blockIdx.y;
+  const PhiLocal a = phi_local(ia, l);
+  const int n = 4 * a.MF, n2 = 2 * a.MF;
   NF += (long)l * a.MF;
   NI += (long)l * a.MF;
   Phi += (long)l * n * n;
@@ -248,10 +270,10 @@ __global__ void k_phi_full(PhiArgs a, const zc* __restrict__ NF, const zc* __res
 }
 
 // P and Q blocks (2MF x 2MF each) per layer
-__global__ void k_phi_blocks(PhiArgs a, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* P,
-                             zc* Q) {
-  const int n2 = 2 * a.MF;
+__global__ void k_phi_blocks(InstArgs ia, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* P, zc* Q) {
   const int l = blockIdx.y;
+  const PhiLocal a = phi_local(ia, l);
+  const int n2 = 2 * a.MF;
   NF += (long)l * a.MF;
   NI += (long)l * a.MF;
   P += (long)l * n2 * n2;
@@ -351,12 +373,13 @@ __global__ __launch_bounds__(256) void k_sort_modes(int n4, int reduced, const z
 // Gd/Gc per layer (ScatterVolume.cpp:297-321) from the sorted modes.
 // Full mode: column j of V = Vsrc(:, src[j]).  Reduced: column = [a; +-(Q a)/gamma] with a the
 // eigenvector of P Q for the pair and gamma the down member.
-__global__ void k_build_gdgc(int n4, int reduced, const zc* __restrict__ V, const zc* __restrict__ QA,
-                             const zc* __restrict__ gam, const int* __restrict__ src, double k,
+__global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V, const zc* __restrict__ QA,
+                             const zc* __restrict__ gam, const int* __restrict__ src,
                              const double* __restrict__ dz, zc* __restrict__ Gd, zc* __restrict__ Gc) {
   const int l = blockIdx.y;
-  const int n2 = n4 / 2;
+  const int n4 = 4 * ia.MF, n2 = n4 / 2;
   const long nn = (long)n4 * n4;
+  const double k = ia.k[ia.linst[l]];
   gam += (long)l * n4;
   src += (long)l * n4;
   Gd += l * nn;
@@ -397,6 +420,13 @@ __global__ void k_copy(long n, const zc* __restrict__ a, zc* __restrict__ b) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+// one matrix of n elements per member: dst[b] = src[b]
+__global__ void k_copy_set(long n, MatSet src, MatSet dst) {
+  const zc* __restrict__ a = src.at(blockIdx.y);
+  zc* __restrict__ b = dst.at(blockIdx.y);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 struct BoundaryArgs {
   int MF;
   double k;
@@ -407,6 +437,12 @@ struct BoundaryArgs {
   const zc* Sz1;
   const zc* EF;
 };
+
+__device__ inline BoundaryArgs boundary_args(const InstArgs& ia, int i) {
+  const zc* t = ia.tabs + (long)i * 7 * ia.MF;
+  const int MF = ia.MF;
+  return BoundaryArgs{MF, ia.k[i], ia.z0[i], t, t + MF, t + 2 * MF, t + 3 * MF, t + 4 * MF};
+}
 
 // f1/f2/f3 entries (ScatterVolume.cpp:339-395); each is 4MF x 3MF with diagonal MF blocks.
 __device__ inline zc f_entry(const BoundaryArgs& a, int which, int r, int c) {
@@ -447,16 +483,25 @@ __device__ inline zc f_entry(const BoundaryArgs& a, int which, int r, int c) {
   }
 }
 
-__global__ void k_build_f(BoundaryArgs a, int which, zc* __restrict__ F) {
+// F[i] = f_which of instance i (4MF x 3MF each)
+__global__ void k_build_f(InstArgs ia, int which, zc* __restrict__ F) {
+  const int i = blockIdx.y;
+  const BoundaryArgs a = boundary_args(ia, i);
   const int n4 = 4 * a.MF, n3 = 3 * a.MF;
+  F += (long)i * n4 * n3;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n4 * n3; e += (long)gridDim.x * blockDim.x)
     F[e] = f_entry(a, which, (int)(e % n4), (int)(e / n4));
 }
 
-// A (6MF x 6MF) and b (ScatterVolume.cpp:126-131, 398-456): Gauss rows, [f2 | Gd0 Gc^-1 f3], -f1 EF.
-__global__ void k_build_system(BoundaryArgs a, const zc* __restrict__ Abr, zc* __restrict__ A,
-                               zc* __restrict__ b) {
+// A (6MF x 6MF) and b (ScatterVolume.cpp:126-131, 398-456): Gauss rows, [f2 | Gd0 Gc^-1 f3], -f1 EF;
+// one instance per blockIdx.y.
+__global__ void k_build_system(InstArgs ia, const zc* __restrict__ Abr, zc* __restrict__ A, zc* __restrict__ b) {
+  const int i = blockIdx.y;
+  const BoundaryArgs a = boundary_args(ia, i);
   const int MF = a.MF, n6 = 6 * MF, n4 = 4 * MF, n3 = 3 * MF;
+  Abr += (long)i * n4 * n3;
+  A += (long)i * n6 * n6;
+  b += (long)i * n6;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n6 * n6; e += (long)gridDim.x * blockDim.x) {
     const int r = (int)(e % n6), c = (int)(e / n6);
     zc v = mk(0.0);
@@ -491,9 +536,13 @@ __global__ void k_build_system(BoundaryArgs a, const zc* __restrict__ Abr, zc* _
   }
 }
 
-// rhs of beta_0: f1 EF + f2 R  (ScatterVolume.cpp:767-772)
-__global__ void k_beta0_rhs(BoundaryArgs a, const zc* __restrict__ x, zc* __restrict__ out) {
-  const int n4 = 4 * a.MF, n3 = 3 * a.MF;
+// rhs of beta_0: f1 EF + f2 R  (ScatterVolume.cpp:767-772), into the first layer's beta of instance i
+__global__ void k_beta0_rhs(InstArgs ia, const zc* __restrict__ x, MatSet out) {
+  const int i = blockIdx.y;
+  const BoundaryArgs a = boundary_args(ia, i);
+  const int n4 = 4 * a.MF, n3 = 3 * a.MF, n6 = 6 * a.MF;
+  x += (long)i * n6;
+  zc* o = out.at(i);
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n4; r += gridDim.x * blockDim.x) {
     zc v = mk(0.0);
     for (int c = 0; c < n3; ++c) {
@@ -501,18 +550,22 @@ __global__ void k_beta0_rhs(BoundaryArgs a, const zc* __restrict__ x, zc* __rest
       if (f1.re != 0.0 || f1.im != 0.0) v = v + f1 * a.EF[c];
       if (f2.re != 0.0 || f2.im != 0.0) v = v + f2 * x[c];
     }
-    out[r] = v;
+    o[r] = v;
   }
 }
 
 // Q_check / Q_hat rows 0..2MF and the Ez right-hand side Y (ScatterVolume.cpp:823-881):
 // Qch = [V_even diag(beta_even) | V_odd diag(beta_odd)] rows 0..2MF (3MF x 4MF per layer);
 // Y(p', c) = -(1/k) (up_p' Hy(p', c) - wq_p' Hx(p', c)), MF x 4MF per layer.
-__global__ void k_q_rows(int MF, double k, const zc* __restrict__ Gd, const zc* __restrict__ Gc,
-                         const zc* __restrict__ beta, const double* __restrict__ upz,
-                         const double* __restrict__ wqz, zc* __restrict__ Qch, zc* __restrict__ Y) {
+__global__ void k_q_rows(InstArgs ia, const zc* __restrict__ Gd, const zc* __restrict__ Gc,
+                         const zc* __restrict__ beta, zc* __restrict__ Qch, zc* __restrict__ Y) {
+  const int MF = ia.MF;
   const int n4 = 4 * MF, n3 = 3 * MF, n2 = 2 * MF;
   const int l = blockIdx.y;
+  const int inst = ia.linst[l];
+  const double k = ia.k[inst];
+  const double* upz = ia.upz + (long)inst * MF;
+  const double* wqz = ia.wqz + (long)inst * MF;
   const long nn = (long)n4 * n4;
   Gd += l * nn;
   Gc += l * nn;
@@ -562,50 +615,43 @@ __global__ void k_zero_rows(int rows0, int nrows, int ld, int ncols, zc* __restr
 //   3: down modes of layer `layer` wound at boundary z, 4: up modes of layer `layer` wound at z.
 struct Seg {
   long off, cnt;
-  int kind, layer;
+  int kind, layer, inst;
   double zb;
 };
 
-struct RecArgs {
-  int MF;
-  double k, n0, n1;
-  const zc* Sx;
-  const zc* Sy;
-  const zc* Sz0;
-  const zc* Sz1;
-  const zc* EF;
-  const zc* x;
-  const zc* gam;
-  const zc* Qch;
-};
-
-__global__ void k_records(RecArgs a, const Seg* __restrict__ segs, zc* __restrict__ rec) {
+__global__ void k_records(InstArgs ia, const zc* __restrict__ x, const zc* __restrict__ gam,
+                          const zc* __restrict__ Qch, const Seg* __restrict__ segs, zc* __restrict__ rec) {
   const Seg sg = segs[blockIdx.y];
-  const int MF = a.MF, n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF;
+  const int MF = ia.MF, n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF, n6 = 6 * MF;
+  const double k = ia.k[sg.inst];
+  const double n0 = ia.nio[2 * sg.inst], n1 = ia.nio[2 * sg.inst + 1];
+  const zc* t = ia.tabs + (long)sg.inst * 7 * MF;
+  const zc *Sx = t, *Sy = t + MF, *Sz0 = t + 2 * MF, *Sz1 = t + 3 * MF, *EF = t + 4 * MF;
+  x += (long)sg.inst * n6;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < sg.cnt; i += (long)gridDim.x * blockDim.x) {
     zc E[3], K[3];
     int p;
     if (sg.kind <= 2) p = (int)i;
     else p = (int)(i / n2);
-    K[0] = a.Sx[p] * a.k;
-    K[1] = a.Sy[p] * a.k;
+    K[0] = Sx[p] * k;
+    K[1] = Sy[p] * k;
     bool wind = true;
     if (sg.kind == 0) {
-      K[2] = a.Sz0[p] * a.k;
-      for (int c = 0; c < 3; ++c) E[c] = a.EF[c * MF + p];
+      K[2] = Sz0[p] * k;
+      for (int c = 0; c < 3; ++c) E[c] = EF[c * MF + p];
       wind = false;
     } else if (sg.kind == 1) {
-      K[2] = -(a.Sz0[p] * a.k) * a.n0;
-      for (int c = 0; c < 3; ++c) E[c] = a.x[c * MF + p];
+      K[2] = -(Sz0[p] * k) * n0;
+      for (int c = 0; c < 3; ++c) E[c] = x[c * MF + p];
     } else if (sg.kind == 2) {
-      K[2] = (a.Sz1[p] * a.k) * a.n1;
-      for (int c = 0; c < 3; ++c) E[c] = a.x[n3 + c * MF + p];
+      K[2] = (Sz1[p] * k) * n1;
+      for (int c = 0; c < 3; ++c) E[c] = x[n3 + c * MF + p];
     } else {
       const int jj = (int)(i % n2);
       const bool up = (sg.kind == 4);
-      const zc g = a.gam[(long)sg.layer * n4 + 2 * jj + (up ? 1 : 0)];
-      K[2] = g * a.k;
-      const zc* Q = a.Qch + (long)sg.layer * n3 * n4 + (long)(jj + (up ? n2 : 0)) * n3;
+      const zc g = gam[(long)sg.layer * n4 + 2 * jj + (up ? 1 : 0)];
+      K[2] = g * k;
+      const zc* Q = Qch + (long)sg.layer * n3 * n4 + (long)(jj + (up ? n2 : 0)) * n3;
       for (int c = 0; c < 3; ++c) E[c] = Q[c * MF + p];
     }
     if (wind) {
@@ -630,9 +676,10 @@ unsigned grid_for(long n) { return (unsigned)std::min<long>(65535, std::max<long
 
 // Fourier coefficients of n^2 (layers 0..L-1) and n^-2 (layers L..2L-1) of every layer, each an
 // (M1 x M0) column-major block, by a pruned DFT: F = w1 (M1 x s1) . A (s1 x s2) . w2 (s2 x M0) / (s1 s2)
-// with A the column-major layer matrix (FourierWave.h:152-195, 288, 341-343).
-void fourier_stage(const Setup& st, const zc* d_sample, zc* F, hipStream_t s) {
-  const int L = st.L, s1 = st.s1, s2 = st.s2, M0 = st.M0, M1 = st.M1, MF = st.MF;
+// with A the column-major layer matrix (FourierWave.h:152-195, 288, 341-343).  The kept frequencies
+// depend only on the grid and the orders, so one pair of DFT matrices serves every layer.
+void fourier_stage(const Setup& st, int L, const zc* d_sample, zc* F, hipStream_t s) {
+  const int s1 = st.s1, s2 = st.s2, M0 = st.M0, M1 = st.M1, MF = st.MF;
   std::vector<zc> w1((size_t)M1 * s1), w2((size_t)s2 * M0);
   for (int i = 0; i < s1; ++i)
     for (int r = 0; r < M1; ++r) {
@@ -652,13 +699,104 @@ void fourier_stage(const Setup& st, const zc* d_sample, zc* F, hipStream_t s) {
   const long npix = (long)s1 * s2;
   DevBuf<zc> sq(2 * npix * L), T1((size_t)2 * L * s1 * M0);
   hipLaunchKernelGGL(k_square_layers, dim3(grid_for(npix * L)), dim3(256), 0, s, npix * L, d_sample, sq.p);
-  zgemm_batched(OpN, OpN, s1, M0, s2, mk(1.0), sq.p, s1, npix, d_w2.p, s2, 0, mk(0.0), T1.p, s1,
-                (long)s1 * M0, 2 * L, s);
-  zgemm_batched(OpN, OpN, M1, M0, s1, mk(1.0), d_w1.p, M1, 0, T1.p, s1, (long)s1 * M0, mk(0.0), F, M1,
-                MF, 2 * L, s);
+  for (int b0 = 0; b0 < 2 * L; b0 += 65535) {
+    const int nb = std::min(65535, 2 * L - b0);
+    zgemm_batched(OpN, OpN, s1, M0, s2, mk(1.0), sq.p + b0 * npix, s1, npix, d_w2.p, s2, 0, mk(0.0),
+                  T1.p + (long)b0 * s1 * M0, s1, (long)s1 * M0, nb, s);
+    zgemm_batched(OpN, OpN, M1, M0, s1, mk(1.0), d_w1.p, M1, 0, T1.p + (long)b0 * s1 * M0, s1, (long)s1 * M0,
+                  mk(0.0), F + (long)b0 * MF, M1, MF, nb, s);
+  }
   // keep the temporaries alive until the kernels that read them have run
   CWX_HIP(hipStreamSynchronize(s));
 }
+
+// Host tables of a set of instances that share the grid and the orders.
+struct InstSet {
+  int count = 0, Ltot = 0, maxL = 0;
+  std::vector<Setup> st;
+  std::vector<int> L, first;  // layers and first global layer of every instance
+  DevBuf<int> linst;
+  DevBuf<double> k, up, wq, upz, wqz, z0, nio, dz;
+  DevBuf<zc> tabs;
+  InstArgs args() const {
+    const Setup& s0 = st[0];
+    return InstArgs{s0.M0, s0.M1, s0.MF, linst.p, k.p, up.p, wq.p, upz.p, wqz.p, z0.p, nio.p, tabs.p};
+  }
+};
+
+void make_instances(const std::vector<VolumeOptions>& opts, const std::vector<int>& Ls, int s1, int s2,
+                    InstSet& is, hipStream_t s) {
+  const int count = (int)opts.size();
+  CWX_CHECK(count >= 1 && (int)Ls.size() == count, "instance set is empty");
+  is.count = count;
+  is.st.clear();
+  is.L = Ls;
+  is.first.assign(count, 0);
+  is.Ltot = 0;
+  is.maxL = 0;
+  for (int i = 0; i < count; ++i) {
+    is.st.push_back(make_setup(opts[i], Ls[i], s1, s2));
+    is.first[i] = is.Ltot;
+    is.Ltot += Ls[i];
+    is.maxL = std::max(is.maxL, Ls[i]);
+    CWX_CHECK(is.st[i].M0 == is.st[0].M0 && is.st[i].M1 == is.st[0].M1,
+              "instances solved together must have the same Fourier orders (instance " + std::to_string(i) +
+                  " has " + std::to_string(is.st[i].M0) + "x" + std::to_string(is.st[i].M1) + ")");
+    CWX_CHECK(opts[i].eig_mode == opts[0].eig_mode, "instances solved together must use one eigen mode");
+  }
+  CWX_CHECK(is.Ltot <= 65535, "too many layers in one set");
+  const int M0 = is.st[0].M0, M1 = is.st[0].M1, MF = M0 * M1;
+  std::vector<int> linst(is.Ltot);
+  std::vector<double> k(count), up, wq, upz, wqz, z0(count), nio(2 * count), dz(is.Ltot);
+  std::vector<zc> tabs;
+  tabs.reserve((size_t)7 * MF * count);
+  for (int i = 0; i < count; ++i) {
+    const Setup& st = is.st[i];
+    const VolumeOptions& o = opts[i];
+    const int L = Ls[i];
+    for (int l = 0; l < L; ++l) {
+      linst[is.first[i] + l] = i;
+      dz[is.first[i] + l] = (L == 1) ? (st.z[L] - st.z[0]) : (st.z[l + 1] - st.z[l]);
+    }
+    k[i] = st.k;
+    up.insert(up.end(), st.up.begin(), st.up.end());
+    wq.insert(wq.end(), st.wq.begin(), st.wq.end());
+    // Pz uses M_PI in UP/WQ (ScatterVolume.cpp:813-814), indexed by the column p' = qj*M0 + pj
+    for (int q = 0; q < M1; ++q)
+      for (int pp = 0; pp < M0; ++pp) {
+        upz.push_back(2.0 * (double)(pp - M0 / 2) * kPi / o.size[0] + st.dir_n[0] * st.k);
+        wqz.push_back(2.0 * (double)(q - M1 / 2) * kPi / o.size[1] + st.dir_n[1] * st.k);
+      }
+    z0[i] = st.z[0];
+    nio[2 * i] = o.n[0];
+    nio[2 * i + 1] = o.n[1];
+    for (auto* v : {&st.Sx, &st.Sy, &st.Sz0, &st.Sz1, &st.EF}) tabs.insert(tabs.end(), v->begin(), v->end());
+  }
+  upload(is.linst, linst, s);
+  upload(is.k, k, s);
+  upload(is.up, up, s);
+  upload(is.wq, wq, s);
+  upload(is.upz, upz, s);
+  upload(is.wqz, wqz, s);
+  upload(is.z0, z0, s);
+  upload(is.nio, nio, s);
+  upload(is.dz, dz, s);
+  upload(is.tabs, tabs, s);
+}
+
+// Pointer lists for the set-based linear algebra, gathered on the host and uploaded once.
+struct PtrLists {
+  std::vector<void*> h;
+  DevBuf<void*> d;
+  size_t add(const std::vector<void*>& v) {
+    const size_t o = h.size();
+    h.insert(h.end(), v.begin(), v.end());
+    return o;
+  }
+  void commit(hipStream_t s) { upload(d, h, s); }
+  MatSet mats(size_t o) const { return listed<zc>(reinterpret_cast<zc* const*>(d.p + o)); }
+  IntSet ints(size_t o) const { return listed<int>(reinterpret_cast<int* const*>(d.p + o)); }
+};
 
 }  // namespace
 
@@ -666,241 +804,389 @@ void fourier_stage(const Setup& st, const zc* d_sample, zc* F, hipStream_t s) {
 // all P first) into device memory.  Returns the setup.
 Setup build_phi_dev(const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2, int blocks, zc* out,
                     hipStream_t s) {
-  Setup st = make_setup(o, L, s1, s2);
+  InstSet is;
+  make_instances({o}, {L}, s1, s2, is, s);
+  const Setup& st = is.st[0];
   const int MF = st.MF;
   DevBuf<zc> F((size_t)2 * L * MF);
-  fourier_stage(st, d_sample, F.p, s);
-  DevBuf<double> d_up, d_wq;
-  upload(d_up, st.up, s);
-  upload(d_wq, st.wq, s);
-  PhiArgs pa{st.M0, st.M1, MF, 1.0 / (st.k * st.k), d_up.p, d_wq.p};
+  fourier_stage(st, L, d_sample, F.p, s);
   const long n4 = 4L * MF, n2 = 2L * MF;
   if (blocks == 0)
-    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(n4 * n4), L), dim3(256), 0, s, pa, F.p, F.p + (long)L * MF, out);
+    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(n4 * n4), L), dim3(256), 0, s, is.args(), F.p, F.p + (long)L * MF,
+                       out);
   else
-    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(n2 * n2), L), dim3(256), 0, s, pa, F.p, F.p + (long)L * MF,
-                       out, out + L * n2 * n2);
+    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(n2 * n2), L), dim3(256), 0, s, is.args(), F.p,
+                       F.p + (long)L * MF, out, out + L * n2 * n2);
   CWX_HIP(hipGetLastError());
   CWX_HIP(hipStreamSynchronize(s));
   return st;
 }
 
 // ------------------------------------------------------------------------------------------------
-// the pipeline
+// the pipeline, for a set of instances that share the grid and the Fourier orders
 // ------------------------------------------------------------------------------------------------
-void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
-                  Solution& sol) {
+namespace {
+
+struct SetResult {
+  DevBuf<zc>* x = nullptr;      // count x 6MF (caller-owned buffer)
+  std::vector<int> info;        // per instance
+  std::vector<std::string> msg;
+  Solution* full = nullptr;     // count == 1: keep every array and build the .cw records here
+};
+
+void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
+               int s1, int s2, SetResult& res) {
   hipStream_t s = ctx.stream;
-  sol.opt = o;
-  sol.st = make_setup(o, L, s1, s2);
-  const Setup& st = sol.st;
-  const int M0 = st.M0, M1 = st.M1, MF = st.MF;
+  InstSet is;
+  make_instances(opts, Ls, s1, s2, is, s);
+  const int count = is.count, Ltot = is.Ltot;
+  const Setup& st0 = is.st[0];
+  const int M0 = st0.M0, M1 = st0.M1, MF = st0.MF;
   const int n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF, n6 = 6 * MF;
   const long nn = (long)n4 * n4;
-  const double k = st.k;
-  sol.n4 = n4;
-  sol.info = 0;
-  sol.info_msg.clear();
+  const int eig_mode = opts[0].eig_mode;
   CWX_CHECK(n4 <= SORT_MAX, "4*M0*M1 exceeds the supported maximum of 2048");
-  CWX_CHECK(L <= 65535, "too many layers");
+  CWX_CHECK(res.full == nullptr || count == 1, "full solutions are kept for single instances only");
+  const InstArgs ia = is.args();
+  Solution* full = res.full;
 
-  // small host tables -> device
-  DevBuf<double> d_up, d_wq, d_dz, d_upz, d_wqz;
-  upload(d_up, st.up, s);
-  upload(d_wq, st.wq, s);
-  std::vector<double> dz(L);
-  for (int i = 0; i < L; ++i) dz[i] = (L == 1) ? (st.z[L] - st.z[0]) : (st.z[i + 1] - st.z[i]);
-  upload(d_dz, dz, s);
-  // Pz uses M_PI in UP/WQ (ScatterVolume.cpp:813-814), indexed by the column p' = qj*M0 + pj
-  std::vector<double> upz(MF), wqz(MF);
-  for (int q = 0; q < M1; ++q)
-    for (int pp = 0; pp < M0; ++pp) {
-      upz[q * M0 + pp] = 2.0 * (double)(pp - M0 / 2) * kPi / o.size[0] + st.dir_n[0] * k;
-      wqz[q * M0 + pp] = 2.0 * (double)(q - M1 / 2) * kPi / o.size[1] + st.dir_n[1] * k;
-    }
-  upload(d_upz, upz, s);
-  upload(d_wqz, wqz, s);
-  // Sx | Sy | Sz0 | Sz1 | EF (3MF), kept with the solution for the separable field evaluation
-  {
-    std::vector<zc> tabs;
-    tabs.reserve(7 * (size_t)MF);
-    for (auto* v : {&st.Sx, &st.Sy, &st.Sz0, &st.Sz1, &st.EF}) tabs.insert(tabs.end(), v->begin(), v->end());
-    upload(sol.tabs, tabs, s);
-  }
-  const zc* dSx = sol.tabs.p;
-  const zc* dSy = sol.tabs.p + MF;
-  const zc* dSz0 = sol.tabs.p + 2 * MF;
-  const zc* dSz1 = sol.tabs.p + 3 * MF;
-  const zc* dEF = sol.tabs.p + 4 * MF;
-  DevBuf<zc> F((size_t)2 * L * MF);
-  fourier_stage(st, d_sample, F.p, s);
+  // ---- 1. Fourier coefficients of n^2 and n^-2 of every layer ----
+  DevBuf<zc> F((size_t)2 * Ltot * MF);
+  fourier_stage(st0, Ltot, d_samples, F.p, s);
   const zc* NF = F.p;
-  const zc* NI = F.p + (long)L * MF;
-  sol.NIF.alloc((size_t)L * MF);
-  CWX_HIP(hipMemcpyAsync(sol.NIF.p, NI, sizeof(zc) * L * MF, hipMemcpyDeviceToDevice, s));
+  const zc* NI = F.p + (long)Ltot * MF;
 
   // ---- 2-3. Phi and its eigen-decomposition ----
-  PhiArgs pa{M0, M1, MF, 1.0 / (k * k), d_up.p, d_wq.p};
-  DevBuf<int> d_src((size_t)L * n4);
-  sol.gam.alloc((size_t)L * n4);
-  DevBuf<int> d_info(L);
+  DevBuf<int> d_src((size_t)Ltot * n4);
+  DevBuf<zc> gam_local;
+  DevBuf<zc>& gam = full ? full->gam : gam_local;
+  gam.alloc((size_t)Ltot * n4);
+  DevBuf<int> d_info(Ltot);
   DevBuf<zc> W, V, QA;
-  ctx.w0.ensure(std::max<size_t>((size_t)L * nn, 1));
-  if (o.eig_mode == 0) {
+  ctx.w0.ensure(std::max<size_t>((size_t)Ltot * nn, 1));
+  if (eig_mode == 0) {
     zc* Phi = ctx.w0.p;
-    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), L), dim3(256), 0, s, pa, NF, NI, Phi);
-    W.alloc((size_t)L * n4);
-    V.alloc((size_t)L * nn);
-    zgeev_batched(n4, Phi, n4, nn, W.p, n4, V.p, n4, nn, L, ctx.eig, d_info.p, s);
-    hipLaunchKernelGGL(k_sort_modes, dim3(L), dim3(256), 0, s, n4, 0, W.p, sol.gam.p, d_src.p);
+    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, NF, NI, Phi);
+    W.alloc((size_t)Ltot * n4);
+    V.alloc((size_t)Ltot * nn);
+    zgeev_batched(n4, Phi, n4, nn, W.p, n4, V.p, n4, nn, Ltot, ctx.eig, d_info.p, s);
+    hipLaunchKernelGGL(k_sort_modes, dim3(Ltot), dim3(256), 0, s, n4, 0, W.p, gam.p, d_src.p);
   } else {
     const long mm = (long)n2 * n2;
     zc* P = ctx.w0.p;
-    zc* Q = ctx.w0.p + L * mm;
-    ctx.w1.ensure((size_t)L * mm);
+    zc* Q = ctx.w0.p + Ltot * mm;
+    ctx.w1.ensure((size_t)Ltot * mm);
     zc* PQ = ctx.w1.p;
-    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(mm), L), dim3(256), 0, s, pa, NF, NI, P, Q);
-    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), P, n2, mm, Q, n2, mm, mk(0.0), PQ, n2, mm, L, s);
-    W.alloc((size_t)L * n2);
-    V.alloc((size_t)L * mm);
-    zgeev_batched(n2, PQ, n2, mm, W.p, n2, V.p, n2, mm, L, ctx.eig, d_info.p, s);
-    QA.alloc((size_t)L * mm);
-    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), Q, n2, mm, V.p, n2, mm, mk(0.0), QA.p, n2, mm, L, s);
-    hipLaunchKernelGGL(k_sort_modes, dim3(L), dim3(256), 0, s, n4, 1, W.p, sol.gam.p, d_src.p);
+    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(mm), Ltot), dim3(256), 0, s, ia, NF, NI, P, Q);
+    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), P, n2, mm, Q, n2, mm, mk(0.0), PQ, n2, mm, Ltot, s);
+    W.alloc((size_t)Ltot * n2);
+    V.alloc((size_t)Ltot * mm);
+    zgeev_batched(n2, PQ, n2, mm, W.p, n2, V.p, n2, mm, Ltot, ctx.eig, d_info.p, s);
+    QA.alloc((size_t)Ltot * mm);
+    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), Q, n2, mm, V.p, n2, mm, mk(0.0), QA.p, n2, mm, Ltot, s);
+    hipLaunchKernelGGL(k_sort_modes, dim3(Ltot), dim3(256), 0, s, n4, 1, W.p, gam.p, d_src.p);
   }
-  sol.Gd.alloc((size_t)L * nn);
-  sol.Gc.alloc((size_t)L * nn);
-  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), L), dim3(256), 0, s, n4, o.eig_mode, V.p, QA.p,
-                     sol.gam.p, d_src.p, k, d_dz.p, sol.Gd.p, sol.Gc.p);
+  DevBuf<zc> Gd_local, Gc_local;
+  DevBuf<zc>& Gd = full ? full->Gd : Gd_local;
+  DevBuf<zc>& Gc = full ? full->Gc : Gc_local;
+  Gd.alloc((size_t)Ltot * nn);
+  Gc.alloc((size_t)Ltot * nn);
+  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, V.p, QA.p, gam.p,
+                     d_src.p, is.dz.p, Gd.p, Gc.p);
   V.release();
   QA.release();
+  W.release();
 
-  // ---- 4. LU of every Gd_i, T_i = Gd_i^-1 Gc_{i-1}, S = Gc_{L-1} T_{L-1} ... T_1 ----
+  // ---- 4. LU of every Gd, T_g = Gd_g^-1 Gc_{g-1} (g not first in its instance), and per instance
+  //         S = Gc_last T_last ... T_first+1 by a pairwise product tree (ScatterVolume.cpp:327-336) ----
   zc* LUd = ctx.w0.p;  // Phi / P,Q no longer needed
-  hipLaunchKernelGGL(k_copy, dim3(grid_for(L * nn)), dim3(256), 0, s, L * nn, sol.Gd.p, LUd);
-  ctx.ipiv.ensure((size_t)L * n4 + n4 + n6);
+  hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * nn)), dim3(256), 0, s, (long)Ltot * nn, Gd.p, LUd);
+  ctx.ipiv.ensure((size_t)Ltot * n4 + (size_t)count * (n4 + n6));
   int* piv = ctx.ipiv.p;
-  DevBuf<int> d_info_lu(L), info2(4);
-  zgetrf_batched(n4, LUd, n4, nn, piv, n4, d_info_lu.p, L, s);
-  ctx.w1.ensure((size_t)std::max(1, L - 1) * nn);
-  zc* Tm = ctx.w1.p;
-  if (L > 1) {
-    hipLaunchKernelGGL(k_copy, dim3(grid_for((L - 1) * nn)), dim3(256), 0, s, (L - 1) * nn, sol.Gc.p, Tm);
-    zgetrs_batched(n4, n4, LUd + nn, n4, nn, piv + n4, n4, Tm, n4, nn, L - 1, s);
-  }
-  ctx.w2.ensure((size_t)nn);
-  ctx.w3.ensure((size_t)nn);
-  zc* S = ctx.w2.p;
-  if (L == 1) {
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(nn)), dim3(256), 0, s, nn, sol.Gc.p, S);
-  } else {
-    // Pm = T_{L-1} ... T_1, accumulated left to right
-    zc* Pm = ctx.w3.p;
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(nn)), dim3(256), 0, s, nn, Tm, Pm);
-    for (int i = 2; i <= L - 1; ++i) {
-      zgemm_batched(OpN, OpN, n4, n4, n4, mk(1.0), Tm + (long)(i - 1) * nn, n4, 0, Pm, n4, 0, mk(0.0), S,
-                    n4, 0, 1, s);
-      std::swap(Pm, S);
+  int* pivS = piv + (long)Ltot * n4;
+  int* pivA = pivS + (long)count * n4;
+  DevBuf<int> d_info_lu(Ltot), infoS(count), infoA(count);
+  zgetrf_batched(n4, LUd, n4, nn, piv, n4, d_info_lu.p, Ltot, s);
+  ctx.w1.ensure((size_t)Ltot * nn);
+  zc* Tm = ctx.w1.p;  // T_g at slot g
+  PtrLists pl;
+  std::vector<void*> tLU, tP, tB;
+  for (int i = 0; i < count; ++i)
+    for (int l = 1; l < is.L[i]; ++l) {
+      const long g = is.first[i] + l;
+      tLU.push_back(LUd + g * nn);
+      tP.push_back(piv + g * n4);
+      tB.push_back(Tm + g * nn);
     }
-    zgemm_batched(OpN, OpN, n4, n4, n4, mk(1.0), sol.Gc.p + (long)(L - 1) * nn, n4, 0, Pm, n4, 0, mk(0.0),
-                  S == Pm ? ctx.w3.p : S, n4, 0, 1, s);
-    if (S == Pm) S = ctx.w3.p;
+  const int nT = (int)tB.size();
+  const size_t oTLU = pl.add(tLU), oTP = pl.add(tP), oTB = pl.add(tB);
+  // product tree: level items are pointers; pairs (X_{2m+1}, X_{2m}) -> X_{2m+1} X_{2m}
+  ctx.w2.ensure((size_t)count * nn);
+  zc* Sm = ctx.w2.p;
+  ctx.w3.ensure((size_t)std::max(1, Ltot) * nn);
+  zc* pool = ctx.w3.p;
+  long pool_used = 0;
+  std::vector<std::vector<const zc*>> seq(count);
+  std::vector<int> single;  // instances with one layer: S = Gc of that layer
+  for (int i = 0; i < count; ++i) {
+    const int f = is.first[i], L = is.L[i];
+    for (int l = 1; l < L; ++l) seq[i].push_back(Tm + (long)(f + l) * nn);
+    seq[i].push_back(Gc.p + (long)(f + L - 1) * nn);
   }
+  struct Level {
+    size_t oA, oB, oC;
+    int batch;
+  };
+  std::vector<Level> levels;
+  for (;;) {
+    std::vector<void*> A, B, Cp;
+    for (int i = 0; i < count; ++i) {
+      auto& q = seq[i];
+      if (q.size() < 2) continue;
+      std::vector<const zc*> next;
+      for (size_t m = 0; m + 1 < q.size(); m += 2) {
+        zc* out = (q.size() == 2) ? Sm + (long)i * nn : pool + (pool_used++) * nn;
+        A.push_back((void*)q[m + 1]);
+        B.push_back((void*)q[m]);
+        Cp.push_back(out);
+        next.push_back(out);
+      }
+      if (q.size() % 2) next.push_back(q.back());
+      q = next;
+    }
+    if (A.empty()) break;
+    levels.push_back(Level{pl.add(A), pl.add(B), pl.add(Cp), (int)A.size()});
+  }
+  CWX_CHECK(pool_used <= std::max(1, Ltot), "product tree exceeded its pool");
+  std::vector<void*> cS, cD;
+  for (int i = 0; i < count; ++i)
+    if (is.L[i] == 1) {
+      cS.push_back(Gc.p + (long)is.first[i] * nn);
+      cD.push_back(Sm + (long)i * nn);
+    }
+  const size_t oCS = pl.add(cS), oCD = pl.add(cD);
+  // boundary and beta pointer lists
+  std::vector<void*> gd0, lu0, piv0, b0;
+  DevBuf<zc> beta_local;
+  const bool modes = (full != nullptr);
+  DevBuf<zc>& beta = full ? full->beta : beta_local;
+  if (modes && !opts[0].external) beta.alloc((size_t)Ltot * n4);
+  for (int i = 0; i < count; ++i) {
+    const long f = is.first[i];
+    gd0.push_back(Gd.p + f * nn);
+    lu0.push_back(LUd + f * nn);
+    piv0.push_back(piv + f * n4);
+    b0.push_back(beta.p ? beta.p + f * n4 : nullptr);
+  }
+  const size_t oGd0 = pl.add(gd0), oLU0 = pl.add(lu0), oPiv0 = pl.add(piv0), oB0 = pl.add(b0);
+  std::vector<Level> bsteps;  // beta_j = T_j beta_{j-1} for every instance with more than j layers
+  if (beta.p) {
+    for (int j = 1; j < is.maxL; ++j) {
+      std::vector<void*> A, B, Cp;
+      for (int i = 0; i < count; ++i)
+        if (is.L[i] > j) {
+          const long g = is.first[i] + j;
+          A.push_back(Tm + g * nn);
+          B.push_back(beta.p + (g - 1) * n4);
+          Cp.push_back(beta.p + g * n4);
+        }
+      bsteps.push_back(Level{pl.add(A), pl.add(B), pl.add(Cp), (int)A.size()});
+    }
+  }
+  pl.commit(s);
 
-  // ---- 5. boundary system: A = [Gauss; f2 | Gd0 S^-1 f3], b = [0; -f1 EF] ----
-  BoundaryArgs ba{MF, k, st.z[0], dSx, dSy, dSz0, dSz1, dEF};
-  DevBuf<zc> Y((size_t)n4 * n3), Abr((size_t)n4 * n3), A((size_t)n6 * n6);
-  sol.x.alloc(n6);
-  int* pivS = piv + (long)L * n4;
-  int* pivA = pivS + n4;
-  zgetrf_batched(n4, S, n4, nn, pivS, n4, info2.p, 1, s);
-  hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3)), dim3(256), 0, s, ba, 3, Y.p);
-  zgetrs_batched(n4, n3, S, n4, nn, pivS, n4, Y.p, n4, (long)n4 * n3, 1, s);
-  zgemm_batched(OpN, OpN, n4, n3, n4, mk(1.0), sol.Gd.p, n4, 0, Y.p, n4, 0, mk(0.0), Abr.p, n4, 0, 1, s);
-  hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6)), dim3(256), 0, s, ba, Abr.p, A.p, sol.x.p);
-  zgetrf_batched(n6, A.p, n6, (long)n6 * n6, pivA, n6, info2.p + 1, 1, s);
-  zgetrs_batched(n6, 1, A.p, n6, (long)n6 * n6, pivA, n6, sol.x.p, n6, n6, 1, s);
+  if (nT > 0) {
+    hipLaunchKernelGGL(k_copy, dim3(grid_for((long)(Ltot - 1) * nn)), dim3(256), 0, s, (long)(Ltot - 1) * nn, Gc.p,
+                       Tm + nn);
+    zgetrs_set(n4, n4, pl.mats(oTLU), n4, pl.ints(oTP), pl.mats(oTB), n4, nT, s);
+  }
+  for (const Level& lv : levels)
+    zgemm_set(OpN, OpN, n4, n4, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
+              lv.batch, s);
+  if (!cS.empty())
+    hipLaunchKernelGGL(k_copy_set, dim3(grid_for(nn), (unsigned)cS.size()), dim3(256), 0, s, nn, pl.mats(oCS),
+                       pl.mats(oCD));
+
+  // ---- 5. boundary system per instance: A = [Gauss; f2 | Gd0 S^-1 f3], b = [0; -f1 EF] ----
+  DevBuf<zc> Y((size_t)count * n4 * n3), Abr((size_t)count * n4 * n3), A((size_t)count * n6 * n6);
+  DevBuf<zc>& x = *res.x;
+  x.alloc((size_t)count * n6);
+  zgetrf_batched(n4, Sm, n4, nn, pivS, n4, infoS.p, count, s);
+  hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), count), dim3(256), 0, s, ia, 3, Y.p);
+  zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, Y.p, n4, (long)n4 * n3, count, s);
+  zgemm_set(OpN, OpN, n4, n3, n4, mk(1.0), pl.mats(oGd0), n4, strided<zc>(Y.p, (long)n4 * n3), n4, mk(0.0),
+            strided<zc>(Abr.p, (long)n4 * n3), n4, count, s);
+  hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6), count), dim3(256), 0, s, ia, Abr.p, A.p, x.p);
+  zgetrf_batched(n6, A.p, n6, (long)n6 * n6, pivA, n6, infoA.p, count, s);
+  zgetrs_batched(n6, 1, A.p, n6, (long)n6 * n6, pivA, n6, x.p, n6, n6, count, s);
 
   // ---- 6. beta recurrence, Q_check / Q_hat, Ez rows ----
-  if (!o.external) {
-    sol.beta.alloc((size_t)L * n4);
-    hipLaunchKernelGGL(k_beta0_rhs, dim3(grid_for(n4)), dim3(256), 0, s, ba, sol.x.p, sol.beta.p);
-    zgetrs_batched(n4, 1, LUd, n4, nn, piv, n4, sol.beta.p, n4, n4, 1, s);
-    for (int i = 1; i < L; ++i)
-      zgemm_batched(OpN, OpN, n4, 1, n4, mk(1.0), Tm + (long)(i - 1) * nn, n4, 0, sol.beta.p + (long)(i - 1) * n4,
-                    n4, 0, mk(0.0), sol.beta.p + (long)i * n4, n4, 0, 1, s);
-    sol.Qch.alloc((size_t)L * n3 * n4);
-    DevBuf<zc> Yz((size_t)L * MF * n4), Cn((size_t)L * MF * MF);
-    hipLaunchKernelGGL(k_q_rows, dim3(grid_for(nn), L), dim3(256), 0, s, MF, k, sol.Gd.p, sol.Gc.p,
-                       sol.beta.p, d_upz.p, d_wqz.p, sol.Qch.p, Yz.p);
-    if (o.npz) {
-      hipLaunchKernelGGL(k_zero_rows, dim3(grid_for((long)MF * n4), L), dim3(256), 0, s, n2, MF, n3, n4,
-                         sol.Qch.p, (long)n3 * n4);
-    } else {
-      hipLaunchKernelGGL(k_cn, dim3(grid_for((long)MF * MF), L), dim3(256), 0, s, M0, M1, sol.NIF.p, Cn.p);
-      zgemm_batched(OpN, OpN, MF, n4, MF, mk(1.0), Cn.p, MF, (long)MF * MF, Yz.p, MF, (long)MF * n4, mk(0.0),
-                    sol.Qch.p + n2, n3, (long)n3 * n4, L, s);
-    }
+  if (beta.p) {
+    hipLaunchKernelGGL(k_beta0_rhs, dim3(grid_for(n4), count), dim3(256), 0, s, ia, x.p, pl.mats(oB0));
+    zgetrs_set(n4, 1, pl.mats(oLU0), n4, pl.ints(oPiv0), pl.mats(oB0), n4, count, s);
+    for (const Level& lv : bsteps)
+      zgemm_set(OpN, OpN, n4, 1, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
+                lv.batch, s);
   }
+  if (full) {
+    Solution& sol = *full;
+    sol.opt = opts[0];
+    sol.st = is.st[0];
+    sol.n4 = n4;
+    sol.NIF.alloc((size_t)Ltot * MF);
+    CWX_HIP(hipMemcpyAsync(sol.NIF.p, NI, sizeof(zc) * Ltot * MF, hipMemcpyDeviceToDevice, s));
+    sol.tabs.alloc((size_t)7 * MF);
+    CWX_HIP(hipMemcpyAsync(sol.tabs.p, is.tabs.p, sizeof(zc) * 7 * MF, hipMemcpyDeviceToDevice, s));
+    if (!sol.opt.external) {
+      sol.Qch.alloc((size_t)Ltot * n3 * n4);
+      DevBuf<zc> Yz((size_t)Ltot * MF * n4), Cn((size_t)Ltot * MF * MF);
+      hipLaunchKernelGGL(k_q_rows, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, Gd.p, Gc.p, beta.p, sol.Qch.p,
+                         Yz.p);
+      if (sol.opt.npz) {
+        hipLaunchKernelGGL(k_zero_rows, dim3(grid_for((long)MF * n4), Ltot), dim3(256), 0, s, n2, MF, n3, n4,
+                           sol.Qch.p, (long)n3 * n4);
+      } else {
+        hipLaunchKernelGGL(k_cn, dim3(grid_for((long)MF * MF), Ltot), dim3(256), 0, s, M0, M1, sol.NIF.p, Cn.p);
+        zgemm_batched(OpN, OpN, MF, n4, MF, mk(1.0), Cn.p, MF, (long)MF * MF, Yz.p, MF, (long)MF * n4, mk(0.0),
+                      sol.Qch.p + n2, n3, (long)n3 * n4, Ltot, s);
+      }
+      CWX_HIP(hipStreamSynchronize(s));  // Yz and Cn are released on return
+    }
 
-  // ---- 7. .cw records ----
-  const int ZL = L + 1;
-  std::vector<Seg> segs;
-  long off = 0;
-  auto add = [&](long cnt, int kind, int layer, double zb) {
-    segs.push_back(Seg{off, cnt, kind, layer, zb});
-    sol.seg_off.push_back(off);
-    sol.seg_cnt.push_back(cnt);
-    off += cnt;
-  };
-  sol.seg_off.clear();
-  sol.seg_cnt.clear();
-  const long inner = o.external ? 0 : (long)n2 * MF;
-  add(MF, 0, 0, 0.0);  // Pi
-  for (int l = 0; l < ZL; ++l) {
-    if (l == 0) {
-      add(MF, 1, 0, st.z[0]);
-      add(inner, 3, 0, st.z[0]);
-    } else if (l == ZL - 1) {
-      add(inner, 4, l - 1, st.z[l]);
-      add(MF, 2, 0, st.z[l]);
-    } else {
-      add(inner, 4, l - 1, st.z[l]);
-      add(inner, 3, l, st.z[l]);
+    // ---- 7. .cw records ----
+    const Setup& st = sol.st;
+    const int L = Ltot, ZL = L + 1;
+    std::vector<Seg> segs;
+    long off = 0;
+    auto add = [&](long cnt, int kind, int layer, double zb) {
+      segs.push_back(Seg{off, cnt, kind, layer, 0, zb});
+      sol.seg_off.push_back(off);
+      sol.seg_cnt.push_back(cnt);
+      off += cnt;
+    };
+    sol.seg_off.clear();
+    sol.seg_cnt.clear();
+    const long inner = sol.opt.external ? 0 : (long)n2 * MF;
+    add(MF, 0, 0, 0.0);  // Pi
+    for (int l = 0; l < ZL; ++l) {
+      if (l == 0) {
+        add(MF, 1, 0, st.z[0]);
+        add(inner, 3, 0, st.z[0]);
+      } else if (l == ZL - 1) {
+        add(inner, 4, l - 1, st.z[l]);
+        add(MF, 2, 0, st.z[l]);
+      } else {
+        add(inner, 4, l - 1, st.z[l]);
+        add(inner, 3, l, st.z[l]);
+      }
     }
+    sol.nrec = off;
+    sol.rec.alloc((size_t)off * 6);
+    DevBuf<Seg> d_segs;
+    upload(d_segs, segs, s);
+    long maxcnt = 1;
+    for (auto& g : segs) maxcnt = std::max(maxcnt, g.cnt);
+    hipLaunchKernelGGL(k_records, dim3(grid_for(maxcnt), (unsigned)segs.size()), dim3(256), 0, s, ia, x.p, gam.p,
+                       sol.opt.external ? nullptr : sol.Qch.p, d_segs.p, sol.rec.p);
+    CWX_HIP(hipGetLastError());
+    CWX_HIP(hipStreamSynchronize(s));  // d_segs is released on return
   }
-  sol.nrec = off;
-  sol.rec.alloc((size_t)off * 6);
-  DevBuf<Seg> d_segs;
-  upload(d_segs, segs, s);
-  long maxcnt = 1;
-  for (auto& g : segs) maxcnt = std::max(maxcnt, g.cnt);
-  RecArgs ra{MF, k, o.n[0], o.n[1], dSx, dSy, dSz0, dSz1, dEF, sol.x.p, sol.gam.p,
-             o.external ? nullptr : sol.Qch.p};
-  hipLaunchKernelGGL(k_records, dim3(grid_for(maxcnt), (unsigned)segs.size()), dim3(256), 0, s, ra, d_segs.p,
-                     sol.rec.p);
   CWX_HIP(hipGetLastError());
 
-  // ---- status ----
-  std::vector<int> hinfo(L), hlu(L), hinfo2(2);
-  CWX_HIP(hipMemcpyAsync(hinfo.data(), d_info.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hlu.data(), d_info_lu.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hinfo2.data(), info2.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  // ---- status per instance ----
+  std::vector<int> hinfo(Ltot), hlu(Ltot), hS(count), hA(count);
+  CWX_HIP(hipMemcpyAsync(hinfo.data(), d_info.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hlu.data(), d_info_lu.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hS.data(), infoS.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hA.data(), infoA.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < L; ++i)
-    if (hinfo[i] != 0) {
-      sol.info = hinfo[i];
-      sol.info_msg = "layer " + std::to_string(i) + ": eigensolver did not converge (info " +
-                     std::to_string(hinfo[i]) + ")";
-      break;
+  res.info.assign(count, 0);
+  res.msg.assign(count, std::string());
+  for (int i = 0; i < count; ++i) {
+    int& inf = res.info[i];
+    std::string& m = res.msg[i];
+    for (int l = 0; l < is.L[i] && !inf; ++l) {
+      const int g = is.first[i] + l;
+      if (hinfo[g] != 0) {
+        inf = hinfo[g];
+        m = "layer " + std::to_string(l) + ": eigensolver did not converge (info " + std::to_string(hinfo[g]) + ")";
+      }
     }
-  for (int i = 0; i < L && !sol.info; ++i)
-    if (hlu[i] != 0) {
-      sol.info = hlu[i];
-      sol.info_msg = "layer " + std::to_string(i) + ": singular Gd (zero pivot " + std::to_string(hlu[i]) + ")";
+    for (int l = 0; l < is.L[i] && !inf; ++l) {
+      const int g = is.first[i] + l;
+      if (hlu[g] != 0) {
+        inf = hlu[g];
+        m = "layer " + std::to_string(l) + ": singular Gd (zero pivot " + std::to_string(hlu[g]) + ")";
+      }
     }
-  if (!sol.info && hinfo2[0]) { sol.info = hinfo2[0]; sol.info_msg = "singular transfer product"; }
-  if (!sol.info && hinfo2[1]) { sol.info = hinfo2[1]; sol.info_msg = "singular boundary system"; }
+    if (!inf && hS[i]) { inf = hS[i]; m = "singular transfer product"; }
+    if (!inf && hA[i]) { inf = hA[i]; m = "singular boundary system"; }
+  }
+}
+
+}  // namespace
+
+void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
+                  Solution& sol) {
+  CWX_CHECK(L <= 65535, "too many layers");
+  SetResult res;
+  res.x = &sol.x;
+  res.full = &sol;
+  solve_set(ctx, {o}, d_sample, {L}, s1, s2, res);
+  sol.info = res.info[0];
+  sol.info_msg = res.msg[0];
+}
+
+int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
+                int s1, int s2, size_t max_bytes, zc* d_x, std::vector<int>& info, std::string& first_msg) {
+  const int count = (int)opts.size();
+  CWX_CHECK(count >= 1 && (int)Ls.size() == count, "batch is empty");
+  // orders of the first instance fix the per-layer working set
+  const Setup st0 = make_setup(opts[0], Ls[0], s1, s2);
+  const long n4 = 4L * st0.MF, n2 = 2L * st0.MF, n6 = 6L * st0.MF;
+  const long nn = n4 * n4;
+  const long n_eig = opts[0].eig_mode ? n2 : n4;
+  // bytes per layer: Phi or P,Q + LU (w0), PQ + T (w1), pool (w3), Gd, Gc, eigen work (Z, X), V, QA
+  const size_t per_layer = sizeof(zc) * (size_t)(nn * 5 + n_eig * n_eig * 4 + (long)s1 * s2 * 2);
+  const size_t per_inst = sizeof(zc) * (size_t)(nn + 2 * n4 * 3 * st0.MF + n6 * n6);
+  if (max_bytes == 0) max_bytes = (size_t)64 << 30;
+  info.assign(count, 0);
+  first_msg.clear();
+  int worst = 0;
+  const long npix = (long)s1 * s2;
+  long layer0 = 0;
+  for (int i0 = 0; i0 < count;) {
+    // grow the chunk while it fits the budget and the launch limits
+    int i1 = i0;
+    size_t bytes = 0;
+    long layers = 0;
+    while (i1 < count) {
+      const size_t add = per_layer * Ls[i1] + per_inst;
+      if (i1 > i0 && (bytes + add > max_bytes || layers + Ls[i1] > 60000)) break;
+      bytes += add;
+      layers += Ls[i1];
+      ++i1;
+    }
+    std::vector<VolumeOptions> o(opts.begin() + i0, opts.begin() + i1);
+    std::vector<int> l(Ls.begin() + i0, Ls.begin() + i1);
+    DevBuf<zc> xs;
+    SetResult res;
+    res.x = &xs;
+    solve_set(ctx, o, d_samples + layer0 * npix, l, s1, s2, res);
+    CWX_HIP(hipMemcpyAsync(d_x + (long)i0 * n6, xs.p, sizeof(zc) * n6 * (i1 - i0), hipMemcpyDeviceToDevice,
+                           ctx.stream));
+    CWX_HIP(hipStreamSynchronize(ctx.stream));
+    for (int i = i0; i < i1; ++i) {
+      info[i] = res.info[i - i0];
+      if (info[i] && !worst) {
+        worst = info[i];
+        first_msg = "instance " + std::to_string(i) + ": " + res.msg[i - i0];
+      }
+    }
+    layer0 += layers;
+    i0 = i1;
+  }
+  return worst;
 }
 
 // ------------------------------------------------------------------------------------------------
