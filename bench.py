@@ -147,6 +147,50 @@ def cpu_field_baseline(sol, g):
             "sample": f"{ny} rows of plane {iz} ({npts} points, {nwaves} waves each), direct FP64 sum"}
 
 
+def batched_leg(ctx, count: int, plain_count: int, dev):
+    """The batched sweep (256 wavelengths x 8 angles over 4-24-layer stacks, 11 x 11 orders): the first
+    `count` instances solved together by cwx_solve_batch_dev, against the first `plain_count` of them
+    solved one at a time through the same entry point (same outputs, so the same work per instance)."""
+    import torch
+    import scattersim_amd as sa
+    from scattersim_amd import workloads
+    packed, Ls, params = workloads.sweep(count)
+    for p in params:
+        p["eig_mode"] = 1
+    N = packed.shape[1]
+    M0, M1 = sa.batch_orders(params, Ls, N, N)
+    n6 = 6 * M0 * M1
+    d_s = torch.from_numpy(packed).to(dev)
+    d_x = torch.zeros((count, n6), dtype=torch.complex128, device=dev)
+    d_xp = torch.zeros((plain_count, n6), dtype=torch.complex128, device=dev)
+    npix = N * N
+    offs = np.concatenate([[0], np.cumsum(Ls)[:-1]]).astype(np.int64)
+
+    def plain(i):
+        ptr = d_s.data_ptr() + int(offs[i]) * npix * 16
+        sa.solve_batch(ctx, ptr, [params[i]], shape=([int(Ls[i])], N, N), d_x=d_xp.data_ptr() + i * n6 * 16)
+
+    plain(0)                                   # warm-up
+    sa.solve_batch(ctx, d_s.data_ptr(), params[:16], shape=(Ls[:16], N, N), d_x=d_x.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sa.solve_batch(ctx, d_s.data_ptr(), params, shape=(Ls, N, N), d_x=d_x.data_ptr())
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(plain_count):
+        plain(i)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    same = bool(torch.equal(d_x[:plain_count], d_xp))
+    fl = sum(solve_flops(M0 * M1, int(L), 1) for L in Ls)
+    return {"workload": "sweep: 256 lambda x 8 angles, stacks of 4-24 layers, 11x11 orders, reduced eigen mode",
+            "instances": count, "layers": int(Ls.sum()), "batched_solves_per_s": count / tb,
+            "batched_s": tb, "plain_solves_per_s": plain_count / tp, "plain_instances": plain_count,
+            "batched_over_plain": (count / tb) / (plain_count / tp), "batched_equals_plain": same,
+            "batched_tflops_nominal": fl / tb / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +201,9 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-field", action="store_true")
+    ap.add_argument("--sweep-count", type=int, default=512, help="instances of the batched sweep leg")
+    ap.add_argument("--sweep-plain", type=int, default=24, help="of which solved one at a time")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a blit of known traffic that scales the "
                          "profiler's FETCH_SIZE / WRITE_SIZE counters)")
@@ -249,6 +296,10 @@ def main():
         cpu_field = cpu_field_baseline(sol, grid)
     sol.free()
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(ctx, args.sweep_count, min(args.sweep_plain, args.sweep_count), dev)
+
     cpu = None
     if not args.no_cpu:
         t_cpu, threads = cpu_baseline(sample, opts, min(args.cpu_layers, L))
@@ -275,6 +326,7 @@ def main():
                    "parallelism": "1 GPU"},
         "solve_tflops_nominal": solve_flops(MF, L, args.eig_mode) / (ms * 1e-3) / 1e12,
         "field": field,
+        "batched": batched,
         "roofline": {"kernel": "eigensolver stage (gehrd+unghr+hqr+trevc+gemm+normalize)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,

@@ -133,7 +133,8 @@ int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const voi
  * of instances whose device workspace stays below max_bytes (0: 64 GiB) is one launch set over all of
  * its layers.  d_x (device, count x 6MF complex) receives [Rx Ry Rz Tx Ty Tz] of every instance,
  * h_info (host, count, may be NULL) its status; *M0 and *M1 (may be NULL) the orders.  With
- * d_x == NULL only the orders are computed.  Returns the first non-zero instance status. */
+ * d_x == NULL only the orders are computed (ctx may then be NULL).  Returns the first non-zero
+ * instance status. */
 int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
                         const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
                         int* M1);

@@ -267,6 +267,19 @@ def scattervolume(ctx: Context, sample: np.ndarray, **opts) -> Solution:
     return Solution(ctx, h)
 
 
+def batch_orders(params, Ls, s1: int, s2: int):
+    """Fourier orders (M0, M1) a batch of these instances resolves to (no device work)."""
+    count = len(params)
+    pa = (_capi.cwx_volume_params * count)()
+    for i, o in enumerate(params):
+        pa[i] = _params(**o)
+    Ls = np.ascontiguousarray(Ls, np.int32)
+    M0, M1 = C.c_int(), C.c_int()
+    check(lib().cwx_solve_batch_dev(None, count, pa, _ptr(Ls), None, int(s1), int(s2), 0, None, None, C.byref(M0),
+                                    C.byref(M1)))
+    return M0.value, M1.value
+
+
 def solve_batch(ctx: Context, samples, params, max_bytes: int = 0, shape=None, d_x: Optional[int] = None):
     """Solve independent samples together (cwx_solve_batch_dev).
 

@@ -386,7 +386,7 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
                         const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
                         int* M1) {
   return guard([&]() -> int {
-    CWX_CHECK(ctx && params && L, "NULL argument");
+    CWX_CHECK(params && L, "NULL argument");
     CWX_CHECK(count >= 1, "count must be positive");
     CWX_CHECK(s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
     CWX_CHECK(max_bytes >= 0, "max_bytes must be non-negative");
@@ -400,7 +400,7 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
     if (M0) *M0 = st.M0;
     if (M1) *M1 = st.M1;
     if (!d_x) return CWX_OK;
-    CWX_CHECK(d_samples != nullptr, "NULL samples");
+    CWX_CHECK(ctx && d_samples, "NULL argument");
     set_device(ctx);
     std::vector<int> info;
     std::string msg;

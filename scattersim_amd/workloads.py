@@ -58,4 +58,52 @@ def volume32(seed: int = 0):
     return sample, opts, grid
 
 
+def _stack_masks(L: int, N: int, cell: float, rng):
+    """Material masks of one seeded stack: every layer holds 3-6 disks (r 0.5-2 um) or 2-4 rectangles
+    (sides 1-5 um), periodically wrapped in a cell x cell square."""
+    xs = (np.arange(N) + 0.5) * cell / N
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    masks = np.zeros((L, N, N), dtype=bool)
+    for l in range(L):
+        if rng.random() < 0.5:
+            for _ in range(int(rng.integers(3, 7))):
+                r = rng.uniform(0.5, 2.0)
+                cx, cy = rng.uniform(0, cell, 2)
+                masks[l] |= _wrap(X - cx, cell) ** 2 + _wrap(Y - cy, cell) ** 2 <= r * r
+        else:
+            for _ in range(int(rng.integers(2, 5))):
+                w, h = rng.uniform(1.0, 5.0, 2)
+                cx, cy = rng.uniform(0, cell, 2)
+                masks[l] |= (np.abs(_wrap(X - cx, cell)) <= w / 2) & (np.abs(_wrap(Y - cy, cell)) <= h / 2)
+    return masks
+
+
+def sweep(count: int = 2048, seed: int = 0, nstacks: int = 16, N: int = 64):
+    """Batched sweep: 256 wavelengths in [0.4, 0.8] um x 8 incidence angles in [0, 35] deg (instance i is
+    wavelength i // 8 at angle i % 8; the first `count` instances are returned).  Instance i uses stack
+    i % nstacks, one of `nstacks` seeded stacks of 4-24 layers (0.1 um each) of disks or rectangles in
+    n = 1.0 made of a Cauchy material n(lambda) = 1.5 + 0.004 / lambda^2; 12 x 12 um cell (11 x 11 orders
+    stay below the cut-off at lambda 0.8 and 35 deg), 11 x 11 orders, s-polarised incidence in the xz plane.
+
+    Returns (packed, Ls, params): the layers of all instances back to back, (sum L, N, N) complex128;
+    the layer count and the option dict of every instance."""
+    rng = np.random.default_rng(seed)
+    cell = 12.0
+    stacks = [_stack_masks(int(rng.integers(4, 25)), N, cell, rng) for _ in range(nstacks)]
+    lams = np.linspace(0.4, 0.8, 256)
+    thetas = np.deg2rad(np.linspace(0.0, 35.0, 8))
+    Ls = np.array([stacks[i % nstacks].shape[0] for i in range(count)], np.int32)
+    packed = np.empty((int(Ls.sum()), N, N), np.complex128)
+    params = []
+    off = 0
+    for i in range(count):
+        lam, th = float(lams[i // 8]), float(thetas[i % 8])
+        m = stacks[i % nstacks]
+        packed[off:off + m.shape[0]] = np.where(m, 1.5 + 0.004 / lam ** 2, 1.0)
+        off += m.shape[0]
+        params.append(dict(lam=lam, direction=(float(np.sin(th)), 0.0, float(np.cos(th))), n=(1.0, 1.0),
+                           size=(cell, cell, 0.1 * m.shape[0]), coef=(11, 11)))
+    return packed, Ls, params
+
+
 CONFIGS = {"volume32": volume32, "plain_small": plain_small}
