@@ -17,7 +17,7 @@ namespace cwx {
 
 namespace {
 
-constexpr int EIG_THREADS = 256;
+constexpr int EIG_THREADS = 512;  // threads of the QR workgroup
 constexpr int QR_W = 32;  // rotations per window
 constexpr double kUlp = 2.220446049250313e-16;      // dlamch('P')
 constexpr double kSafmin = 2.2250738585072014e-308;  // dlamch('S')
@@ -315,6 +315,10 @@ constexpr int MS_NB = 16;                               // bulges (shifts) per s
 constexpr int MS_S = 16;                                // chase steps per window
 constexpr int MS_LEN = MS_S + 2 * (MS_NB - 1) + 2;      // rows spanned by one window (48)
 constexpr int MS_NMIN = 72;                             // smallest active block swept this way
+// Leading dimensions of the LDS window blocks: a column stride of 4 (mod 32) dwords puts the 16-byte
+// accesses of 8 consecutive lanes walking along a row into distinct bank groups.
+constexpr int MS_LD = MS_LEN + 1;                       // 49 complex = 196 dwords
+constexpr int QR_LD = QR_W + 3;                         // 35 complex = 140 dwords
 
 struct Rot2 {
   zc t1, v2, t2;  // reflector I - t1 v v^H with v = (1, v2); t2 = t1 v2
@@ -495,11 +499,62 @@ __device__ void real_subdiag(zc* H, int ldh, zc* Z, int ldz, int n, int L, int I
   __syncthreads();
 }
 
+// Applies one window's rotations (left: to a column, on rows kbase + q; right: to a row, on columns
+// kbase + q) to one line, bulge by bulge.  Bulge j touches elements 30-2j .. 46-2j, so two elements
+// enter before it and two leave after it, and only that live window is held in registers.  Element q
+// lives at p[(kbase + q) * stride]; elements with q outside [qlo, qhi] read as zero and are not stored.
+template <bool LEFT>
+__device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int qhi, const Rot2* R, int nsteps) {
+  constexpr int Q0 = 2 * (MS_NB - 1);
+  zc x[MS_LEN];
+  auto ld = [&](int q) { return (q >= qlo && q <= qhi) ? p[(long)(kbase + q) * stride] : mk(0.0); };
+  auto st = [&](int q) {
+    if (q >= qlo && q <= qhi) p[(long)(kbase + q) * stride] = x[q];
+  };
+#pragma unroll
+  for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = ld(q);
+#pragma unroll
+  for (int j = 0; j < MS_NB; ++j) {
+    const int qb = Q0 - 2 * j;
+    if (j > 0) {
+      x[qb] = ld(qb);
+      x[qb + 1] = ld(qb + 1);
+    }
+#pragma unroll
+    for (int s = 0; s < MS_S; ++s) {
+      if (s < nsteps) {
+        if (LEFT) rot2_left(R[s * MS_NB + j], x[qb + s], x[qb + s + 1]);
+        else rot2_right(R[s * MS_NB + j], x[qb + s], x[qb + s + 1]);
+      }
+    }
+    st(qb + MS_S - 1);
+    st(qb + MS_S);
+  }
+#pragma unroll
+  for (int q = 0; q < MS_S - 1; ++q) st(q);
+}
+
+// Applies the single-shift rotations 0..wlen-1 (rotation j on elements j, j+1) to one line whose
+// element j lives at p[j * stride]; two elements are live at a time.
+template <bool LEFT>
+__device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen) {
+  zc a = p[0];
+#pragma unroll 4
+  for (int j = 0; j < wlen; ++j) {
+    zc b = p[(long)(j + 1) * stride];
+    if (LEFT) rot_left(R[j], a, b);
+    else rot_right(R[j], a, b);
+    p[(long)j * stride] = a;
+    a = b;
+  }
+  p[(long)wlen * stride] = a;
+}
+
 __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
                                                      long sZ, zc* W, long sW, int* info) {
-  constexpr int BR = QR_W + 2, BC = QR_W + 3;
-  static_assert(BR * BC <= MS_LEN * (MS_LEN + 1), "window buffers share one LDS array");
-  __shared__ zc MB[MS_LEN * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
+  constexpr int BR = QR_LD, BC = QR_W + 3;  // single-shift block: QR_W + 2 rows (leading dimension QR_LD)
+  static_assert(BR * BC <= MS_LD * (MS_LEN + 1), "window buffers share one LDS array");
+  __shared__ zc MB[MS_LD * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
   __shared__ Rot rots[QR_W];
   __shared__ Rot2 rots2[MS_S * MS_NB];
   __shared__ zc shifts[MS_NB], betas[MS_NB];
@@ -521,7 +576,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
   }
   const int ilo = 0, ihi = n - 1;
   const double smlnum = kSafmin * ((double)n / kUlp);
-  constexpr int kScratch = MS_LEN * (MS_LEN + 1);
+  constexpr int kScratch = MS_LD * (MS_LEN + 1);
   if (n <= kScratch) {
     real_subdiag(H, ldh, Z, ldz, n, ilo, ihi, MB);
   } else {
@@ -600,7 +655,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
         const int kbase = L + t0 - 2 * (MS_NB - 1);
         const int rlo = max(kbase, L), rhi = min(kbase + MS_LEN - 1, I);
         const int clo = max(kbase - 1, L);
-        auto MBp = [&](int r, int c) -> zc& { return MB[(c - kbase + 1) * MS_LEN + (r - kbase)]; };
+        auto MBp = [&](int r, int c) -> zc& { return MB[(c - kbase + 1) * MS_LD + (r - kbase)]; };
         const int nr = rhi - rlo + 1, nc = rhi - clo + 1;
         for (int e = tid; e < nr * nc; e += nt) {
           const int r = rlo + e % nr, c = clo + e / nr;
@@ -668,54 +723,12 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
         const int nfr = n - 1 - rhi, nfa = rlo, nz = n;
         const int lines = nfr + nfa + nz;
         for (int ln = tid; ln < lines; ln += nt) {
-          zc x[MS_LEN];
           if (ln < nfr) {
-            const int c = rhi + 1 + ln;
-#pragma unroll
-            for (int q = 0; q < MS_LEN; ++q) {
-              const int r = kbase + q;
-              x[q] = (r >= rlo && r <= rhi) ? Hp(r, c) : mk(0.0);
-            }
-#pragma unroll
-            for (int s = 0; s < MS_S; ++s) {
-              if (s < nsteps) {
-#pragma unroll
-                for (int j = 0; j < MS_NB; ++j) {
-                  const int q = 2 * (MS_NB - 1) + s - 2 * j;
-                  rot2_left(rots2[s * MS_NB + j], x[q], x[q + 1]);
-                }
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < MS_LEN; ++q) {
-              const int r = kbase + q;
-              if (r >= rlo && r <= rhi) Hp(r, c) = x[q];
-            }
+            ms_apply_line<true>(H + (long)(rhi + 1 + ln) * ldh, 1, kbase, rlo - kbase, rhi - kbase, rots2, nsteps);
           } else {
             const bool isz = ln >= nfr + nfa;
             const int r = isz ? ln - nfr - nfa : ln - nfr;
-            zc* base = isz ? (Z + r) : (H + r);
-            const int ld = isz ? ldz : ldh;
-#pragma unroll
-            for (int q = 0; q < MS_LEN; ++q) {
-              const int c = kbase + q;
-              x[q] = (c >= L && c <= I) ? base[(long)c * ld] : mk(0.0);
-            }
-#pragma unroll
-            for (int s = 0; s < MS_S; ++s) {
-              if (s < nsteps) {
-#pragma unroll
-                for (int j = 0; j < MS_NB; ++j) {
-                  const int q = 2 * (MS_NB - 1) + s - 2 * j;
-                  rot2_right(rots2[s * MS_NB + j], x[q], x[q + 1]);
-                }
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < MS_LEN; ++q) {
-              const int c = kbase + q;
-              if (c >= L && c <= I) base[(long)c * ld] = x[q];
-            }
+            ms_apply_line<false>(isz ? Z + r : H + r, isz ? ldz : ldh, kbase, L - kbase, I - kbase, rots2, nsteps);
           }
         }
         __syncthreads();
@@ -823,32 +836,12 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
       const int nfr = n - 1 - re, nfa = ks, nz = n;
       const int lines = nfr + nfa + nz;
       for (int ln = tid; ln < lines; ln += nt) {
-        zc x[QR_W + 1];
         if (ln < nfr) {
-          const int c = re + 1 + ln;
-#pragma unroll
-          for (int j = 0; j <= QR_W; ++j)
-            if (j <= wlen) x[j] = Hp(ks + j, c);
-#pragma unroll
-          for (int j = 0; j < QR_W; ++j)
-            if (j < wlen) rot_left(rots[j], x[j], x[j + 1]);
-#pragma unroll
-          for (int j = 0; j <= QR_W; ++j)
-            if (j <= wlen) Hp(ks + j, c) = x[j];
+          ss_apply_line<true>(H + (long)(re + 1 + ln) * ldh + ks, 1, rots, wlen);
         } else {
           const bool isz = ln >= nfr + nfa;
           const int r = isz ? ln - nfr - nfa : ln - nfr;
-          zc* base = isz ? (Z + (long)ks * ldz + r) : (H + (long)ks * ldh + r);
-          const int ld = isz ? ldz : ldh;
-#pragma unroll
-          for (int j = 0; j <= QR_W; ++j)
-            if (j <= wlen) x[j] = base[(long)j * ld];
-#pragma unroll
-          for (int j = 0; j < QR_W; ++j)
-            if (j < wlen) rot_right(rots[j], x[j], x[j + 1]);
-#pragma unroll
-          for (int j = 0; j <= QR_W; ++j)
-            if (j <= wlen) base[(long)j * ld] = x[j];
+          ss_apply_line<false>(isz ? Z + (long)ks * ldz + r : H + (long)ks * ldh + r, isz ? ldz : ldh, rots, wlen);
         }
       }
       __syncthreads();
