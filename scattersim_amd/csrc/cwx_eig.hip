@@ -141,7 +141,7 @@ __device__ inline void rank1_sub(zc* __restrict__ A, int lda, int rows, int cols
 // ------------------------------------------------------------------------------------------------
 // 1. Hessenberg reduction (zgehd2).  A(j+1:n, j) -> beta e1, v stored below, tau[j].
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(HH_THREADS) void k_gehrd(int n, zc* A, int lda, long sA, zc* tau) {
+__global__ __launch_bounds__(HH_THREADS) void k_gehrd(int n, int j0, zc* A, int lda, long sA, zc* tau) {
   extern __shared__ zc smem[];
   zc* v = smem;           // n
   zc* w = smem + n;       // n
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(HH_THREADS) void k_gehrd(int n, zc* A, int lda, lon
   A += sA * b;
   tau += (long)n * b;
   const int tid = threadIdx.x;
-  for (int j = 0; j < n - 2; ++j) {
+  for (int j = j0; j < n - 2; ++j) {
     const int m = n - j - 1;  // length of x = A(j+1:n, j)
     zc* x = A + (long)j * lda + j + 1;
     double ss = 0.0;
@@ -192,10 +192,11 @@ __global__ __launch_bounds__(HH_THREADS) void k_gehrd(int n, zc* A, int lda, lon
 }
 
 // ------------------------------------------------------------------------------------------------
-// 2. Q = H(0) H(1) ... H(n-3) applied to the identity (zung2r on the trailing block), then the
-//    Householder vectors below the subdiagonal of A are cleared so that A holds H.
+// 2. Q = H(0) H(1) ... H(n-3) applied to the identity, from the last reflector backwards (zung2r on
+//    the trailing block): this kernel sets Z = I and applies H(n-3) .. H(jlo); the reflectors of the
+//    blocked panels follow as block reflectors (zlarfb) by batched GEMMs.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, zc* A, int lda, long sA, const zc* tau,
+__global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, int jlo, const zc* A, int lda, long sA, const zc* tau,
                                                       zc* Z, int ldz, long sZ) {
   extern __shared__ zc smem[];
   zc* v = smem;      // n
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, zc* A, int lda, lon
     Z[(long)c * ldz + r] = mk(r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  for (int j = n - 3; j >= 0; --j) {
+  for (int j = n - 3; j >= jlo; --j) {
     const zc t = tau[j];
     if (t.re == 0.0 && t.im == 0.0) continue;
     const int m = n - j - 1;
@@ -223,9 +224,146 @@ __global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, zc* A, int lda, lon
     rank1_sub(Zb, ldz, m, m, v, w, false);
     __syncthreads();
   }
-  for (long e = tid; e < (long)n * n; e += blockDim.x) {
-    int r = (int)(e % n), c = (int)(e / n);
+}
+
+// The Householder vectors below the subdiagonal are cleared so that A holds H.
+__global__ void k_clear_below_sub(int n, zc* A, int lda, long sA) {
+  A += sA * blockIdx.y;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
     if (r > c + 1) A[(long)c * lda + r] = mk(0.0);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 1b. Blocked Hessenberg reduction (zgehrd with zlahr2 panels).  Panel i0 .. i0+nb-1 is reduced by
+//     one workgroup per matrix, reading the trailing matrix once per column (the product with the
+//     new Householder vector, corrected by the panel's earlier reflectors through Y and T); the rest
+//     of the matrix is updated afterwards by batched GEMMs.  V is kept dense (unit lower trapezoidal,
+//     explicit zeros and ones), V(r, j) for matrix row i0+1+r; T is upper triangular with
+//     H(i0) ... H(i0+nb-1) = I - V T V^H; Y(i0+1:n, :) = A(i0+1:n, i0+1:n) V T.
+// ------------------------------------------------------------------------------------------------
+constexpr int HB = 32;     // panel width
+constexpr int HB_NX = 64;  // the last columns are reduced unblocked
+
+__global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc* A, int lda, long sA, zc* V,
+                                                      int ldv, long sV, zc* Y, int ldy, long sY, zc* T, long sT,
+                                                      zc* tau) {
+  extern __shared__ zc smem[];
+  zc* bcol = smem;           // m: the current column, rows i0+1..n-1; then the new column of Y
+  zc* vv = smem + n;         // m: the new Householder vector over columns c+1..n-1
+  zc* part = smem + 2 * n;   // blockDim.x
+  __shared__ zc red[HH_THREADS / 64];
+  __shared__ zc w[HB], t2[HB];
+  const int bi = blockIdx.x;
+  A += sA * bi;
+  V += sV * bi;
+  Y += sY * bi;
+  T += sT * bi;
+  tau += (long)n * bi;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  const int m = n - i0 - 1;
+  auto Ap = [&](int r, int c) -> zc& { return A[(long)c * lda + r]; };
+  auto Vp = [&](int r, int j) -> zc& { return V[(long)j * ldv + r]; };
+  auto Yp = [&](int r, int j) -> zc& { return Y[(long)j * ldy + r]; };
+  auto Tp = [&](int r, int j) -> zc& { return T[(long)j * HB + r]; };
+  for (int j = 0; j < nb; ++j) {
+    const int c = i0 + j;
+    for (int r = tid; r < m; r += nt) bcol[r] = Ap(i0 + 1 + r, c);
+    __syncthreads();
+    if (j > 0) {
+      // b -= Y(:, 0:j) conj(V(j-1, 0:j)): the right update of this column by the earlier reflectors
+      for (int r = tid; r < m; r += nt) {
+        zc acc = bcol[r];
+        for (int q = 0; q < j; ++q) acc = acc - Yp(r, q) * conj(Vp(j - 1, q));
+        bcol[r] = acc;
+      }
+      __syncthreads();
+      // b -= V T^H V^H b: the left update by the earlier reflectors
+      for (int q = wid; q < j; q += nw) {
+        zc acc = mk(0.0);
+        for (int r = q + lane; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), bcol[r]);
+        acc.re = wave_sum(acc.re);
+        acc.im = wave_sum(acc.im);
+        if (lane == 0) w[q] = acc;
+      }
+      __syncthreads();
+      if (tid < j) {
+        zc acc = mk(0.0);
+        for (int q = 0; q <= tid; ++q) acc = acc + conj(Tp(q, tid)) * w[q];
+        t2[tid] = acc;
+      }
+      __syncthreads();
+      for (int r = tid; r < m; r += nt) {
+        zc acc = bcol[r];
+        for (int q = 0; q < j; ++q) acc = acc - Vp(r, q) * t2[q];
+        bcol[r] = acc;
+      }
+      __syncthreads();
+    }
+    // reflector annihilating b(j+1 .. m-1) (zlarfg): alpha = b(j) is the subdiagonal entry A(c+1, c)
+    double ss = 0.0;
+    for (int r = j + 1 + tid; r < m; r += nt) ss += norm2(bcol[r]);
+    const zc tot = block_sum(mk(ss), red);
+    const double xnorm = sqrt(tot.re);
+    const zc alpha = bcol[j];
+    zc tj = mk(0.0);
+    double beta = alpha.re;
+    const bool trivial = (xnorm == 0.0 && alpha.im == 0.0);
+    zc scal = mk(1.0);
+    if (!trivial) {
+      beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
+      tj = mk((beta - alpha.re) / beta, -alpha.im / beta);
+      scal = zdiv(mk(1.0), alpha - mk(beta));
+    }
+    for (int r = tid; r < m; r += nt) {
+      zc vr;
+      if (r < j) {
+        vr = mk(0.0);
+        Ap(i0 + 1 + r, c) = bcol[r];
+      } else if (r == j) {
+        vr = mk(1.0);
+        Ap(i0 + 1 + r, c) = trivial ? alpha : mk(beta);
+      } else {
+        vr = trivial ? bcol[r] : bcol[r] * scal;
+        Ap(i0 + 1 + r, c) = vr;
+      }
+      Vp(r, j) = vr;
+      if (r >= j) vv[r - j] = vr;
+    }
+    if (tid == 0) tau[c] = tj;
+    __syncthreads();
+    // y = A(i0+1:n, c+1:n) v  (the trailing matrix, not yet touched by this panel's right updates)
+    gemv_rows(A + (long)(c + 1) * lda + i0 + 1, lda, m, m - j, vv, mk(1.0), bcol, part);
+    // t = V(:, 0:j)^H v
+    for (int q = wid; q < j; q += nw) {
+      zc acc = mk(0.0);
+      for (int r = j + lane; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), vv[r - j]);
+      acc.re = wave_sum(acc.re);
+      acc.im = wave_sum(acc.im);
+      if (lane == 0) w[q] = acc;
+    }
+    __syncthreads();
+    // Y(:, j) = tau (y - Y(:, 0:j) t)
+    for (int r = tid; r < m; r += nt) {
+      zc acc = bcol[r];
+      for (int q = 0; q < j; ++q) acc = acc - Yp(r, q) * w[q];
+      Yp(r, j) = tj * acc;
+    }
+    // T(0:j, j) = -tau T(0:j, 0:j) t,  T(j, j) = tau
+    if (tid <= j) {
+      zc acc = mk(0.0);
+      if (tid < j) {
+        for (int q = tid; q < j; ++q) acc = acc + Tp(tid, q) * w[q];
+        acc = -(tj * acc);
+      } else {
+        acc = tj;
+      }
+      Tp(tid, j) = acc;
+    }
+    if (tid >= j + 1 && tid < HB) Tp(tid, j) = mk(0.0);
+    __syncthreads();
   }
 }
 
@@ -968,6 +1106,10 @@ void EigWork::ensure(int n, int batch) {
   X.ensure(nn);
   tau.ensure((size_t)n * batch);
   info.ensure((size_t)batch);
+  const size_t npan = (size_t)std::max(1, n / HB + 1);
+  Y.ensure((size_t)n * HB * batch);
+  T.ensure(npan * HB * HB * batch);
+  W.ensure((size_t)2 * n * HB * batch);
 }
 
 void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
@@ -977,10 +1119,55 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   work.ensure(n, batch);
   const long nn = (long)n * n;
   const size_t sh2 = sizeof(zc) * (2 * (size_t)n + HH_THREADS);
-  CWX_CHECK(sh2 <= 64 * 1024, "zgeev: n too large for the LDS work vectors");
-  hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(HH_THREADS), sh2, s, n, A, lda, sA, work.tau.p);
-  hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(HH_THREADS), sh2, s, n, A, lda, sA, work.tau.p,
-                     work.Z.p, n, nn);
+  CWX_CHECK(sh2 + sizeof(zc) * (HH_THREADS / 64 + 2 * HB) <= 64 * 1024, "zgeev: n too large for the LDS work vectors");
+  // ---- Hessenberg reduction: blocked panels, then the last columns unblocked ----
+  const long npan_max = n / HB + 1;
+  const long sY = (long)n * HB, sT = npan_max * HB * HB, sWk = 2L * n * HB;
+  zc* Vd = work.X.p;  // dense V of every panel (n x n per matrix); X is needed only after the QR
+  zc* Wk = work.W.p;
+  zc* Wk2 = work.W.p + (long)n * HB;
+  auto S = [](const zc* p, long st) { return strided<zc>(p, st); };
+  int i0 = 0, npan = 0;
+  for (; i0 < n - 1 - HB_NX; i0 += HB, ++npan) {
+    const int nb = std::min(HB, n - 1 - i0);
+    const int m = n - i0 - 1;
+    const int nc = n - i0 - nb;
+    zc* Vp = Vd + (long)npan * HB * n;
+    zc* Tp = work.T.p + (long)npan * HB * HB;
+    hipLaunchKernelGGL(k_lahr2, dim3(batch), dim3(HH_THREADS), sh2, s, n, i0, nb, A, lda, sA, Vp, n, nn,
+                       work.Y.p + i0 + 1, n, sY, Tp, sT, work.tau.p);
+    // Y(0:i0+1, :) = A(0:i0+1, i0+1:n) V T
+    zgemm_set(OpN, OpN, i0 + 1, nb, m, mk(1.0), S(A + (long)(i0 + 1) * lda, sA), lda, S(Vp, nn), n, mk(0.0),
+              S(Wk, sWk), n, batch, s);
+    zgemm_set(OpN, OpN, i0 + 1, nb, nb, mk(1.0), S(Wk, sWk), n, S(Tp, sT), HB, mk(0.0), S(work.Y.p, sY), n, batch,
+              s);
+    // right update of the trailing columns: A(:, i0+nb:n) -= Y V(i0+nb:n, :)^H
+    zgemm_set(OpN, OpC, n, nc, nb, mk(-1.0), S(work.Y.p, sY), n, S(Vp + (nb - 1), nn), n, mk(1.0),
+              S(A + (long)(i0 + nb) * lda, sA), lda, batch, s);
+    // rows 0..i0 of the panel columns: A(0:i0+1, i0+1:i0+nb) -= Y(0:i0+1, 0:nb-1) V1^H
+    if (nb > 1)
+      zgemm_set(OpN, OpC, i0 + 1, nb - 1, nb - 1, mk(-1.0), S(work.Y.p, sY), n, S(Vp, nn), n, mk(1.0),
+                S(A + (long)(i0 + 1) * lda, sA), lda, batch, s);
+    // left update: A(i0+1:n, i0+nb:n) = (I - V T^H V^H) A(i0+1:n, i0+nb:n)
+    zc* Asub = A + (long)(i0 + nb) * lda + i0 + 1;
+    zgemm_set(OpC, OpN, nb, nc, m, mk(1.0), S(Vp, nn), n, S(Asub, sA), lda, mk(0.0), S(Wk, sWk), HB, batch, s);
+    zgemm_set(OpC, OpN, nb, nc, nb, mk(1.0), S(Tp, sT), HB, S(Wk, sWk), HB, mk(0.0), S(Wk2, sWk), HB, batch, s);
+    zgemm_set(OpN, OpN, m, nc, nb, mk(-1.0), S(Vp, nn), n, S(Wk2, sWk), HB, mk(1.0), S(Asub, sA), lda, batch, s);
+  }
+  if (i0 < n - 2) hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(HH_THREADS), sh2, s, n, i0, A, lda, sA, work.tau.p);
+  // ---- Q: the unblocked reflectors backwards, then the panels' block reflectors backwards ----
+  hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(HH_THREADS), sh2, s, n, i0, A, lda, sA, work.tau.p, work.Z.p, n,
+                     nn);
+  for (int p = npan - 1; p >= 0; --p) {
+    const int j0 = p * HB, nb = std::min(HB, n - 1 - j0), m = n - j0 - 1;
+    zc* Vp = Vd + (long)p * HB * n;
+    zc* Tp = work.T.p + (long)p * HB * HB;
+    zc* Zsub = work.Z.p + (long)(j0 + 1) * n + j0 + 1;
+    zgemm_set(OpC, OpN, nb, m, m, mk(1.0), S(Vp, nn), n, S(Zsub, nn), n, mk(0.0), S(Wk, sWk), HB, batch, s);
+    zgemm_set(OpN, OpN, nb, m, nb, mk(1.0), S(Tp, sT), HB, S(Wk, sWk), HB, mk(0.0), S(Wk2, sWk), HB, batch, s);
+    zgemm_set(OpN, OpN, m, m, nb, mk(-1.0), S(Vp, nn), n, S(Wk2, sWk), HB, mk(1.0), S(Zsub, nn), n, batch, s);
+  }
+  hipLaunchKernelGGL(k_clear_below_sub, dim3(cdiv(n * n, 256 * 4), batch), dim3(256), 0, s, n, A, lda, sA);
   hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n, nn, W,
                      sW, info);
   const double smlnum = kSafmin * ((double)n / kUlp);

@@ -103,6 +103,7 @@ struct EigWork {
   DevBuf<zc> X;      // n*n*batch triangular eigenvectors of T
   DevBuf<zc> tau;    // n*batch
   DevBuf<int> info;  // batch
+  DevBuf<zc> Y, T, W; // blocked Hessenberg reduction: panel Y, T of every panel, GEMM scratch
   void ensure(int n, int batch);
 };
 
