@@ -94,6 +94,157 @@ __global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int
 }
 
 // ------------------------------------------------------------------------------------------------
+// Matrix-vector products (n <= 4 right-hand sides, op(A) = A): 64 rows per workgroup, the four
+// wavefronts take interleaved quarters of K and combine their partial sums in LDS (a fixed order, so
+// results do not depend on scheduling).
+// ------------------------------------------------------------------------------------------------
+constexpr int GV_MAXN = 4;
+
+__global__ __launch_bounds__(256) void k_zgemv(int m, int n, int k, zc alpha, MatSet Aset, int lda, MatSet Bset,
+                                               int ldb, zc beta, MatSet Cset, int ldc) {
+  __shared__ zc part[4][GV_MAXN][64];
+  const int b = blockIdx.y;
+  const zc* __restrict__ A = Aset.at(b);
+  const zc* __restrict__ B = Bset.at(b);
+  zc* __restrict__ C = Cset.at(b);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  zc acc[GV_MAXN][2];
+#pragma unroll
+  for (int j = 0; j < GV_MAXN; ++j) acc[j][0] = acc[j][1] = mk(0.0);
+  if (r < m) {
+    int c = wv;
+    for (; c + 4 < k; c += 8) {
+      const zc a0 = A[(long)c * lda + r], a1 = A[(long)(c + 4) * lda + r];
+#pragma unroll
+      for (int j = 0; j < GV_MAXN; ++j)
+        if (j < n) {
+          fma_acc(acc[j][0], a0, B[(long)j * ldb + c]);
+          fma_acc(acc[j][1], a1, B[(long)j * ldb + c + 4]);
+        }
+    }
+    for (; c < k; c += 4) {
+      const zc a0 = A[(long)c * lda + r];
+#pragma unroll
+      for (int j = 0; j < GV_MAXN; ++j)
+        if (j < n) fma_acc(acc[j][0], a0, B[(long)j * ldb + c]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < GV_MAXN; ++j) part[wv][j][lane] = acc[j][0] + acc[j][1];
+  __syncthreads();
+  if (wv == 0 && r < m) {
+    const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
+    for (int j = 0; j < n; ++j) {
+      const zc sum = (part[0][j][lane] + part[1][j][lane]) + (part[2][j][lane] + part[3][j][lane]);
+      zc* cp = C + (long)j * ldc + r;
+      zc v = alpha * sum;
+      if (!beta0) v = v + beta * (*cp);
+      *cp = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// GEMM on the FP64 matrix cores: 64x64 output tile per 256-lane workgroup, each wavefront a 32x32
+// quarter as 2x2 tiles of v_mfma_f64_16x16x4f64.  A complex product is four real products
+// (Re Re, Im Im, Re Im, Im Re) into three accumulators; K is staged through LDS in steps of 16 as
+// separate real and imaginary planes.  The operands enter the instruction swapped (B tile as its
+// first operand), so each lane's results are consecutive rows of the column-major C.
+// Lane layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// D[(l / 16) + 4q][l % 16] in register q of lane l.
+// ------------------------------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int MK = 16;
+
+__global__ __launch_bounds__(256, 2) void k_zgemm_mfma(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet Aset,
+                                                    int lda, MatSet Bset, int ldb, zc beta, MatSet Cset, int ldc) {
+  __shared__ double Ar[MK][GM + 1], Ai[MK][GM + 1], Br[MK][GN + 1], Bi[MK][GN + 1];
+  const int b = blockIdx.z;
+  const zc* __restrict__ A = Aset.at(b);
+  const zc* __restrict__ B = Bset.at(b);
+  zc* __restrict__ C = Cset.at(b);
+  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+  d4 P[2][2], Q[2][2], R[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      P[i][j] = d4{0, 0, 0, 0};
+      Q[i][j] = d4{0, 0, 0, 0};
+      R[i][j] = d4{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < k; k0 += MK) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      int r, kk;
+      if (opa == OpN) { r = e & (GM - 1); kk = e >> 6; }
+      else { kk = e & (MK - 1); r = e >> 4; }
+      const int gr = m0 + r, gk = k0 + kk;
+      const zc v = (gr < m && gk < k) ? load_op(A, lda, opa, gr, gk) : mk(0.0);
+      Ar[kk][r] = v.re;
+      Ai[kk][r] = v.im;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      int kk, c;
+      if (opb == OpN) { kk = e & (MK - 1); c = e >> 4; }
+      else { c = e & (GN - 1); kk = e >> 6; }
+      const int gk = k0 + kk, gc = n0 + c;
+      const zc v = (gk < k && gc < n) ? load_op(B, ldb, opb, gk, gc) : mk(0.0);
+      Br[kk][c] = v.re;
+      Bi[kk][c] = v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < MK; k4 += 4) {
+      const int kk = k4 + (lane >> 4);
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[kk][wm + 16 * i + (lane & 15)];
+        ai[i] = Ai[kk][wm + 16 * i + (lane & 15)];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        br[j] = Br[kk][wn + 16 * j + (lane & 15)];
+        bi[j] = Bi[kk][wn + 16 * j + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          P[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], P[i][j], 0, 0, 0);
+          Q[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ai[i], Q[i][j], 0, 0, 0);
+          R[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ar[i], R[i][j], 0, 0, 0);
+          R[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ai[i], R[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gr = m0 + wm + 16 * i + (lane & 15);
+        const int gc = n0 + wn + 16 * j + (lane >> 4) + 4 * q;
+        if (gr < m && gc < n) {
+          zc* cp = C + (long)gc * ldc + gr;
+          zc v = alpha * mk(P[i][j][q] - Q[i][j][q], R[i][j][q]);
+          if (!beta0) v = v + beta * (*cp);
+          *cp = v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
 // LU panel: one workgroup per matrix factors columns [j0, j0+nb) over rows [j0, n) with partial
 // pivoting (pivot = first maximum of |re|+|im|, as izamax).  Swaps are applied to the panel columns
 // only; the other columns are swapped by k_lu_swap_trsm.
@@ -285,6 +436,12 @@ void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda,
   if (m <= 0 || n <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgemm: batch too large");
   CWX_CHECK(k > 0, "zgemm: k must be positive");
+  if (n <= GV_MAXN && opa == OpN && opb == OpN) {
+    hipLaunchKernelGGL(k_zgemv, dim3(cdiv(m, 64), batch), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                       ldc);
+    CWX_HIP(hipGetLastError());
+    return;
+  }
   dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
   hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   CWX_HIP(hipGetLastError());
@@ -314,26 +471,43 @@ void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int 
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
   hipLaunchKernelGGL(k_laswp, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, n, nrhs, B, ldb, ipiv);
+  // Blocks of TS_NB rows: inside a block, diagonal LU_NB blocks and updates of the block's own rows;
+  // the rows below (above) are updated once per block, which bounds the passes over B.
+  constexpr int TS_NB = 128;
   // forward: L (unit lower)
-  for (int j0 = 0; j0 < n; j0 += LU_NB) {
-    const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 1, LU, lda, nrhs, B,
-                       ldb);
-    const int rest = n - j0 - nb;
-    if (rest > 0)
-      zgemm_set(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda + j0 + nb), lda, B.shifted(j0), ldb,
-                mk(1.0), B.shifted(j0 + nb), ldb, batch, s);
+  for (int b0 = 0; b0 < n; b0 += TS_NB) {
+    const int bn = std::min(TS_NB, n - b0);
+    for (int j0 = b0; j0 < b0 + bn; j0 += LU_NB) {
+      const int nb = std::min(LU_NB, b0 + bn - j0);
+      hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 1, LU, lda, nrhs, B,
+                         ldb);
+      const int rest = b0 + bn - j0 - nb;
+      if (rest > 0)
+        zgemm_set(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda + j0 + nb), lda, B.shifted(j0), ldb,
+                  mk(1.0), B.shifted(j0 + nb), ldb, batch, s);
+    }
+    const int below = n - b0 - bn;
+    if (below > 0)
+      zgemm_set(OpN, OpN, below, nrhs, bn, mk(-1.0), LU.shifted((long)b0 * lda + b0 + bn), lda, B.shifted(b0), ldb,
+                mk(1.0), B.shifted(b0 + bn), ldb, batch, s);
   }
   // backward: U
-  const int nblk = (n + LU_NB - 1) / LU_NB;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int j0 = bi * LU_NB;
-    const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 0, LU, lda, nrhs, B,
-                       ldb);
-    if (j0 > 0)
-      zgemm_set(OpN, OpN, j0, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda), lda, B.shifted(j0), ldb, mk(1.0), B,
-                ldb, batch, s);
+  const int nbig = (n + TS_NB - 1) / TS_NB;
+  for (int bi = nbig - 1; bi >= 0; --bi) {
+    const int b0 = bi * TS_NB, bn = std::min(TS_NB, n - b0);
+    const int nsub = (bn + LU_NB - 1) / LU_NB;
+    for (int si = nsub - 1; si >= 0; --si) {
+      const int j0 = b0 + si * LU_NB;
+      const int nb = std::min(LU_NB, b0 + bn - j0);
+      hipLaunchKernelGGL(k_trsm_block, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, j0, nb, 0, LU, lda, nrhs, B,
+                         ldb);
+      if (j0 > b0)
+        zgemm_set(OpN, OpN, j0 - b0, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda + b0), lda, B.shifted(j0), ldb,
+                  mk(1.0), B.shifted(b0), ldb, batch, s);
+    }
+    if (b0 > 0)
+      zgemm_set(OpN, OpN, b0, nrhs, bn, mk(-1.0), LU.shifted((long)b0 * lda), lda, B.shifted(b0), ldb, mk(1.0), B, ldb,
+                batch, s);
   }
   CWX_HIP(hipGetLastError());
 }
