@@ -329,31 +329,275 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
   }
 }
 
-// Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
-// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.
-__global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda,
-                                                     IntSet pset) {
+// ------------------------------------------------------------------------------------------------
+// LU panel held in registers (same pivots and arithmetic order as k_lu_panel): 512 lanes as 256
+// pairs; pair p owns panel rows p + 256 q (q < LP_Q), and the panel is factored in two halves of 16
+// columns, each lane of a pair holding 8 of the half's columns.  Between the halves the first half's
+// row interchanges are applied to the second half (one gather of the composite permutation),
+// U12 = L11^-1 A12 is solved in LDS and A22 -= L21 U12 is applied in registers; at the end the second
+// half's interchanges are applied to the first half.  Panels of at most 256 LP_Q rows.
+// ------------------------------------------------------------------------------------------------
+constexpr int LP_THREADS = 512, LP_Q = 4, LP_W = 8;
+
+// Rows j0 + pos[i] take the old contents of rows j0 + src[i] in columns [cl, cl + ncol), where
+// (pos, src) is the composite permutation of the interchanges (t, piv[t]) for t in [t0, t1)
+// (panel-relative rows).  Called by the whole workgroup; ends with a barrier.
+__device__ void lp_gather_swaps(zc* A, int lda, int j0, const int* piv, int t0, int t1, int cl, int ncol, int* pos,
+                                int* src, int* npos) {
+  if (threadIdx.x == 0) {
+    int cnt = 0;
+    for (int t = t0; t < t1; ++t) {
+      const int a = t, b = piv[t];
+      if (a == b) continue;
+      int ia = -1, ib = -1;
+      for (int i = 0; i < cnt; ++i) {
+        if (pos[i] == a) ia = i;
+        if (pos[i] == b) ib = i;
+      }
+      if (ia < 0) { pos[cnt] = a; src[cnt] = a; ia = cnt++; }
+      if (ib < 0) { pos[cnt] = b; src[cnt] = b; ib = cnt++; }
+      const int tmp = src[ia];
+      src[ia] = src[ib];
+      src[ib] = tmp;
+    }
+    *npos = cnt;
+  }
+  __syncthreads();
+  const int cnt = *npos;
+  zc val = mk(0.0);
+  const int e = threadIdx.x, i = e / max(ncol, 1), c = e % max(ncol, 1);
+  const bool act = ncol > 0 && i < cnt;
+  if (act) val = A[(long)(j0 + cl + c) * lda + j0 + src[i]];
+  __syncthreads();
+  if (act) A[(long)(j0 + cl + c) * lda + j0 + pos[i]] = val;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int nb, MatSet Aset, int lda,
+                                                             IntSet pset, int* info) {
+  __shared__ double sv[LP_THREADS / 64];
+  __shared__ int si[LP_THREADS / 64];
+  __shared__ double pv;
+  __shared__ int pi;
+  __shared__ zc prow[16], orow[16];
+  __shared__ zc L11[16][17], U12[16][17];
+  __shared__ int spiv[32], gpos[64], gsrc[64], gcnt;
+  const int b = blockIdx.x;
+  zc* A = Aset.at(b);
+  int* ipiv = pset.at(b);
+  const int R = n - j0;
+  const int tid = threadIdx.x, p = tid >> 1, h = tid & 1;
+  for (int half = 0; half < 2; ++half) {
+    const int c0 = half * 16;
+    const int hn = min(16, nb - c0);
+    if (hn <= 0) break;
+    if (half == 1) {
+      // the first half's interchanges on this half's columns, then U12 = L11^-1 A12
+      lp_gather_swaps(A, lda, j0, spiv, 0, 16, 16, hn, gpos, gsrc, &gcnt);
+      for (int e = tid; e < 16 * 16; e += LP_THREADS) {
+        const int r = e & 15, c = e >> 4;
+        L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
+        U12[r][c] = (c < hn) ? A[(long)(j0 + 16 + c) * lda + j0 + r] : mk(0.0);
+      }
+      __syncthreads();
+      if (tid < hn) {
+        for (int r = 1; r < 16; ++r) {
+          zc acc = U12[r][tid];
+          for (int q = 0; q < r; ++q) acc = acc - L11[r][q] * U12[q][tid];
+          U12[r][tid] = acc;
+        }
+      }
+      __syncthreads();
+    }
+    zc x[LP_Q][LP_W];
+#pragma unroll
+    for (int q = 0; q < LP_Q; ++q) {
+      const int r = p + 256 * q;
+#pragma unroll
+      for (int c = 0; c < LP_W; ++c) {
+        const int col = c0 + h * LP_W + c;
+        x[q][c] = (r < R && col < c0 + hn) ? A[(long)(j0 + col) * lda + j0 + r] : mk(0.0);
+      }
+    }
+    if (half == 1) {
+      // A22 -= L21 U12 (rows below 16), rows 0..15 become U12
+#pragma unroll 1
+      for (int k = 0; k < 16; ++k) {
+        zc u[LP_W];
+#pragma unroll
+        for (int c = 0; c < LP_W; ++c) u[c] = U12[k][h * LP_W + c];
+#pragma unroll
+        for (int q = 0; q < LP_Q; ++q) {
+          const int r = p + 256 * q;
+          if (r >= 16 && r < R) {
+            const zc l = A[(long)(j0 + k) * lda + j0 + r];
+#pragma unroll
+            for (int c = 0; c < LP_W; ++c) x[q][c] = x[q][c] - l * u[c];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < LP_Q; ++q) {
+        const int r = p + 256 * q;
+        if (r < 16) {
+#pragma unroll
+          for (int c = 0; c < LP_W; ++c) x[q][c] = U12[r][h * LP_W + c];
+        }
+      }
+    }
+    // the column loop stays rolled; register indices are kept static by predicated selects
+#pragma unroll 1
+    for (int jj = 0; jj < hn; ++jj) {
+      const int j = c0 + jj;  // panel-relative row and column
+      const int hj = jj >> 3, cj = jj & 7;
+      double best = -1.0;
+      int bidx = R;
+      if (h == hj) {
+#pragma unroll
+        for (int q = 0; q < LP_Q; ++q) {
+          const int r = p + 256 * q;
+          zc v = mk(0.0);
+#pragma unroll
+          for (int c = 0; c < LP_W; ++c)
+            if (c == cj) v = x[q][c];
+          if (r >= j && r < R && abs1(v) > best) {
+            best = abs1(v);
+            bidx = r;
+          }
+        }
+      }
+      block_argmax(best, bidx, sv, si, &pv, &pi);
+      const int pr = pi;
+      const double pval = pv;
+      if (tid == 0) {
+        ipiv[j0 + j] = j0 + pr;
+        spiv[j] = pr;
+        if (pval == 0.0 && info[b] == 0) info[b] = j0 + j + 1;
+      }
+      // interchange rows j and pr within this half
+      const int pp = pr & 255, qp = pr >> 8;
+      if (p == pp) {
+#pragma unroll
+        for (int q = 0; q < LP_Q; ++q)
+          if (q == qp) {
+#pragma unroll
+            for (int c = 0; c < LP_W; ++c) prow[h * LP_W + c] = x[q][c];
+          }
+      }
+      if (p == j) {
+#pragma unroll
+        for (int c = 0; c < LP_W; ++c) orow[h * LP_W + c] = x[0][c];
+      }
+      __syncthreads();
+      if (pr != j) {
+        if (p == j) {
+#pragma unroll
+          for (int c = 0; c < LP_W; ++c) x[0][c] = prow[h * LP_W + c];
+        }
+        if (p == pp) {
+#pragma unroll
+          for (int q = 0; q < LP_Q; ++q)
+            if (q == qp) {
+#pragma unroll
+              for (int c = 0; c < LP_W; ++c) x[q][c] = orow[h * LP_W + c];
+            }
+        }
+      }
+      const zc inv = (pval != 0.0) ? zdiv(mk(1.0), prow[jj]) : mk(0.0);
+#pragma unroll
+      for (int q = 0; q < LP_Q; ++q) {
+        const int r = p + 256 * q;
+        const bool act = (r > j && r < R);
+        zc v = mk(0.0);
+#pragma unroll
+        for (int c = 0; c < LP_W; ++c)
+          if (c == cj) v = x[q][c];
+        zc l = (h == hj) ? v * inv : mk(0.0);
+        zc lo;
+        lo.re = __shfl_xor(l.re, 1);
+        lo.im = __shfl_xor(l.im, 1);
+        if (h != hj) l = lo;
+        if (act) {
+#pragma unroll
+          for (int c = 0; c < LP_W; ++c) {
+            const int col = h * LP_W + c;
+            if (col == jj) x[q][c] = l;
+            else if (col > jj) x[q][c] = x[q][c] - l * prow[col];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < LP_Q; ++q) {
+      const int r = p + 256 * q;
+#pragma unroll
+      for (int c = 0; c < LP_W; ++c) {
+        const int col = c0 + h * LP_W + c;
+        if (r < R && col < c0 + hn) A[(long)(j0 + col) * lda + j0 + r] = x[q][c];
+      }
+    }
+    __syncthreads();
+  }
+  // the second half's interchanges on the first half's columns
+  if (nb > 16) lp_gather_swaps(A, lda, j0, spiv, 16, nb, 0, 16, gpos, gsrc, &gcnt);
+}
+
+// Apply the panel's interchanges to the columns outside [j0, j0+nb) and, for columns right of the panel,
+// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  64 columns per 256-lane workgroup: the
+// interchanges are applied as one gather of their composite permutation, staged through LDS, then one
+// lane per column solves with L11.
+__global__ __launch_bounds__(256) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda, IntSet pset) {
   __shared__ zc L11[LU_NB][LU_NB];
-  __shared__ int piv[LU_NB];
+  __shared__ zc stage[2 * LU_NB][64];
+  __shared__ int pos[2 * LU_NB], src[2 * LU_NB], s_cnt;
   const int b = blockIdx.y;
   zc* A = Aset.at(b);
   const int* ipiv = pset.at(b);
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    int r = e % nb, c = e / nb;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int cnt = 0;
+    for (int t = 0; t < nb; ++t) {
+      const int a = j0 + t, q = ipiv[j0 + t];
+      if (a == q) continue;
+      int ia = -1, iq = -1;
+      for (int i = 0; i < cnt; ++i) {
+        if (pos[i] == a) ia = i;
+        if (pos[i] == q) iq = i;
+      }
+      if (ia < 0) { pos[cnt] = a; src[cnt] = a; ia = cnt++; }
+      if (iq < 0) { pos[cnt] = q; src[cnt] = q; iq = cnt++; }
+      const int tmp = src[ia];
+      src[ia] = src[iq];
+      src[iq] = tmp;
+    }
+    s_cnt = cnt;
+  }
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
     L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
   }
-  for (int e = threadIdx.x; e < nb; e += blockDim.x) piv[e] = ipiv[j0 + e];
   __syncthreads();
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  // columns [0, j0) and [j0+nb, n)
-  if (c >= j0) c += nb;
-  if (c >= n) return;
-  zc* col = A + (long)c * lda;
-  for (int jj = 0; jj < nb; ++jj) {
-    int p = piv[jj], j = j0 + jj;
-    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
+  const int cnt = s_cnt;
+  // columns [0, j0) and [j0+nb, n), 64 per workgroup
+  auto column = [&](int k) { return (k >= j0) ? k + nb : k; };
+  const int cbase = blockIdx.x * 64;
+  const int others = n - nb;
+  for (int e = tid; e < cnt * 64; e += blockDim.x) {
+    const int i = e >> 6, k = cbase + (e & 63);
+    if (k < others) stage[i][e & 63] = A[(long)column(k) * lda + src[i]];
   }
+  __syncthreads();
+  for (int e = tid; e < cnt * 64; e += blockDim.x) {
+    const int i = e >> 6, k = cbase + (e & 63);
+    if (k < others) A[(long)column(k) * lda + pos[i]] = stage[i][e & 63];
+  }
+  __syncthreads();
+  if (tid >= 64) return;
+  const int k = cbase + tid;
+  if (k >= others) return;
+  const int c = column(k);
   if (c < j0) return;
+  zc* col = A + (long)c * lda;
   zc x[LU_NB];
 #pragma unroll
   for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[j0 + r] : mk(0.0);
@@ -454,10 +698,13 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
   hipLaunchKernelGGL(k_fill_int, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, info, 0);
   for (int j0 = 0; j0 < n; j0 += LU_NB) {
     const int nb = std::min(LU_NB, n - j0);
-    hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
+    if (n - j0 <= 256 * LP_Q)
+      hipLaunchKernelGGL(k_lu_panel_reg, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else
+      hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     const int others = n - nb;
     if (others > 0)
-      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A, lda, ipiv);
+      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(256), 0, s, n, j0, nb, A, lda, ipiv);
     const int rest = n - j0 - nb;
     if (rest > 0)
       zgemm_set(OpN, OpN, rest, rest, nb, mk(-1.0), A.shifted((long)j0 * lda + j0 + nb), lda,
