@@ -651,12 +651,24 @@ __device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int
   };
 #pragma unroll
   for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = ld(q);
+  // the elements entering at bulges j + 1 and j + 2 are loaded two bulges ahead, so that their
+  // latency overlaps the rotations of the bulges in between
+  zc pf[2][2];
+#pragma unroll
+  for (int a = 1; a <= 2; ++a) {
+    pf[a & 1][0] = ld(Q0 - 2 * a);
+    pf[a & 1][1] = ld(Q0 - 2 * a + 1);
+  }
 #pragma unroll
   for (int j = 0; j < MS_NB; ++j) {
     const int qb = Q0 - 2 * j;
     if (j > 0) {
-      x[qb] = ld(qb);
-      x[qb + 1] = ld(qb + 1);
+      x[qb] = pf[j & 1][0];
+      x[qb + 1] = pf[j & 1][1];
+      if (j + 2 < MS_NB) {
+        pf[j & 1][0] = ld(qb - 4);
+        pf[j & 1][1] = ld(qb - 3);
+      }
     }
 #pragma unroll
     for (int s = 0; s < MS_S; ++s) {
@@ -677,9 +689,13 @@ __device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int
 template <bool LEFT>
 __device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen) {
   zc a = p[0];
-#pragma unroll 4
+  zc b1 = (wlen >= 1) ? p[stride] : mk(0.0);
+  zc b2 = (wlen >= 2) ? p[2 * stride] : mk(0.0);
+#pragma unroll 2
   for (int j = 0; j < wlen; ++j) {
-    zc b = p[(long)(j + 1) * stride];
+    zc b = b1;
+    b1 = b2;
+    b2 = (j + 3 <= wlen) ? p[(long)(j + 3) * stride] : mk(0.0);  // two elements ahead
     if (LEFT) rot_left(R[j], a, b);
     else rot_right(R[j], a, b);
     p[(long)j * stride] = a;
