@@ -116,6 +116,20 @@ def field_leg(ctx, sol, g, stream, dev):
             "hbm_write_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
 
 
+def field_c3_leg(ctx, stream, dev):
+    """The 7 x 7 solution of the plain_small sample evaluated on a 512 x 512 x 128 grid over the cell and
+    z in [-1, 1] um (above, through and below the layer), complex64 out."""
+    import scattersim_amd as sa
+    from scattersim_amd import workloads
+    sample, opts, _ = workloads.plain_small()
+    sol = sa.scattervolume(ctx, sample, **opts)
+    g = dict(x0=0.0, y0=0.0, z0=-1.0 + 1.0 / 128, dx=4.0 / 512, dy=4.0 / 512, dz=2.0 / 128, nx=512, ny=512, nz=128)
+    res = field_leg(ctx, sol, g, stream, dev)
+    sol.free()
+    res["workload"] = "plain_small solution on 512x512x128 (z in [-1, 1] um)"
+    return res
+
+
 def cpu_field_baseline(sol, g):
     """The oracle's C restatement of the reference's CPU field sum (OpenMP, FP64) on a few planes."""
     from oracle import build as ob
@@ -289,8 +303,10 @@ def main():
 
     # ---- field evaluation of the solution on the configuration grid ----
     field = None
+    field_c3 = None
     if not args.no_field:
         field = field_leg(ctx, sol, grid, stream, dev)
+        field_c3 = field_c3_leg(ctx, stream, dev)
     cpu_field = None
     if not args.no_field and not args.no_cpu:
         cpu_field = cpu_field_baseline(sol, grid)
@@ -326,6 +342,7 @@ def main():
                    "parallelism": "1 GPU"},
         "solve_tflops_nominal": solve_flops(MF, L, args.eig_mode) / (ms * 1e-3) / 1e12,
         "field": field,
+        "field_c3": field_c3,
         "batched": batched,
         "roofline": {"kernel": "eigensolver stage (gehrd+unghr+hqr+trevc+gemm+normalize)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

@@ -13,6 +13,9 @@
 #include "cwx_common.h"
 #include "cwx_solver.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace cwx {
 
 namespace {
@@ -160,6 +163,166 @@ __global__ __launch_bounds__(256) void k_sep_synth(SepArgs a, GridSpec g, const 
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Complex64 output path: coefficients of all planes of one layer in one pass over [Q_check | Q_hat],
+// then T = X C per plane (FP64, rounded to complex64), then E = T Y^T on the FP32 matrix cores.
+// ------------------------------------------------------------------------------------------------
+struct PlaneGroup {
+  int iz0, count, region;  // planes iz0 .. iz0+count-1, all in region `region` (0: above, L+1: below)
+};
+constexpr int SEP_G = 8;  // planes per group
+
+// 64 coefficients per workgroup; the SEP_W wavefronts take slices of the 4MF-term sums and their
+// partial sums are added in LDS in a fixed order.
+constexpr int SEP_W = 8;
+__global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec g, const PlaneGroup* __restrict__ groups,
+                                                      zc* __restrict__ C) {
+  extern __shared__ zc ph[];  // count x 4MF mode phases
+  __shared__ zc part[SEP_W][SEP_G][64];
+  const PlaneGroup gr = groups[blockIdx.y];
+  const int MF = a.MF, n3 = 3 * MF, n4 = 4 * MF;
+  const int l = gr.region;
+  const bool layer = (l >= 1 && l <= a.L);
+  const int i = l - 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  if (layer && !a.external) {
+    for (int t = threadIdx.x; t < gr.count * n4; t += blockDim.x) {
+      const int pl = t / n4, m = t % n4;
+      const double z = (double)coord(g.z0, g.dz, gr.iz0 + pl);
+      const bool up = m >= 2 * MF;
+      const int mode = up ? 2 * (m - 2 * MF) + 1 : 2 * m;
+      const zc kz = a.gam[(long)i * n4 + mode] * a.k;
+      ph[t] = cexpi(kz, z - (up ? a.zl[i + 1] : a.zl[i]));
+    }
+    __syncthreads();
+    zc acc[SEP_G];
+#pragma unroll
+    for (int t = 0; t < SEP_G; ++t) acc[t] = mk(0.0);
+    if (e < n3) {
+      const int mq = (n4 + SEP_W - 1) / SEP_W, m0 = wv * mq, m1 = min(n4, m0 + mq);
+      const zc* Q = a.Qch + (long)i * n3 * n4 + e;
+#pragma unroll 4
+      for (int m = m0; m < m1; ++m) {
+        const zc q = Q[(long)m * n3];
+#pragma unroll
+        for (int t = 0; t < SEP_G; ++t)
+          if (t < gr.count) fma_acc(acc[t], q, ph[t * n4 + m]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < SEP_G; ++t) part[wv][t][lane] = acc[t];
+    __syncthreads();
+    if (wv == 0 && e < n3) {
+      for (int t = 0; t < gr.count; ++t) {
+        zc sum = part[0][t][lane];
+        for (int w = 1; w < SEP_W; ++w) sum = sum + part[w][t][lane];
+        C[(long)(gr.iz0 + t) * n3 + e] = sum;
+      }
+    }
+    return;
+  }
+  if (wv != 0 || e >= n3) return;
+  const int c = e / MF, p = e % MF;
+  for (int t = 0; t < gr.count; ++t) {
+    const double z = (double)coord(g.z0, g.dz, gr.iz0 + t);
+    zc v = mk(0.0);
+    if (l == 0) {
+      const zc kzi = a.Sz0[p] * a.k;
+      const zc kzr = -(a.Sz0[p] * a.k) * a.n0;
+      v = a.EF[c * MF + p] * cexpi(kzi, z) + a.x[c * MF + p] * cexpi(kzr, z - a.zl[0]);
+    } else if (l == a.L + 1) {
+      const zc kzt = (a.Sz1[p] * a.k) * a.n1;
+      v = a.x[n3 + c * MF + p] * cexpi(kzt, z - a.zl[a.L]);
+    }
+    C[(long)(gr.iz0 + t) * n3 + e] = v;
+  }
+}
+
+// T[iz][c][b][ix] = sum_a exp(i kx_a x_ix) C[iz][c][a M1 + b] as complex64, b padded to K4 with zeros;
+// Yf[b][iy] = exp(i ky_b y_iy) as complex64 (same padding).
+__global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, const zc* __restrict__ C,
+                                                  const zc* __restrict__ X, int K4, float2* __restrict__ T) {
+  extern __shared__ zc sCp[];  // 3 MF of this plane
+  const int iz = blockIdx.y, MF = a.MF, M0 = a.M0, M1 = a.M1;
+  for (int e = threadIdx.x; e < 3 * MF; e += blockDim.x) sCp[e] = C[(long)iz * 3 * MF + e];
+  __syncthreads();
+  const long per = (long)3 * K4 * g.nx;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= per) return;
+  const int ix = (int)(e % g.nx), b = (int)((e / g.nx) % K4), c = (int)(e / ((long)g.nx * K4));
+  zc acc = mk(0.0);
+  if (b < M1)
+    for (int ai = 0; ai < M0; ++ai) fma_acc(acc, X[(long)ix * M0 + ai], sCp[c * MF + ai * M1 + b]);
+  T[(long)iz * per + e] = make_float2((float)acc.re, (float)acc.im);
+}
+
+__global__ void k_sep_ytab(SepArgs a, GridSpec g, int K4, float2* __restrict__ Yf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= K4 * g.ny) return;
+  const int iy = t % g.ny, b = t / g.ny;
+  zc v = mk(0.0);
+  if (b < a.M1) v = cexpi(a.Sy[b] * a.k, (double)coord(g.y0, g.dy, iy));
+  Yf[t] = make_float2((float)v.re, (float)v.im);
+}
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int SY_TILES = 4;  // y tiles of 16 rows per wavefront
+
+// E(x, y) = sum_b Y(y, b) T(b, x) per component with v_mfma_f32_16x16x4f32 (A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k, D[4 (l / 16) + q][l % 16] in register q of lane l).  A wavefront owns 16
+// columns and SY_TILES x 16 rows of one plane; the T fragments of its columns stay in registers.
+__global__ __launch_bounds__(256) void k_sep_synth_mfma(GridSpec g, int K4, const float2* __restrict__ T,
+                                                        const float2* __restrict__ Yf, float2* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int iz = blockIdx.z;
+  const int x0 = blockIdx.x * 16;
+  const int y0 = (blockIdx.y * 4 + wv) * 16 * SY_TILES;
+  if (y0 >= g.ny) return;
+  const int lx = lane & 15, lk = lane >> 4;
+  const long per = (long)K4 * g.nx;
+  const float2* Tz = T + (long)iz * 3 * per;
+  constexpr int KMAX = 16;  // up to 64 orders along y
+  float2 tf[3][KMAX];
+  const int nk = K4 / 4;
+#pragma unroll
+  for (int kb = 0; kb < KMAX; ++kb)
+    if (kb < nk)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        tf[c][kb] = (x0 + lx < g.nx) ? Tz[c * per + (long)(kb * 4 + lk) * g.nx + x0 + lx] : make_float2(0.f, 0.f);
+  for (int yt = 0; yt < SY_TILES; ++yt) {
+    const int yb = y0 + 16 * yt;
+    if (yb >= g.ny) break;
+    f4 P[3], Q[3], R[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P[c] = Q[c] = R[c] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KMAX; ++kb) {
+      if (kb < nk) {
+        const float2 y = (yb + lx < g.ny) ? Yf[(long)(kb * 4 + lk) * g.ny + yb + lx] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          P[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.x, tf[c][kb].x, P[c], 0, 0, 0);
+          Q[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.y, tf[c][kb].y, Q[c], 0, 0, 0);
+          R[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.x, tf[c][kb].y, R[c], 0, 0, 0);
+          R[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(y.y, tf[c][kb].x, R[c], 0, 0, 0);
+        }
+      }
+    }
+    const int ix = x0 + lx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int iy = yb + 4 * lk + q;
+      if (ix < g.nx && iy < g.ny) {
+        float2* o = out + (((long)iz * g.ny + iy) * g.nx + ix) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = make_float2(P[c][q] - Q[c][q], R[c][q]);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int out_f64, void* out,
@@ -184,6 +347,50 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
   SepArgs a{st.M0, st.M1, MF, L, sol.opt.external ? 1 : 0, st.k, sol.opt.n[0], sol.opt.n[1],
             sol.tabs.p, sol.tabs.p + MF, sol.tabs.p + 2 * MF, sol.tabs.p + 3 * MF, sol.tabs.p + 4 * MF,
             sol.x.p, sol.gam.p, sol.Qch.p, w.zf.p, w.zd.p, tol_mode};
+  if (!out_f64) {
+    // plane groups: consecutive planes of one region, at most SEP_G and at most what the phase
+    // table in LDS holds; regions computed with the device's coordinate and test
+    const int n4 = 4 * MF;
+    const int gmax = std::max(1, std::min(SEP_G, (int)((32 * 1024) / (sizeof(zc) * n4))));
+    std::vector<PlaneGroup> groups;
+    auto region = [&](int iz) {
+      const float z = std::fmaf(g.dz, (float)iz, g.z0);
+      int l = 0;
+      for (int li = 0; li <= L; ++li) {
+        if (tol_mode == 0) {
+          if ((double)z >= (double)zf[li] - 0.001) l = li + 1;
+        } else if (z >= zf[li]) {
+          l = li + 1;
+        }
+      }
+      return l;
+    };
+    for (int iz = 0; iz < g.nz;) {
+      const int r = region(iz);
+      int cnt = 1;
+      while (iz + cnt < g.nz && cnt < gmax && region(iz + cnt) == r) ++cnt;
+      groups.push_back(PlaneGroup{iz, cnt, r});
+      iz += cnt;
+    }
+    CWX_CHECK(groups.size() <= 65535, "too many plane groups");
+    w.groups.ensure(sizeof(PlaneGroup) * groups.size());
+    CWX_HIP(hipMemcpyAsync(w.groups.p, groups.data(), sizeof(PlaneGroup) * groups.size(), hipMemcpyHostToDevice, s));
+    const int K4 = (st.M1 + 3) / 4 * 4;
+    CWX_CHECK(K4 <= 64, "more than 64 orders along y in the complex64 evaluator");
+    w.Tf.ensure((size_t)g.nz * 3 * K4 * g.nx);
+    w.Yf.ensure((size_t)K4 * g.ny);
+    hipLaunchKernelGGL(k_sep_coeffs_g, dim3(cdiv(3 * MF, 64), (unsigned)groups.size()), dim3(64 * SEP_W),
+                       sizeof(zc) * gmax * n4, s, a, g, (const PlaneGroup*)w.groups.p, w.C.p);
+    hipLaunchKernelGGL(k_sep_tables, dim3(cdiv(g.nx * st.M0, 256)), dim3(256), 0, s, a, g, w.X.p, w.Y.p);
+    const long per = (long)3 * K4 * g.nx;
+    hipLaunchKernelGGL(k_sep_rows, dim3((unsigned)cdiv((int)per, 256), g.nz), dim3(256), sizeof(zc) * 3 * MF, s, a,
+                       g, w.C.p, w.X.p, K4, w.Tf.p);
+    hipLaunchKernelGGL(k_sep_ytab, dim3(cdiv(K4 * g.ny, 256)), dim3(256), 0, s, a, g, K4, w.Yf.p);
+    dim3 grid(cdiv(g.nx, 16), cdiv(g.ny, 4 * 16 * SY_TILES), g.nz);
+    hipLaunchKernelGGL(k_sep_synth_mfma, grid, dim3(256), 0, s, g, K4, w.Tf.p, w.Yf.p, (float2*)out);
+    CWX_HIP(hipGetLastError());
+    return;
+  }
   const size_t sh_coef = sizeof(zc) * 4 * (size_t)MF;
   CWX_CHECK(sh_coef <= 64 * 1024, "too many orders for the separable evaluator");
   hipLaunchKernelGGL(k_sep_coeffs, dim3(cdiv(3 * MF, 256), g.nz), dim3(256), sh_coef, s, a, g, w.C.p);

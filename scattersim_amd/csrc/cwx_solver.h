@@ -81,6 +81,8 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
 // Work buffers of the separable field evaluation.
 struct SepWork {
   DevBuf<zc> C, X, Y;
+  DevBuf<float2> Tf, Yf;          // complex64 path: T = X C per plane, y phase table
+  DevBuf<unsigned char> groups;   // plane groups of the coefficient pass
   DevBuf<float> zf;
   DevBuf<double> zd;
 };
