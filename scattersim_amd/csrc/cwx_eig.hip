@@ -394,17 +394,26 @@ __device__ inline Rot make_reflector(zc& alpha, zc x) {
   return g;
 }
 
+// The rotations are written out as fused multiply-adds (12 of them and 2 additions each).
 __device__ inline void rot_left(const Rot& g, zc& a, zc& b) {
   // rows k, k+1:  sum = conj(t1) a + t2 b; a -= sum; b -= sum v2
-  zc s = conj(g.t1) * a + g.t2 * b;
-  a = a - s;
-  b = b - s * g.v2;
+  zc s;
+  s.re = fma(g.t1.re, a.re, fma(g.t1.im, a.im, g.t2 * b.re));
+  s.im = fma(g.t1.re, a.im, fma(-g.t1.im, a.re, g.t2 * b.im));
+  a.re -= s.re;
+  a.im -= s.im;
+  b.re = fma(-s.re, g.v2.re, fma(s.im, g.v2.im, b.re));
+  b.im = fma(-s.re, g.v2.im, fma(-s.im, g.v2.re, b.im));
 }
 __device__ inline void rot_right(const Rot& g, zc& a, zc& b) {
   // cols k, k+1:  sum = t1 a + t2 b; a -= sum; b -= sum conj(v2)
-  zc s = g.t1 * a + g.t2 * b;
-  a = a - s;
-  b = b - s * conj(g.v2);
+  zc s;
+  s.re = fma(g.t1.re, a.re, fma(-g.t1.im, a.im, g.t2 * b.re));
+  s.im = fma(g.t1.re, a.im, fma(g.t1.im, a.re, g.t2 * b.im));
+  a.re -= s.re;
+  a.im -= s.im;
+  b.re = fma(-s.re, g.v2.re, fma(-s.im, g.v2.im, b.re));
+  b.im = fma(-s.im, g.v2.re, fma(s.re, g.v2.im, b.im));
 }
 
 __device__ inline bool subdiag_negligible(const zc* H, int ldh, int k, int ilo, int ihi, double smlnum) {
@@ -478,14 +487,24 @@ __device__ inline Rot2 make_reflector2(zc& alpha, zc x) {
   return g;
 }
 __device__ inline void rot2_left(const Rot2& g, zc& a, zc& b) {
-  const zc s = conj(g.t1) * a + conj(g.t2) * b;
-  a = a - s;
-  b = b - s * g.v2;
+  // s = conj(t1) a + conj(t2) b; a -= s; b -= s v2
+  zc s;
+  s.re = fma(g.t1.re, a.re, fma(g.t1.im, a.im, fma(g.t2.re, b.re, g.t2.im * b.im)));
+  s.im = fma(g.t1.re, a.im, fma(-g.t1.im, a.re, fma(g.t2.re, b.im, -(g.t2.im * b.re))));
+  a.re -= s.re;
+  a.im -= s.im;
+  b.re = fma(-s.re, g.v2.re, fma(s.im, g.v2.im, b.re));
+  b.im = fma(-s.re, g.v2.im, fma(-s.im, g.v2.re, b.im));
 }
 __device__ inline void rot2_right(const Rot2& g, zc& a, zc& b) {
-  const zc s = g.t1 * a + g.t2 * b;
-  a = a - s;
-  b = b - s * conj(g.v2);
+  // s = t1 a + t2 b; a -= s; b -= s conj(v2)
+  zc s;
+  s.re = fma(g.t1.re, a.re, fma(-g.t1.im, a.im, fma(g.t2.re, b.re, -(g.t2.im * b.im))));
+  s.im = fma(g.t1.re, a.im, fma(g.t1.im, a.re, fma(g.t2.re, b.im, g.t2.im * b.re)));
+  a.re -= s.re;
+  a.im -= s.im;
+  b.re = fma(-s.re, g.v2.re, fma(-s.im, g.v2.im, b.re));
+  b.im = fma(-s.im, g.v2.re, fma(s.re, g.v2.im, b.im));
 }
 
 __device__ inline void wave_sync() {
@@ -637,17 +656,48 @@ __device__ void real_subdiag(zc* H, int ldh, zc* Z, int ldz, int n, int L, int I
   __syncthreads();
 }
 
-// Applies one window's rotations (left: to a column, on rows kbase + q; right: to a row, on columns
-// kbase + q) to one line, bulge by bulge.  Bulge j touches elements 30-2j .. 46-2j, so two elements
-// enter before it and two leave after it, and only that live window is held in registers.  Element q
-// lives at p[(kbase + q) * stride]; elements with q outside [qlo, qhi] read as zero and are not stored.
-template <bool LEFT>
-__device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int qhi, const Rot2* R, int nsteps) {
+// The deferred products read a window's rotations as scalars: every lane of a wave applies the same
+// rotation, so its six doubles are loaded once per wave into scalar registers (through the scalar
+// data cache, from a copy in global memory) instead of being broadcast from LDS into all 64 lanes,
+// which would make the LDS return path the bottleneck of the deferred products.
+typedef __attribute__((address_space(4))) const double KDouble;
+constexpr int MS_RING = MS_S * MS_NB * 6;  // doubles of one window's rotations, bulge-major
+
+// Hides the pointer from the optimiser so that the scalar loads through it are neither hoisted above
+// the barrier that publishes the window's rotations nor merged across windows.
+__device__ inline const KDouble* scalar_view(const double* p) {
+  asm volatile("" : "+s"(p));
+  return (const KDouble*)p;
+}
+__device__ inline Rot2 scalar_rot(const KDouble* R, int i) {
+  Rot2 g;
+  g.t1 = mk(R[6 * i], R[6 * i + 1]);
+  g.v2 = mk(R[6 * i + 2], R[6 * i + 3]);
+  g.t2 = mk(R[6 * i + 4], R[6 * i + 5]);
+  return g;
+}
+
+// Applies one window's left rotations (rotation (j, s) on elements 30-2j+s, 31-2j+s) to one line,
+// bulge by bulge.  Bulge j touches elements 30-2j .. 46-2j, so two elements enter before it and two
+// leave after it, and only that live window is held in registers.  Element q lives at p[q * st].
+// A column right of the window takes the left rotations as they are; a row above it or a row of Z
+// takes the right rotations, and since right-multiplying a row equals left-multiplying its conjugate
+// by the same reflector, such a line is conjugated on load and store (sgn flips the sign bit of the
+// imaginary parts).  Every lane therefore runs the same code whatever its line.  Elements outside
+// [qlo, qhi] lie outside the active block: they enter as zero, see only identity rotations and are
+// not stored.  FULL: the whole window lies inside the active block.
+template <bool FULL>
+__device__ inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int qlo, int qhi, const KDouble* R) {
   constexpr int Q0 = 2 * (MS_NB - 1);
   zc x[MS_LEN];
-  auto ld = [&](int q) { return (q >= qlo && q <= qhi) ? p[(long)(kbase + q) * stride] : mk(0.0); };
-  auto st = [&](int q) {
-    if (q >= qlo && q <= qhi) p[(long)(kbase + q) * stride] = x[q];
+  auto in = [&](int q) { return FULL || (q >= qlo && q <= qhi); };
+  auto flip = [&](zc v) {
+    v.im = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, v.im) ^ sgn);
+    return v;
+  };
+  auto ld = [&](int q) { return in(q) ? flip(p[q * st]) : mk(0.0); };
+  auto store = [&](int q) {
+    if (in(q)) p[q * st] = flip(x[q]);
   };
 #pragma unroll
   for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = ld(q);
@@ -659,6 +709,9 @@ __device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int
     pf[a & 1][0] = ld(Q0 - 2 * a);
     pf[a & 1][1] = ld(Q0 - 2 * a + 1);
   }
+  // the rotations stream through scalar registers one ahead of their use; the scheduling barrier
+  // keeps each scalar load above the rotation before it (only ALU instructions may cross)
+  Rot2 g = scalar_rot(R, 0);
 #pragma unroll
   for (int j = 0; j < MS_NB; ++j) {
     const int qb = Q0 - 2 * j;
@@ -672,16 +725,18 @@ __device__ inline void ms_apply_line(zc* p, long stride, int kbase, int qlo, int
     }
 #pragma unroll
     for (int s = 0; s < MS_S; ++s) {
-      if (s < nsteps) {
-        if (LEFT) rot2_left(R[s * MS_NB + j], x[qb + s], x[qb + s + 1]);
-        else rot2_right(R[s * MS_NB + j], x[qb + s], x[qb + s + 1]);
-      }
+      const int i = j * MS_S + s;
+      const Rot2 gn = scalar_rot(R, i + 1 < MS_S * MS_NB ? i + 1 : i);
+      __builtin_amdgcn_sched_barrier(1);
+      // steps past nsteps hold zero (identity) rotations
+      rot2_left(g, x[qb + s], x[qb + s + 1]);
+      g = gn;
     }
-    st(qb + MS_S - 1);
-    st(qb + MS_S);
+    store(qb + MS_S - 1);
+    store(qb + MS_S);
   }
 #pragma unroll
-  for (int q = 0; q < MS_S - 1; ++q) st(q);
+  for (int q = 0; q < MS_S - 1; ++q) store(q);
 }
 
 // Applies the single-shift rotations 0..wlen-1 (rotation j on elements j, j+1) to one line whose
@@ -705,7 +760,7 @@ __device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen)
 }
 
 __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
-                                                     long sZ, zc* W, long sW, int* info) {
+                                                     long sZ, zc* W, long sW, int* info, double* ring) {
   constexpr int BR = QR_LD, BC = QR_W + 3;  // single-shift block: QR_W + 2 rows (leading dimension QR_LD)
   static_assert(BR * BC <= MS_LD * (MS_LEN + 1), "window buffers share one LDS array");
   __shared__ zc MB[MS_LD * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
@@ -720,6 +775,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
   H += sH * b;
   Z += sZ * b;
   W += sW * b;
+  Rot2* const rg = reinterpret_cast<Rot2*>(ring + (long)MS_RING * b);
   const int tid = threadIdx.x, nt = blockDim.x;
   auto Hp = [&](int r, int c) -> zc& { return H[(long)c * ldh + r]; };
   auto Zp = [&](int r, int c) -> zc& { return Z[(long)c * ldz + r]; };
@@ -868,22 +924,66 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
           }
           __syncthreads();
         }
+        if (nsteps < MS_S) {
+          // the last window of a sweep: its unused steps become identity rotations
+          for (int e = tid; e < (MS_S - nsteps) * MS_NB; e += nt) {
+            Rot2& g = rots2[nsteps * MS_NB + e];
+            g.t1 = g.v2 = g.t2 = mk(0.0);
+          }
+          __syncthreads();
+        }
+        for (int e = tid; e < MS_S * MS_NB; e += nt) rg[(e % MS_NB) * MS_S + e / MS_NB] = rots2[e];
         for (int e = tid; e < nr * nc; e += nt) {
           const int r = rlo + e % nr, c = clo + e / nr;
           Hp(r, c) = MBp(r, c);
+        }
+        // the ring stores reach L2 before the barrier; the scalar cache then drops what it holds of
+        // the previous window's rotations (s_waitcnt encodings of gfx9: 0x0F70 vmcnt(0), 0xC07F lgkmcnt(0))
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        __builtin_amdgcn_s_dcache_inv();
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        {
+          // the waves fetch the ring's 64-byte lines into the scalar cache together (one dword of
+          // each, all loads in flight), so the deferred products start on hits
+          constexpr int kLines = MS_RING * 8 / 64, kWaves = EIG_THREADS / 64;
+          static_assert(kLines % kWaves == 0, "ring lines split evenly over the waves");
+          typedef __attribute__((address_space(4))) const unsigned KU32;
+          const unsigned* rp = reinterpret_cast<const unsigned*>(rg);
+          asm volatile("" : "+s"(rp));
+          const KU32* ku = (const KU32*)rp;
+          const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+          unsigned acc = 0;
+#pragma unroll
+          for (int i = 0; i < kLines / kWaves; ++i) acc ^= ku[(i * kWaves + w) * 16];
+          asm volatile("" ::"s"(acc));
         }
         // deferred products: columns right of the window (left rotations on rows kbase + q),
         // rows above it and all rows of Z (right rotations on columns kbase + q)
         const int nfr = n - 1 - rhi, nfa = rlo, nz = n;
         const int lines = nfr + nfa + nz;
+        const bool full = kbase >= L && kbase + MS_LEN - 1 <= I;
+        constexpr unsigned long long kSign = 1ull << 63;
         for (int ln = tid; ln < lines; ln += nt) {
+          const KDouble* R = scalar_view(reinterpret_cast<const double*>(rg));
+          zc* p;
+          long st;
+          unsigned long long sg;
           if (ln < nfr) {
-            ms_apply_line<true>(H + (long)(rhi + 1 + ln) * ldh, 1, kbase, rlo - kbase, rhi - kbase, rots2, nsteps);
+            p = H + (long)(rhi + 1 + ln) * ldh + kbase;
+            st = 1;
+            sg = 0;
+          } else if (ln < nfr + nfa) {
+            p = H + (ln - nfr) + (long)kbase * ldh;
+            st = ldh;
+            sg = kSign;
           } else {
-            const bool isz = ln >= nfr + nfa;
-            const int r = isz ? ln - nfr - nfa : ln - nfr;
-            ms_apply_line<false>(isz ? Z + r : H + r, isz ? ldz : ldh, kbase, L - kbase, I - kbase, rots2, nsteps);
+            p = Z + (ln - nfr - nfa) + (long)kbase * ldz;
+            st = ldz;
+            sg = kSign;
           }
+          if (full) ms_apply_line<true>(p, st, sg, 0, MS_LEN - 1, R);
+          else ms_apply_line<false>(p, st, sg, rlo - kbase, rhi - kbase, R);
         }
         __syncthreads();
       }
@@ -1126,6 +1226,7 @@ void EigWork::ensure(int n, int batch) {
   Y.ensure((size_t)n * HB * batch);
   T.ensure(npan * HB * HB * batch);
   W.ensure((size_t)2 * n * HB * batch);
+  ring.ensure((size_t)MS_RING * batch);
 }
 
 void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
@@ -1185,7 +1286,7 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   }
   hipLaunchKernelGGL(k_clear_below_sub, dim3(cdiv(n * n, 256 * 4), batch), dim3(256), 0, s, n, A, lda, sA);
   hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n, nn, W,
-                     sW, info);
+                     sW, info, work.ring.p);
   const double smlnum = kSafmin * ((double)n / kUlp);
   // one wavefront per eigenvector, as many per workgroup as fit 64 KiB of LDS
   const int waves = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / (sizeof(zc) * n)));

@@ -104,6 +104,7 @@ struct EigWork {
   DevBuf<zc> tau;    // n*batch
   DevBuf<int> info;  // batch
   DevBuf<zc> Y, T, W; // blocked Hessenberg reduction: panel Y, T of every panel, GEMM scratch
+  DevBuf<double> ring; // QR: the current window's rotations of every matrix, read back as scalars
   void ensure(int n, int batch);
 };
 

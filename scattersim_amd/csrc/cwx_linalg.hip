@@ -542,62 +542,31 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
   if (nb > 16) lp_gather_swaps(A, lda, j0, spiv, 16, nb, 0, 16, gpos, gsrc, &gcnt);
 }
 
-// Apply the panel's interchanges to the columns outside [j0, j0+nb) and, for columns right of the panel,
-// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  64 columns per 256-lane workgroup: the
-// interchanges are applied as one gather of their composite permutation, staged through LDS, then one
-// lane per column solves with L11.
-__global__ __launch_bounds__(256) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda, IntSet pset) {
+// Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
+// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.
+__global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda,
+                                                     IntSet pset) {
   __shared__ zc L11[LU_NB][LU_NB];
-  __shared__ zc stage[2 * LU_NB][64];
-  __shared__ int pos[2 * LU_NB], src[2 * LU_NB], s_cnt;
+  __shared__ int piv[LU_NB];
   const int b = blockIdx.y;
   zc* A = Aset.at(b);
   const int* ipiv = pset.at(b);
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    int cnt = 0;
-    for (int t = 0; t < nb; ++t) {
-      const int a = j0 + t, q = ipiv[j0 + t];
-      if (a == q) continue;
-      int ia = -1, iq = -1;
-      for (int i = 0; i < cnt; ++i) {
-        if (pos[i] == a) ia = i;
-        if (pos[i] == q) iq = i;
-      }
-      if (ia < 0) { pos[cnt] = a; src[cnt] = a; ia = cnt++; }
-      if (iq < 0) { pos[cnt] = q; src[cnt] = q; iq = cnt++; }
-      const int tmp = src[ia];
-      src[ia] = src[iq];
-      src[iq] = tmp;
-    }
-    s_cnt = cnt;
-  }
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    int r = e % nb, c = e / nb;
     L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
   }
+  for (int e = threadIdx.x; e < nb; e += blockDim.x) piv[e] = ipiv[j0 + e];
   __syncthreads();
-  const int cnt = s_cnt;
-  // columns [0, j0) and [j0+nb, n), 64 per workgroup
-  auto column = [&](int k) { return (k >= j0) ? k + nb : k; };
-  const int cbase = blockIdx.x * 64;
-  const int others = n - nb;
-  for (int e = tid; e < cnt * 64; e += blockDim.x) {
-    const int i = e >> 6, k = cbase + (e & 63);
-    if (k < others) stage[i][e & 63] = A[(long)column(k) * lda + src[i]];
-  }
-  __syncthreads();
-  for (int e = tid; e < cnt * 64; e += blockDim.x) {
-    const int i = e >> 6, k = cbase + (e & 63);
-    if (k < others) A[(long)column(k) * lda + pos[i]] = stage[i][e & 63];
-  }
-  __syncthreads();
-  if (tid >= 64) return;
-  const int k = cbase + tid;
-  if (k >= others) return;
-  const int c = column(k);
-  if (c < j0) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  // columns [0, j0) and [j0+nb, n)
+  if (c >= j0) c += nb;
+  if (c >= n) return;
   zc* col = A + (long)c * lda;
+  for (int jj = 0; jj < nb; ++jj) {
+    int p = piv[jj], j = j0 + jj;
+    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
+  }
+  if (c < j0) return;
   zc x[LU_NB];
 #pragma unroll
   for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[j0 + r] : mk(0.0);
@@ -704,7 +673,7 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
       hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     const int others = n - nb;
     if (others > 0)
-      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(256), 0, s, n, j0, nb, A, lda, ipiv);
+      hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A, lda, ipiv);
     const int rest = n - j0 - nb;
     if (rest > 0)
       zgemm_set(OpN, OpN, rest, rest, nb, mk(-1.0), A.shifted((long)j0 * lda + j0 + nb), lda,
