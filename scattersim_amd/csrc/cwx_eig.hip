@@ -47,6 +47,31 @@ __device__ inline double dlapy3(double x, double y, double z) {
   return w * sqrt((x / w) * (x / w) + (y / w) * (y / w) + (z / w) * (z / w));
 }
 
+// zlarfg for n = 2: beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha) / beta,
+// v2 = x / (alpha - beta).  When |(alpha, x)|^2 is comfortably inside the double range it is formed
+// directly, so the chain is one square root and two independent reciprocals (|alpha - beta| >= |beta|,
+// no cancellation); otherwise the scaled forms of zlarfg (dlapy3 and Smith's division) are used.
+// Returns false when x = 0 and alpha is real (tau = 0, the identity).
+__device__ inline bool reflector2(zc& alpha, zc x, zc& tau, zc& v2) {
+  if (x.re == 0.0 && x.im == 0.0 && alpha.im == 0.0) return false;
+  const double s2 = fma(alpha.re, alpha.re, fma(alpha.im, alpha.im, norm2(x)));
+  double beta;
+  if (s2 > 1e-280 && s2 < 1e280) {
+    beta = -copysign(sqrt(s2), alpha.re);
+    const double rb = 1.0 / beta;
+    const zc d = mk(alpha.re - beta, alpha.im);
+    const double rd = 1.0 / norm2(d);
+    tau = mk((beta - alpha.re) * rb, -alpha.im * rb);
+    v2 = x * mk(d.re * rd, -d.im * rd);
+  } else {
+    beta = -copysign(dlapy3(alpha.re, alpha.im, absz(x)), alpha.re);
+    tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+    v2 = x * zdiv(mk(1.0), alpha - mk(beta));
+  }
+  alpha = mk(beta);
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Level-2 building blocks for one workgroup operating on a column-major block in global memory.
 // Loads are unrolled with independent accumulators so that several are in flight per lane: the
@@ -379,17 +404,12 @@ struct Rot {
 // 2-element Householder (zlarfg with n = 2): returns (t1, v2, t2 = Re(t1 v2)) and beta in alpha.
 __device__ inline Rot make_reflector(zc& alpha, zc x) {
   Rot g;
-  const double xnorm = absz(x);
-  if (xnorm == 0.0 && alpha.im == 0.0) {
+  if (!reflector2(alpha, x, g.t1, g.v2)) {
     g.t1 = mk(0.0);
     g.v2 = x;
     g.t2 = 0.0;
     return g;
   }
-  const double beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
-  g.t1 = mk((beta - alpha.re) / beta, -alpha.im / beta);
-  g.v2 = x * zdiv(mk(1.0), alpha - mk(beta));
-  alpha = mk(beta);
   g.t2 = (g.t1 * g.v2).re;
   return g;
 }
@@ -474,15 +494,10 @@ struct Rot2 {
 // 2-element Householder (zlarfg, n = 2) for a general complex second entry.
 __device__ inline Rot2 make_reflector2(zc& alpha, zc x) {
   Rot2 g;
-  const double xnorm = absz(x);
-  if (xnorm == 0.0 && alpha.im == 0.0) {
+  if (!reflector2(alpha, x, g.t1, g.v2)) {
     g.t1 = g.v2 = g.t2 = mk(0.0);
     return g;
   }
-  const double beta = -copysign(dlapy3(alpha.re, alpha.im, xnorm), alpha.re);
-  g.t1 = mk((beta - alpha.re) / beta, -alpha.im / beta);
-  g.v2 = x * zdiv(mk(1.0), alpha - mk(beta));
-  alpha = mk(beta);
   g.t2 = g.t1 * g.v2;
   return g;
 }
@@ -695,19 +710,32 @@ __device__ inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int
     v.im = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, v.im) ^ sgn);
     return v;
   };
-  auto ld = [&](int q) { return in(q) ? flip(p[q * st]) : mk(0.0); };
-  auto store = [&](int q) {
-    if (in(q)) p[q * st] = flip(x[q]);
+  // element addresses are formed where they are used, from a copy of the line's stride that the
+  // optimiser cannot see through: precomputed for all 48 elements they would fill the registers
+  // the rotations need
+  auto stride = [&]() {
+    long sb = st;
+    asm volatile("" : "+v"(sb));
+    return sb;
   };
-#pragma unroll
-  for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = ld(q);
+  auto ld = [&](int q, long sb) { return in(q) ? p[q * sb] : mk(0.0); };
+  auto store = [&](int q, long sb) {
+    if (in(q)) p[q * sb] = flip(x[q]);
+  };
   // the elements entering at bulges j + 1 and j + 2 are loaded two bulges ahead, so that their
   // latency overlaps the rotations of the bulges in between
   zc pf[2][2];
+  {
+    const long sb = stride();
 #pragma unroll
-  for (int a = 1; a <= 2; ++a) {
-    pf[a & 1][0] = ld(Q0 - 2 * a);
-    pf[a & 1][1] = ld(Q0 - 2 * a + 1);
+    for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = ld(q, sb);
+#pragma unroll
+    for (int a = 1; a <= 2; ++a) {
+      pf[a & 1][0] = ld(Q0 - 2 * a, sb);
+      pf[a & 1][1] = ld(Q0 - 2 * a + 1, sb);
+    }
+#pragma unroll
+    for (int q = Q0; q <= Q0 + MS_S; ++q) x[q] = flip(x[q]);
   }
   // the rotations stream through scalar registers one ahead of their use; the scheduling barrier
   // keeps each scalar load above the rotation before it (only ALU instructions may cross)
@@ -715,12 +743,13 @@ __device__ inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int
 #pragma unroll
   for (int j = 0; j < MS_NB; ++j) {
     const int qb = Q0 - 2 * j;
+    const long sb = stride();
     if (j > 0) {
-      x[qb] = pf[j & 1][0];
-      x[qb + 1] = pf[j & 1][1];
+      x[qb] = flip(pf[j & 1][0]);
+      x[qb + 1] = flip(pf[j & 1][1]);
       if (j + 2 < MS_NB) {
-        pf[j & 1][0] = ld(qb - 4);
-        pf[j & 1][1] = ld(qb - 3);
+        pf[j & 1][0] = ld(qb - 4, sb);
+        pf[j & 1][1] = ld(qb - 3, sb);
       }
     }
 #pragma unroll
@@ -732,11 +761,12 @@ __device__ inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int
       rot2_left(g, x[qb + s], x[qb + s + 1]);
       g = gn;
     }
-    store(qb + MS_S - 1);
-    store(qb + MS_S);
+    store(qb + MS_S - 1, sb);
+    store(qb + MS_S, sb);
   }
+  const long sb = stride();
 #pragma unroll
-  for (int q = 0; q < MS_S - 1; ++q) store(q);
+  for (int q = 0; q < MS_S - 1; ++q) store(q, sb);
 }
 
 // Applies the single-shift rotations 0..wlen-1 (rotation j on elements j, j+1) to one line whose
@@ -765,7 +795,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
   static_assert(BR * BC <= MS_LD * (MS_LEN + 1), "window buffers share one LDS array");
   __shared__ zc MB[MS_LD * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
   __shared__ Rot rots[QR_W];
-  __shared__ Rot2 rots2[MS_S * MS_NB];
+  __shared__ Rot2 rots2[(MS_S + 1) * MS_NB];  // row s: step s of the window; row MS_S: next window's step 0
   __shared__ zc shifts[MS_NB], betas[MS_NB];
   __shared__ int s_k, s_m;
   __shared__ zc s_t, s_v1;
@@ -871,34 +901,37 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
           const int r = rlo + e % nr, c = clo + e / nr;
           MBp(r, c) = Hp(r, c);
         }
+        // reflector of bulge j at step t (zero when the bulge is outside the active block) and the
+        // value it leaves in the bulge column
+        auto reflector = [&](int j, int t, Rot2& g, zc& beta) {
+          const int k = L + t - 2 * j;
+          g.t1 = g.v2 = g.t2 = mk(0.0);
+          beta = mk(0.0);
+          if (j < nb && k >= L && k <= I - 1) {
+            zc v1, v2;
+            if (k == L) {
+              const zc h11s = MBp(L, L) - shifts[j], h21 = MBp(L + 1, L);
+              const double sc = abs1(h11s) + abs1(h21);
+              v1 = h11s * (1.0 / sc);
+              v2 = h21 * (1.0 / sc);
+            } else {
+              v1 = MBp(k, k - 1);
+              v2 = MBp(k + 1, k - 1);
+            }
+            g = make_reflector2(v1, v2);
+            beta = v1;
+          }
+        };
+        if (t0 == 0) {
+          __syncthreads();
+          if (tid < MS_NB) reflector(tid, 0, rots2[tid], betas[tid]);
+        } else if (tid < MS_NB) {
+          rots2[tid] = rots2[MS_S * MS_NB + tid];  // formed in the last step of the previous window
+        }
         __syncthreads();
         const int nsteps = min(MS_S, T_total - t0);
         for (int s = 0; s < nsteps; ++s) {
           const int t = t0 + s;
-          if (tid < MS_NB) {
-            const int j = tid;
-            const int k = L + t - 2 * j;
-            Rot2 g;
-            g.t1 = g.v2 = g.t2 = mk(0.0);
-            zc beta = mk(0.0);
-            if (j < nb && k >= L && k <= I - 1) {
-              zc v1, v2;
-              if (k == L) {
-                const zc h11s = MBp(L, L) - shifts[j], h21 = MBp(L + 1, L);
-                const double sc = abs1(h11s) + abs1(h21);
-                v1 = h11s * (1.0 / sc);
-                v2 = h21 * (1.0 / sc);
-              } else {
-                v1 = MBp(k, k - 1);
-                v2 = MBp(k + 1, k - 1);
-              }
-              g = make_reflector2(v1, v2);
-              beta = v1;
-            }
-            rots2[s * MS_NB + j] = g;
-            betas[j] = beta;
-          }
-          __syncthreads();
           // left products on rows k, k+1 (columns k..rhi) and the bulge column of chase steps
           for (int e = tid; e < MS_NB * MS_LEN; e += nt) {
             const int j = e / MS_LEN, ci = e % MS_LEN;
@@ -913,14 +946,30 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
             }
           }
           __syncthreads();
-          // right products on columns k, k+1 (rows rlo..min(k+2, I))
-          for (int e = tid; e < MS_NB * MS_LEN; e += nt) {
-            const int j = e / MS_LEN, ri = e % MS_LEN;
-            const int k = L + t - 2 * j;
-            if (j >= nb || k < L || k > I - 1) continue;
-            const int r = rlo + ri;
-            if (r > min(k + 2, I)) continue;
-            rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
+          // right products on columns k, k+1 (rows rlo..min(k+2, I)).  Rows k+1 and k+2 of column k
+          // are what bulge j's next reflector is formed from, and nothing else in this step writes
+          // them, so lane j of the last wavefront (which takes no other rows) handles those two rows
+          // and then forms that reflector: a bulge that enters at the next step reads H(L, L) and
+          // H(L+1, L), final after the left products above.
+          const int rl = tid - (nt - 64);
+          if (rl >= 0) {
+            const int j = rl;
+            if (j < MS_NB) {
+              const int k = L + t - 2 * j;
+              if (j < nb && k >= L && k <= I - 1)
+                for (int r = max(k + 1, rlo); r <= min(k + 2, I); ++r)
+                  rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
+              reflector(j, t + 1, rots2[(s + 1) * MS_NB + j], betas[j]);
+            }
+          } else {
+            for (int e = tid; e < MS_NB * MS_LEN; e += nt - 64) {
+              const int j = e / MS_LEN, ri = e % MS_LEN;
+              const int k = L + t - 2 * j;
+              if (j >= nb || k < L || k > I - 1) continue;
+              const int r = rlo + ri;
+              if (r > min(k, I)) continue;
+              rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
+            }
           }
           __syncthreads();
         }
@@ -1053,34 +1102,39 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
       }
       __syncthreads();
       auto Bp = [&](int r, int c) -> zc& { return B[(c - cs) * BR + (r - ks)]; };
-      for (int k = ks; k < ke; ++k) {
-        zc v1;
-        zc v2c;
-        if (k == M) {
-          v1 = s_v1;
-          v2c = mk(s_v2);
-        } else {
-          v1 = Bp(k, k - 1);
-          v2c = Bp(k + 1, k - 1);
-        }
-        // every lane forms the same reflector from the same LDS values; column k-1 is rewritten
-        // only in the right phase below, after the barrier
-        const Rot g = make_reflector(v1, v2c);
-        // left: rows k, k+1, cols k..re
-        for (int c = k + tid; c <= re; c += nt) rot_left(g, Bp(k, c), Bp(k + 1, c));
-        __syncthreads();
-        // right: cols k, k+1, rows ks..min(k+2, I)
-        const int rmax = min(k + 2, I);
-        for (int r = ks + tid; r <= rmax; r += nt) rot_right(g, Bp(r, k), Bp(r, k + 1));
-        if (tid == 0) {
-          if (k > M) {
-            Bp(k, k - 1) = v1;
-            Bp(k + 1, k - 1) = mk(0.0);
+      // the block is at most QR_W + 3 wide: one wavefront chases the bulge through it, with
+      // wavefront-level ordering between the phases
+      static_assert(QR_W + 3 <= 64, "one lane per block row / column");
+      if (tid < 64) {
+        for (int k = ks; k < ke; ++k) {
+          zc v1;
+          zc v2c;
+          if (k == M) {
+            v1 = s_v1;
+            v2c = mk(s_v2);
+          } else {
+            v1 = Bp(k, k - 1);
+            v2c = Bp(k + 1, k - 1);
           }
-          rots[k - ks] = g;
+          // every lane forms the same reflector from the same LDS values; column k-1 is rewritten
+          // only in the right phase below
+          const Rot g = make_reflector(v1, v2c);
+          // left: rows k, k+1, cols k..re
+          if (k + tid <= re) rot_left(g, Bp(k, k + tid), Bp(k + 1, k + tid));
+          wave_sync();
+          // right: cols k, k+1, rows ks..min(k+2, I)
+          if (ks + tid <= min(k + 2, I)) rot_right(g, Bp(ks + tid, k), Bp(ks + tid, k + 1));
+          if (tid == 0) {
+            if (k > M) {
+              Bp(k, k - 1) = v1;
+              Bp(k + 1, k - 1) = mk(0.0);
+            }
+            rots[k - ks] = g;
+          }
+          wave_sync();
         }
-        __syncthreads();
       }
+      __syncthreads();
       for (int e = tid; e < nr * nc; e += nt) {
         int r = e % nr, c = e / nr;
         Hp(ks + r, cs + c) = B[c * BR + r];
