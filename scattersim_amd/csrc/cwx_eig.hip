@@ -17,7 +17,10 @@ namespace cwx {
 
 namespace {
 
-constexpr int EIG_THREADS = 512;  // threads of the QR workgroup
+// Threads of the QR workgroup: 512 when the matrices do not fill the device (one workgroup per CU);
+// 256 when there are at least two matrices per CU, so that two workgroups share a CU and the
+// latency-bound chase of one overlaps the arithmetic-bound deferred products of the other.
+constexpr int EIG_THREADS = 512, EIG_THREADS_SMALL = 256;
 constexpr int QR_W = 32;  // rotations per window
 constexpr double kUlp = 2.220446049250313e-16;      // dlamch('P')
 constexpr double kSafmin = 2.2250738585072014e-308;  // dlamch('S')
@@ -702,7 +705,8 @@ __device__ inline Rot2 scalar_rot(const KDouble* R, int i) {
 // [qlo, qhi] lie outside the active block: they enter as zero, see only identity rotations and are
 // not stored.  FULL: the whole window lies inside the active block.
 template <bool FULL>
-__device__ inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int qlo, int qhi, const KDouble* R) {
+__device__ __attribute__((always_inline)) inline void ms_apply_line(zc* p, long st, unsigned long long sgn, int qlo, int qhi,
+                                                                  const KDouble* R) {
   constexpr int Q0 = 2 * (MS_NB - 1);
   zc x[MS_LEN];
   auto in = [&](int q) { return FULL || (q >= qlo && q <= qhi); };
@@ -789,7 +793,8 @@ __device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen)
   p[(long)wlen * stride] = a;
 }
 
-__global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
+template <int NT>
+__global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
                                                      long sZ, zc* W, long sW, int* info, double* ring) {
   constexpr int BR = QR_LD, BC = QR_W + 3;  // single-shift block: QR_W + 2 rows (leading dimension QR_LD)
   static_assert(BR * BC <= MS_LD * (MS_LEN + 1), "window buffers share one LDS array");
@@ -995,7 +1000,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_hqr(int n, zc* H, int ldh, long
         {
           // the waves fetch the ring's 64-byte lines into the scalar cache together (one dword of
           // each, all loads in flight), so the deferred products start on hits
-          constexpr int kLines = MS_RING * 8 / 64, kWaves = EIG_THREADS / 64;
+          constexpr int kLines = MS_RING * 8 / 64, kWaves = NT / 64;
           static_assert(kLines % kWaves == 0, "ring lines split evenly over the waves");
           typedef __attribute__((address_space(4))) const unsigned KU32;
           const unsigned* rp = reinterpret_cast<const unsigned*>(rg);
@@ -1339,8 +1344,15 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     zgemm_set(OpN, OpN, m, m, nb, mk(-1.0), S(Vp, nn), n, S(Wk2, sWk), HB, mk(1.0), S(Zsub, nn), n, batch, s);
   }
   hipLaunchKernelGGL(k_clear_below_sub, dim3(cdiv(n * n, 256 * 4), batch), dim3(256), 0, s, n, A, lda, sA);
-  hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n, nn, W,
-                     sW, info, work.ring.p);
+  int dev = 0, cus = 0;
+  CWX_HIP(hipGetDevice(&dev));
+  CWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (batch >= 2 * cus)
+    hipLaunchKernelGGL(k_hqr<EIG_THREADS_SMALL>, dim3(batch), dim3(EIG_THREADS_SMALL), 0, s, n, A, lda, sA,
+                       work.Z.p, n, nn, W, sW, info, work.ring.p);
+  else
+    hipLaunchKernelGGL(k_hqr<EIG_THREADS>, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n,
+                       nn, W, sW, info, work.ring.p);
   const double smlnum = kSafmin * ((double)n / kUlp);
   // one wavefront per eigenvector, as many per workgroup as fit 64 KiB of LDS
   const int waves = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / (sizeof(zc) * n)));

@@ -135,3 +135,35 @@ def test_batched_dev_entry_points(ctx):
     Cm = dC.numpy().transpose(0, 2, 1)
     for i in range(b):
         assert np.max(np.abs(Cm[i] - G[i].conj().T @ G[i])) < 1e-12 * np.abs(Cm[i]).max()
+
+
+def test_zgeev_large_batch_matches_small_batch(ctx):
+    # A batch of at least two matrices per CU runs the QR with two smaller workgroups per CU; the
+    # arithmetic of every matrix is the same either way, so results are bit-identical to solving
+    # the matrices a few at a time.  n = 100 takes the multishift sweeps (active block >= 72).
+    rng = np.random.default_rng(21)
+    n, b = 100, 640
+    A = crand(rng, b, n, n)
+    lib = sa.lib()
+
+    def run(Ab):
+        k = Ab.shape[0]
+        dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(Ab.transpose(0, 2, 1)))
+        dW = sa.DeviceArray(ctx, (k, n), np.complex128)
+        dV = sa.DeviceArray(ctx, (k, n, n), np.complex128)
+        info = np.zeros(k, np.int32)
+        sa.check(lib.cwx_zgeev_batched_dev(ctx.h, n, k, dA.ptr, n * n, dW.ptr, n, dV.ptr, n * n,
+                                           info.ctypes.data))
+        out = dW.numpy(), dV.numpy().transpose(0, 2, 1), info
+        for d in (dA, dW, dV):
+            d.free()
+        return out
+
+    W, V, info = run(A)
+    assert np.all(info == 0)
+    picks = [0, 1, 317, b - 1]
+    Ws, Vs, infos = run(A[picks])
+    assert np.all(infos == 0)
+    for i, p in enumerate(picks):
+        assert np.array_equal(W[p], Ws[i]) and np.array_equal(V[p], Vs[i])
+        check_eig(A[p], W[p], V[p])
