@@ -584,16 +584,44 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
 // ------------------------------------------------------------------------------------------------
 // getrs pieces: row interchanges on B, and small diagonal-block triangular solves.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_laswp(int n, int nrhs, MatSet Bset, int ldb, IntSet pset) {
-  const int b = blockIdx.y;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nrhs) return;
+// Row interchanges ipiv (zlaswp order: rows 0..n-1 in turn) on the nrhs columns of B, as one gather.
+// The interchanges compose to B'(i, :) = B(src(i), :); src is formed in LDS by one lane (n dependent
+// steps, once per matrix), then columns pass through LDS in chunks of ch: read along the column and
+// written back permuted, both coalesced, instead of n dependent swaps of scattered 16-byte elements
+// per column.  One workgroup per matrix; dynamic LDS: n ints, then ch x n complex.
+constexpr int SWP_THREADS = 256;
+__global__ __launch_bounds__(SWP_THREADS) void k_laswp(int n, int nrhs, int ch, MatSet Bset, int ldb, IntSet pset) {
+  extern __shared__ __align__(16) unsigned char swp_lds[];
+  zc* buf = reinterpret_cast<zc*>(swp_lds);
+  int* src = reinterpret_cast<int*>(swp_lds + sizeof(zc) * (size_t)ch * n);
+  const int b = blockIdx.x;
   zc* B = Bset.at(b);
   const int* ipiv = pset.at(b);
-  zc* col = B + (long)c * ldb;
-  for (int j = 0; j < n; ++j) {
-    int p = ipiv[j];
-    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) src[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < n; ++j) {
+      const int p = ipiv[j];
+      if (p != j) {
+        const int t = src[j];
+        src[j] = src[p];
+        src[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < nrhs; c0 += ch) {
+    const int cn = min(ch, nrhs - c0);
+    for (int e = threadIdx.x; e < cn * n; e += blockDim.x) {
+      const int c = e / n, r = e % n;
+      buf[e] = B[(long)(c0 + c) * ldb + r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < cn * n; e += blockDim.x) {
+      const int c = e / n, r = e % n;
+      B[(long)(c0 + c) * ldb + r] = buf[c * n + src[r]];
+    }
+    __syncthreads();
   }
 }
 
@@ -686,7 +714,14 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
 void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int ldb, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
-  hipLaunchKernelGGL(k_laswp, dim3(cdiv(nrhs, 64), batch), dim3(64), 0, s, n, nrhs, B, ldb, ipiv);
+  {
+    // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map
+    const size_t avail = 64 * 1024 - sizeof(int) * (size_t)n;
+    CWX_CHECK(sizeof(int) * (size_t)n + sizeof(zc) * (size_t)n <= 64 * 1024, "zgetrs: n too large for the LDS row map");
+    const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
+    const size_t lds = sizeof(zc) * (size_t)ch * n + sizeof(int) * (size_t)n;
+    hipLaunchKernelGGL(k_laswp, dim3(batch), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
+  }
   // Blocks of TS_NB rows: inside a block, diagonal LU_NB blocks and updates of the block's own rows;
   // the rows below (above) are updated once per block, which bounds the passes over B.
   constexpr int TS_NB = 128;
