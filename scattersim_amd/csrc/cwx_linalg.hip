@@ -252,8 +252,10 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_mfma(Op opa, Op opb, int m, in
 constexpr int LU_NB = 32;
 constexpr int LU_THREADS = 256;
 
-__device__ inline void block_argmax(double v, int idx, double* sv, int* si, double* outv, int* outi) {
-  // reduce (value, index) pairs: larger value wins, ties -> smaller index
+__device__ inline void block_argmax(double v, int idx, double* sv, int* si, double& outv, int& outi) {
+  // reduce (value, index) pairs: larger value wins, ties -> smaller index.  After one barrier every
+  // lane reduces the per-wavefront partials itself.  The partials are rewritten by the next call, so
+  // callers put a barrier between consecutive calls.
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int off = 32; off > 0; off >>= 1) {
     double ov = __shfl_xor(v, off);
@@ -262,23 +264,18 @@ __device__ inline void block_argmax(double v, int idx, double* sv, int* si, doub
   }
   if (lane == 0) { sv[wid] = v; si[wid] = idx; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double bv = sv[0];
-    int bi = si[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-      if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) { bv = sv[w]; bi = si[w]; }
-    *outv = bv;
-    *outi = bi;
-  }
-  __syncthreads();
+  double bv = sv[0];
+  int bi = si[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+    if (sv[w] > bv || (sv[w] == bv && si[w] < bi)) { bv = sv[w]; bi = si[w]; }
+  outv = bv;
+  outi = bi;
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, MatSet Aset, int lda,
                                                          IntSet pset, int* info) {
   __shared__ double sv[LU_THREADS / 64];
   __shared__ int si[LU_THREADS / 64];
-  __shared__ double pv;
-  __shared__ int pi;
   const int b = blockIdx.x;
   zc* A = Aset.at(b);
   int* ipiv = pset.at(b);
@@ -291,9 +288,9 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
       double v = abs1(A[(long)j * lda + r]);
       if (v > best) { best = v; bidx = r; }
     }
-    block_argmax(best, bidx, sv, si, &pv, &pi);
-    const int p = pi;
-    const double pval = pv;
+    int p;
+    double pval;
+    block_argmax(best, bidx, sv, si, pval, p);
     if (threadIdx.x == 0) {
       ipiv[j] = p;
       if (pval == 0.0 && info[b] == 0) info[b] = j + 1;
@@ -339,37 +336,37 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
 // ------------------------------------------------------------------------------------------------
 constexpr int LP_THREADS = 512, LP_Q = 4, LP_W = 8;
 
-// Rows j0 + pos[i] take the old contents of rows j0 + src[i] in columns [cl, cl + ncol), where
-// (pos, src) is the composite permutation of the interchanges (t, piv[t]) for t in [t0, t1)
-// (panel-relative rows).  Called by the whole workgroup; ends with a barrier.
-__device__ void lp_gather_swaps(zc* A, int lda, int j0, const int* piv, int t0, int t1, int cl, int ncol, int* pos,
-                                int* src, int* npos) {
+// The interchanges (t, piv[t]), t0 <= t < t1 (panel-relative rows, applied in order), on the columns
+// [cl, cl + ncol) of the panel.  Only the rows they touch change: rowmap (LDS, R entries) tracks
+// which original row each position holds, one lane updating it in O(1) per interchange; then every
+// (touched row, column) pair reads its source before any of them writes.  Needs
+// 2 (t1 - t0) ncol <= blockDim.x.
+__device__ void lp_gather_swaps(zc* A, int lda, int j0, const int* piv, int t0, int t1, int cl, int ncol,
+                                int* rowmap, int R) {
+  for (int i = threadIdx.x; i < R; i += blockDim.x) rowmap[i] = i;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int cnt = 0;
     for (int t = t0; t < t1; ++t) {
       const int a = t, b = piv[t];
       if (a == b) continue;
-      int ia = -1, ib = -1;
-      for (int i = 0; i < cnt; ++i) {
-        if (pos[i] == a) ia = i;
-        if (pos[i] == b) ib = i;
-      }
-      if (ia < 0) { pos[cnt] = a; src[cnt] = a; ia = cnt++; }
-      if (ib < 0) { pos[cnt] = b; src[cnt] = b; ib = cnt++; }
-      const int tmp = src[ia];
-      src[ia] = src[ib];
-      src[ib] = tmp;
+      const int tmp = rowmap[a];
+      rowmap[a] = rowmap[b];
+      rowmap[b] = tmp;
     }
-    *npos = cnt;
   }
   __syncthreads();
-  const int cnt = *npos;
+  const int nt = t1 - t0, e = threadIdx.x;
+  const bool act = ncol > 0 && e < 2 * nt * ncol;
+  int row = 0, c = 0;
   zc val = mk(0.0);
-  const int e = threadIdx.x, i = e / max(ncol, 1), c = e % max(ncol, 1);
-  const bool act = ncol > 0 && i < cnt;
-  if (act) val = A[(long)(j0 + cl + c) * lda + j0 + src[i]];
+  if (act) {
+    const int cand = e / ncol;
+    c = e % ncol;
+    row = cand < nt ? t0 + cand : piv[t0 + cand - nt];  // duplicates move the same value twice
+    val = A[(long)(j0 + cl + c) * lda + j0 + rowmap[row]];
+  }
   __syncthreads();
-  if (act) A[(long)(j0 + cl + c) * lda + j0 + pos[i]] = val;
+  if (act) A[(long)(j0 + cl + c) * lda + j0 + row] = val;
   __syncthreads();
 }
 
@@ -377,11 +374,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
                                                              IntSet pset, int* info) {
   __shared__ double sv[LP_THREADS / 64];
   __shared__ int si[LP_THREADS / 64];
-  __shared__ double pv;
-  __shared__ int pi;
   __shared__ zc prow[16], orow[16];
   __shared__ zc L11[16][17], U12[16][17];
-  __shared__ int spiv[32], gpos[64], gsrc[64], gcnt;
+  __shared__ int spiv[32], rowmap[256 * LP_Q];
   const int b = blockIdx.x;
   zc* A = Aset.at(b);
   int* ipiv = pset.at(b);
@@ -393,7 +388,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
     if (hn <= 0) break;
     if (half == 1) {
       // the first half's interchanges on this half's columns, then U12 = L11^-1 A12
-      lp_gather_swaps(A, lda, j0, spiv, 0, 16, 16, hn, gpos, gsrc, &gcnt);
+      lp_gather_swaps(A, lda, j0, spiv, 0, 16, 16, hn, rowmap, R);
       for (int e = tid; e < 16 * 16; e += LP_THREADS) {
         const int r = e & 15, c = e >> 4;
         L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
@@ -401,11 +396,16 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       }
       __syncthreads();
       if (tid < hn) {
-        for (int r = 1; r < 16; ++r) {
-          zc acc = U12[r][tid];
-          for (int q = 0; q < r; ++q) acc = acc - L11[r][q] * U12[q][tid];
-          U12[r][tid] = acc;
-        }
+        // forward substitution down one column, held in registers
+        zc u[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) u[r] = U12[r][tid];
+#pragma unroll
+        for (int r = 1; r < 16; ++r)
+#pragma unroll
+          for (int q = 0; q < r; ++q) u[r] = u[r] - L11[r][q] * u[q];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) U12[r][tid] = u[r];
       }
       __syncthreads();
     }
@@ -421,18 +421,30 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
     }
     if (half == 1) {
       // A22 -= L21 U12 (rows below 16), rows 0..15 become U12
+      // L21 is read from memory two columns at a time, so that the loads of a pair are in flight
+      // together
 #pragma unroll 1
-      for (int k = 0; k < 16; ++k) {
-        zc u[LP_W];
+      for (int k0 = 0; k0 < 16; k0 += 2) {
+        zc lk[2][LP_Q];
 #pragma unroll
-        for (int c = 0; c < LP_W; ++c) u[c] = U12[k][h * LP_W + c];
+        for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-        for (int q = 0; q < LP_Q; ++q) {
-          const int r = p + 256 * q;
-          if (r >= 16 && r < R) {
-            const zc l = A[(long)(j0 + k) * lda + j0 + r];
+          for (int q = 0; q < LP_Q; ++q) {
+            const int r = p + 256 * q;
+            lk[kk][q] = (r >= 16 && r < R) ? A[(long)(j0 + k0 + kk) * lda + j0 + r] : mk(0.0);
+          }
 #pragma unroll
-            for (int c = 0; c < LP_W; ++c) x[q][c] = x[q][c] - l * u[c];
+        for (int kk = 0; kk < 2; ++kk) {
+          zc u[LP_W];
+#pragma unroll
+          for (int c = 0; c < LP_W; ++c) u[c] = U12[k0 + kk][h * LP_W + c];
+#pragma unroll
+          for (int q = 0; q < LP_Q; ++q) {
+            const int r = p + 256 * q;
+            if (r >= 16 && r < R) {
+#pragma unroll
+              for (int c = 0; c < LP_W; ++c) x[q][c] = x[q][c] - lk[kk][q] * u[c];
+            }
           }
         }
       }
@@ -466,9 +478,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
           }
         }
       }
-      block_argmax(best, bidx, sv, si, &pv, &pi);
-      const int pr = pi;
-      const double pval = pv;
+      int pr;
+      double pval;
+      block_argmax(best, bidx, sv, si, pval, pr);
       if (tid == 0) {
         ipiv[j0 + j] = j0 + pr;
         spiv[j] = pr;
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
     __syncthreads();
   }
   // the second half's interchanges on the first half's columns
-  if (nb > 16) lp_gather_swaps(A, lda, j0, spiv, 16, nb, 0, 16, gpos, gsrc, &gcnt);
+  if (nb > 16) lp_gather_swaps(A, lda, j0, spiv, 16, nb, 0, 16, rowmap, R);
 }
 
 // Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
