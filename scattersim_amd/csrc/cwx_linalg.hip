@@ -257,7 +257,38 @@ __device__ inline void block_argmax(double v, int idx, double* sv, int* si, doub
   // lane reduces the per-wavefront partials itself.  The partials are rewritten by the next call, so
   // callers put a barrier between consecutive calls.
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int off = 32; off > 0; off >>= 1) {
+  // within each row of 16 lanes: rotations by 1, 2, 4, 8 (DPP row_ror) leave the row's best pair in
+  // every lane of the row; then across rows through LDS permutes
+  auto ror = [](double x, int ctrl_sel) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    unsigned lo, hi;
+    switch (ctrl_sel) {
+      case 1: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x121, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x121, 0xF, 0xF, false); break;
+      case 2: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x122, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x122, 0xF, 0xF, false); break;
+      case 4: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x124, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x124, 0xF, 0xF, false); break;
+      default: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x128, 0xF, 0xF, false);
+               hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false); break;
+    }
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  };
+  auto rori = [](int x, int ctrl_sel) {
+    switch (ctrl_sel) {
+      case 1: return __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false);
+      case 2: return __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false);
+      case 4: return __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);
+      default: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+    }
+  };
+#pragma unroll
+  for (int sh = 1; sh <= 8; sh <<= 1) {
+    const double ov = ror(v, sh);
+    const int oi = rori(idx, sh);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  for (int off = 16; off < 64; off <<= 1) {
     double ov = __shfl_xor(v, off);
     int oi = __shfl_xor(idx, off);
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
@@ -328,13 +359,22 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
 
 // ------------------------------------------------------------------------------------------------
 // LU panel held in registers (same pivots and arithmetic order as k_lu_panel): 512 lanes as 256
-// pairs; pair p owns panel rows p + 256 q (q < LP_Q), and the panel is factored in two halves of 16
+// pairs; pair p owns panel rows p + 256 q (q < Q), and the panel is factored in two halves of 16
 // columns, each lane of a pair holding 8 of the half's columns.  Between the halves the first half's
 // row interchanges are applied to the second half (one gather of the composite permutation),
-// U12 = L11^-1 A12 is solved in LDS and A22 -= L21 U12 is applied in registers; at the end the second
-// half's interchanges are applied to the first half.  Panels of at most 256 LP_Q rows.
+// U12 = L11^-1 A12 is solved in registers and A22 -= L21 U12 is applied in registers; at the end the
+// second half's interchanges are applied to the first half.  Panels of at most 256 Q rows; Q is the
+// smallest of 1, 2, 4 that covers the panel, since every lane issues the work of all Q rows.
 // ------------------------------------------------------------------------------------------------
-constexpr int LP_THREADS = 512, LP_Q = 4, LP_W = 8;
+constexpr int LP_THREADS = 512, LP_QMAX = 4, LP_W = 8;
+
+// The value of lane ^ 1 (a DPP quad permutation [1, 0, 3, 2], no LDS round trip).
+__device__ inline double pair_swap(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0xB1, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 
 // The interchanges (t, piv[t]), t0 <= t < t1 (panel-relative rows, applied in order), on the columns
 // [cl, cl + ncol) of the panel.  Only the rows they touch change: rowmap (LDS, R entries) tracks
@@ -370,13 +410,14 @@ __device__ void lp_gather_swaps(zc* A, int lda, int j0, const int* piv, int t0, 
   __syncthreads();
 }
 
+template <int Q>
 __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int nb, MatSet Aset, int lda,
                                                              IntSet pset, int* info) {
   __shared__ double sv[LP_THREADS / 64];
   __shared__ int si[LP_THREADS / 64];
   __shared__ zc prow[16], orow[16];
   __shared__ zc L11[16][17], U12[16][17];
-  __shared__ int spiv[32], rowmap[256 * LP_Q];
+  __shared__ int spiv[32], rowmap[256 * Q];
   const int b = blockIdx.x;
   zc* A = Aset.at(b);
   int* ipiv = pset.at(b);
@@ -409,9 +450,9 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       }
       __syncthreads();
     }
-    zc x[LP_Q][LP_W];
+    zc x[Q][LP_W];
 #pragma unroll
-    for (int q = 0; q < LP_Q; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int r = p + 256 * q;
 #pragma unroll
       for (int c = 0; c < LP_W; ++c) {
@@ -425,11 +466,11 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       // together
 #pragma unroll 1
       for (int k0 = 0; k0 < 16; k0 += 2) {
-        zc lk[2][LP_Q];
+        zc lk[2][Q];
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-          for (int q = 0; q < LP_Q; ++q) {
+          for (int q = 0; q < Q; ++q) {
             const int r = p + 256 * q;
             lk[kk][q] = (r >= 16 && r < R) ? A[(long)(j0 + k0 + kk) * lda + j0 + r] : mk(0.0);
           }
@@ -439,7 +480,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
 #pragma unroll
           for (int c = 0; c < LP_W; ++c) u[c] = U12[k0 + kk][h * LP_W + c];
 #pragma unroll
-          for (int q = 0; q < LP_Q; ++q) {
+          for (int q = 0; q < Q; ++q) {
             const int r = p + 256 * q;
             if (r >= 16 && r < R) {
 #pragma unroll
@@ -449,7 +490,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
         }
       }
 #pragma unroll
-      for (int q = 0; q < LP_Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int r = p + 256 * q;
         if (r < 16) {
 #pragma unroll
@@ -466,7 +507,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       int bidx = R;
       if (h == hj) {
 #pragma unroll
-        for (int q = 0; q < LP_Q; ++q) {
+        for (int q = 0; q < Q; ++q) {
           const int r = p + 256 * q;
           zc v = mk(0.0);
 #pragma unroll
@@ -490,7 +531,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       const int pp = pr & 255, qp = pr >> 8;
       if (p == pp) {
 #pragma unroll
-        for (int q = 0; q < LP_Q; ++q)
+        for (int q = 0; q < Q; ++q)
           if (q == qp) {
 #pragma unroll
             for (int c = 0; c < LP_W; ++c) prow[h * LP_W + c] = x[q][c];
@@ -508,7 +549,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
         }
         if (p == pp) {
 #pragma unroll
-          for (int q = 0; q < LP_Q; ++q)
+          for (int q = 0; q < Q; ++q)
             if (q == qp) {
 #pragma unroll
               for (int c = 0; c < LP_W; ++c) x[q][c] = orow[h * LP_W + c];
@@ -517,7 +558,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       }
       const zc inv = (pval != 0.0) ? zdiv(mk(1.0), prow[jj]) : mk(0.0);
 #pragma unroll
-      for (int q = 0; q < LP_Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int r = p + 256 * q;
         const bool act = (r > j && r < R);
         zc v = mk(0.0);
@@ -526,8 +567,8 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
           if (c == cj) v = x[q][c];
         zc l = (h == hj) ? v * inv : mk(0.0);
         zc lo;
-        lo.re = __shfl_xor(l.re, 1);
-        lo.im = __shfl_xor(l.im, 1);
+        lo.re = pair_swap(l.re);
+        lo.im = pair_swap(l.im);
         if (h != hj) l = lo;
         if (act) {
 #pragma unroll
@@ -540,7 +581,7 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
       }
     }
 #pragma unroll
-    for (int q = 0; q < LP_Q; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int r = p + 256 * q;
 #pragma unroll
       for (int c = 0; c < LP_W; ++c) {
@@ -707,8 +748,12 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
   hipLaunchKernelGGL(k_fill_int, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, info, 0);
   for (int j0 = 0; j0 < n; j0 += LU_NB) {
     const int nb = std::min(LU_NB, n - j0);
-    if (n - j0 <= 256 * LP_Q)
-      hipLaunchKernelGGL(k_lu_panel_reg, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
+    if (n - j0 <= 256)
+      hipLaunchKernelGGL(k_lu_panel_reg<1>, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (n - j0 <= 512)
+      hipLaunchKernelGGL(k_lu_panel_reg<2>, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (n - j0 <= 256 * LP_QMAX)
+      hipLaunchKernelGGL(k_lu_panel_reg<4>, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     else
       hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     const int others = n - nb;
