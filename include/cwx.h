@@ -178,6 +178,17 @@ const void* cwx_solution_device_ptr(const cwx_solution* sol, int what);
 int cwx_solution_lists(const cwx_solution* sol, int64_t* offsets, int64_t* counts);
 /* CoupledWaveStructure::save (CoupledWaveStructure.h:63-92) with 96-byte records. */
 int cwx_solution_write_cw(cwx_context* ctx, const cwx_solution* sol, const char* path);
+/* Compact heterogeneous .cw (docs/cw_file_format.pptx slide 1, read by ScatterView.py:68-178):
+ * is_volume = 1, the incident, reflected and transmitted waves only, then per layer the mode data
+ * beta, gamma, gg (4MF x 4MF, row-major) and the n^-2 coefficients, instead of every interior plane
+ * wave.  A trailer after the layers (past the reference reader's end) keeps the wavelength, the
+ * incident field and the boundary solution, so cwx_solution_read_cw2 can rebuild the solution.
+ * An --external solution (no beta) is refused with CWX_ERR_ARG. */
+int cwx_solution_write_cw2(cwx_context* ctx, const cwx_solution* sol, const char* path);
+/* Rebuilds a device solution from a file written by cwx_solution_write_cw2 (the interior wave
+ * records, Q rows and tables are recomputed from the mode data, as cpuEvaluator.cpp:109-210 does
+ * from the reference's compact files).  Files without the trailer are refused with CWX_ERR_IO. */
+int cwx_solution_read_cw2(cwx_context* ctx, const char* path, cwx_solution** out);
 int cwx_solution_free(cwx_solution* sol);
 
 /* ---- .cw files (CoupledWaveStructure.h:94-125 + cpuEvaluator.cpp:34-102) --------------------- */

@@ -22,7 +22,7 @@ from . import _capi
 from ._capi import check, lib
 
 __all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
-           "cw_read_unpacked", "WAVE_F32", "WAVE_F64"]
+           "cw_read_unpacked", "read_cw2", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
 WAVE_F64 = np.dtype([("E0", np.complex128, 3), ("k", np.complex128, 3)])
@@ -236,6 +236,17 @@ class Solution:
 
     def write_cw(self, path: str):
         check(lib().cwx_solution_write_cw(self.ctx.h, self.h, path.encode()))
+
+    def write_cw2(self, path: str):
+        """Compact heterogeneous .cw: external waves plus the mode data of every layer (cwx_solution_write_cw2)."""
+        check(lib().cwx_solution_write_cw2(self.ctx.h, self.h, path.encode()))
+
+
+def read_cw2(ctx: Context, path: str) -> Solution:
+    """Rebuild a device solution from a compact .cw file written by ``Solution.write_cw2``."""
+    h = C.c_void_p()
+    check(lib().cwx_solution_read_cw2(ctx.h, path.encode(), C.byref(h)))
+    return Solution(ctx, h)
 
 
 def scattervolume(ctx: Context, sample: np.ndarray, **opts) -> Solution:

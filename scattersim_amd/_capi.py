@@ -83,6 +83,8 @@ SIGNATURES = {
     "cwx_solution_device_ptr": (P, [P, I]),
     "cwx_solution_lists": (I, [P, P, P]),
     "cwx_solution_write_cw": (I, [P, P, C.c_char_p]),
+    "cwx_solution_write_cw2": (I, [P, P, C.c_char_p]),
+    "cwx_solution_read_cw2": (I, [P, C.c_char_p, P]),
     "cwx_solution_free": (I, [P]),
     "cwx_cw_read_unpacked": (I, [C.c_char_p, C.POINTER(I64), C.POINTER(I), P, P, P, P]),
     "cwx_field_init_f32": (I, [P, P, I64, P, I, P, P]),

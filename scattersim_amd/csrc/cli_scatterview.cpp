@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -60,33 +61,62 @@ int main(int argc, char** argv) {
   if (axis >= 0 && axis <= 2) plane[axis] = slice;
   const auto t0 = std::chrono::system_clock::now();
 
-  int64_t nw = 0;
-  int layers = 0;
-  if (cwx_cw_read_unpacked(input.c_str(), &nw, &layers, nullptr, nullptr, nullptr, nullptr) != CWX_OK) {
-    std::cout << "ERROR: file " << input << " not found" << std::endl;
-    return 1;
-  }
-  std::vector<cwx_wave_f64> W(nw);
-  std::vector<double> zl(layers);
-  std::vector<int> wb(layers + 1), we(layers + 1);
-  if (cwx_cw_read_unpacked(input.c_str(), &nw, &layers, W.data(), zl.data(), wb.data(), we.data()) != CWX_OK) {
-    std::cerr << "error: " << cwx_last_error() << std::endl;
-    return 1;
-  }
-  // cw_unpack (cpuEvaluator.cpp:61-102): the evaluator works on FP32 copies of the waves
-  std::vector<cwx_wave_f32> Wf(nw);
-  for (int64_t i = 0; i < nw; ++i)
-    for (int c = 0; c < 3; ++c) {
-      Wf[i].E0[c] = cwx_complexf{(float)W[i].E0[c].re, (float)W[i].E0[c].im};
-      Wf[i].k[c] = cwx_complexf{(float)W[i].k[c].re, (float)W[i].k[c].im};
+  // A compact heterogeneous file (is_volume byte 1, docs/cw_file_format.pptx slide 1) holds mode data
+  // instead of the interior waves: rebuild the solution from it and evaluate its FP64 records.
+  bool compact = false;
+  {
+    std::ifstream f(input, std::ios::binary);
+    if (!f) {
+      std::cout << "ERROR: file " << input << " not found" << std::endl;
+      return 1;
     }
-  std::vector<float> zf(zl.begin(), zl.end());
-  cwx_context* ctx = nullptr;
-  if (cwx_create(device, &ctx) != CWX_OK ||
-      cwx_field_init_f32(ctx, Wf.data(), nw, zf.data(), layers, wb.data(), we.data()) != CWX_OK) {
-    std::cerr << "error: " << cwx_last_error() << std::endl;
-    return 2;
+    char b0 = 0;
+    compact = f.read(&b0, 1) && b0 == 1;
   }
+  cwx_context* ctx = nullptr;
+  cwx_solution* sol = nullptr;
+  if (compact) {
+    if (cwx_create(device, &ctx) != CWX_OK || cwx_solution_read_cw2(ctx, input.c_str(), &sol) != CWX_OK ||
+        cwx_field_init_solution(ctx, sol) != CWX_OK) {
+      std::cerr << "error: " << cwx_last_error() << std::endl;
+      if (sol) cwx_solution_free(sol);
+      if (ctx) cwx_destroy(ctx);
+      return 2;
+    }
+  } else {
+    int64_t nw = 0;
+    int layers = 0;
+    if (cwx_cw_read_unpacked(input.c_str(), &nw, &layers, nullptr, nullptr, nullptr, nullptr) != CWX_OK) {
+      std::cout << "ERROR: file " << input << " not found" << std::endl;
+      return 1;
+    }
+    std::vector<cwx_wave_f64> W(nw);
+    std::vector<double> zl(layers);
+    std::vector<int> wb(layers + 1), we(layers + 1);
+    if (cwx_cw_read_unpacked(input.c_str(), &nw, &layers, W.data(), zl.data(), wb.data(), we.data()) != CWX_OK) {
+      std::cerr << "error: " << cwx_last_error() << std::endl;
+      return 1;
+    }
+    // cw_unpack (cpuEvaluator.cpp:61-102): the evaluator works on FP32 copies of the waves
+    std::vector<cwx_wave_f32> Wf(nw);
+    for (int64_t i = 0; i < nw; ++i)
+      for (int c = 0; c < 3; ++c) {
+        Wf[i].E0[c] = cwx_complexf{(float)W[i].E0[c].re, (float)W[i].E0[c].im};
+        Wf[i].k[c] = cwx_complexf{(float)W[i].k[c].re, (float)W[i].k[c].im};
+      }
+    std::vector<float> zf(zl.begin(), zl.end());
+    if (cwx_create(device, &ctx) != CWX_OK ||
+        cwx_field_init_f32(ctx, Wf.data(), nw, zf.data(), layers, wb.data(), we.data()) != CWX_OK) {
+      std::cerr << "error: " << cwx_last_error() << std::endl;
+      if (ctx) cwx_destroy(ctx);
+      return 2;
+    }
+  }
+  auto finish = [&](int rc) {
+    if (sol) cwx_solution_free(sol);
+    cwx_destroy(ctx);
+    return rc;
+  };
   if (verbose) std::cout << "Using device " << device << " (" << cwx_version() << ")" << std::endl;
   // EvaluateVectorSlices (ScatterView.cpp:354-376)
   const unsigned N = (unsigned)std::pow(2, resolution);
@@ -96,8 +126,7 @@ int main(int argc, char** argv) {
   if (cwx_field_evaluate(ctx, Exy.data(), Exz.data(), Eyz.data(), xs, ys, zs, plane[0], plane[1], plane[2], d,
                          (int)N) != CWX_OK) {
     std::cerr << "error: " << cwx_last_error() << std::endl;
-    cwx_destroy(ctx);
-    return 2;
+    return finish(2);
   }
   const auto t1 = std::chrono::system_clock::now();
   std::cout << "Evaluation (no gui) takes:" << std::chrono::duration<double>(t1 - t0).count() << "s" << std::endl;
@@ -108,14 +137,11 @@ int main(int argc, char** argv) {
     else if (axis == 0) cwx_npy::save_c64(out, shape, Eyz.data());
     else {
       std::cout << "Wrong click at the wrong region. " << std::endl;
-      cwx_destroy(ctx);
-      return 1;
+      return finish(1);
     }
   } catch (const std::exception& e) {
     std::cerr << "error: " << e.what() << std::endl;
-    cwx_destroy(ctx);
-    return 2;
+    return finish(2);
   }
-  cwx_destroy(ctx);
-  return 1;
+  return finish(1);
 }

@@ -39,6 +39,7 @@ const char* kHelp =
     "  --npz                  calculate Px and Py only\n"
     "  --reduced              (cwx) solve the 2MF eigenproblem of P Q instead of the 4MF Phi\n"
     "  --no-cutoff            (cwx) keep the requested orders (skip the order cut-off heuristic)\n"
+    "  --compact              (cwx) write the compact heterogeneous .cw (mode data per layer)\n"
     "  --device arg (=0)      (cwx) HIP device\n";
 
 std::vector<double> nums(const cwx_cli::Args& a, const std::string& k, std::vector<double> def) {
@@ -54,7 +55,7 @@ int main(int argc, char** argv) {
   const auto t0 = std::chrono::steady_clock::now();
   cwx_cli::Args a;
   try {
-    a = cwx_cli::parse(argc, argv, {}, {"help", "psf", "external", "log", "npz", "reduced", "no-cutoff"});
+    a = cwx_cli::parse(argc, argv, {}, {"help", "psf", "external", "log", "npz", "reduced", "no-cutoff", "compact"});
   } catch (const std::exception& e) {
     std::cerr << e.what() << std::endl;
     return 1;
@@ -150,7 +151,9 @@ int main(int argc, char** argv) {
   std::cout << "Number of Fourier coefficients (Mx, My): [" << info.M0 << "," << info.M1 << "]" << std::endl;
   std::cout << "Effective number of Fourier coefficients: [" << info.MF << "]" << std::endl;
   if (!output.empty()) {
-    if (cwx_solution_write_cw(ctx, sol, output.c_str()) != CWX_OK) {
+    const int wst = a.has("compact") ? cwx_solution_write_cw2(ctx, sol, output.c_str())
+                                     : cwx_solution_write_cw(ctx, sol, output.c_str());
+    if (wst != CWX_OK) {
       std::cerr << "error: " << cwx_last_error() << std::endl;
       cwx_solution_free(sol);
       cwx_destroy(ctx);

@@ -535,6 +535,40 @@ int cwx_solution_write_cw(cwx_context* ctx, const cwx_solution* sol, const char*
   });
 }
 
+int cwx_solution_write_cw2(cwx_context* ctx, const cwx_solution* sol, const char* path) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && sol && path, "NULL argument");
+    if (sol->s.opt.external)
+      return fail(CWX_ERR_ARG, "an --external solution has no interior mode amplitudes (beta); "
+                               "cwx_solution_write_cw already writes only its external waves");
+    set_device(ctx);
+    try {
+      cwx::cw2_write_file(sol->s, path, ctx->c.stream);
+    } catch (const cwx::Error& e) {
+      if (e.code == -1) return fail(CWX_ERR_IO, e.what());
+      throw;
+    }
+    return CWX_OK;
+  });
+}
+
+int cwx_solution_read_cw2(cwx_context* ctx, const char* path, cwx_solution** out) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && path && out, "NULL argument");
+    *out = nullptr;
+    set_device(ctx);
+    std::unique_ptr<cwx_solution> sol(new cwx_solution);
+    try {
+      cwx::cw2_read_file(ctx->c, path, sol->s);
+    } catch (const cwx::Error& e) {
+      if (e.code == -1) return fail(CWX_ERR_IO, e.what());
+      throw;
+    }
+    *out = sol.release();
+    return CWX_OK;
+  });
+}
+
 int cwx_solution_free(cwx_solution* sol) {
   return guard([&]() -> int {
     delete sol;

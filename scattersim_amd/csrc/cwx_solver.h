@@ -78,6 +78,11 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
 int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
                 int s1, int s2, size_t max_bytes, zc* d_x, std::vector<int>& info, std::string& first_msg);
 
+// Compact heterogeneous .cw (mode data instead of interior plane waves) and its reverse: a device
+// solution rebuilt from such a file (see cwx_solver.hip for the layout).
+void cw2_write_file(const Solution& sol, const std::string& path, hipStream_t s);
+void cw2_read_file(Context& ctx, const std::string& path, Solution& sol);
+
 // Work buffers of the separable field evaluation.
 struct SepWork {
   DevBuf<zc> C, X, Y;

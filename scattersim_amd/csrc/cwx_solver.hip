@@ -834,6 +834,82 @@ struct SetResult {
   Solution* full = nullptr;     // count == 1: keep every array and build the .cw records here
 };
 
+// The arrays a full solution keeps besides gam, Gd, Gc, beta and x (already in sol): the n^-2
+// coefficients, the order tables, the Q_check / Q_hat rows and the .cw records
+// (ScatterVolume.cpp:802-987).  is holds the one instance; NI its L x MF n^-2 coefficients.
+void finish_solution(Solution& sol, const VolumeOptions& o, const InstSet& is, const zc* NI, hipStream_t s) {
+  const InstArgs ia = is.args();
+  const int Ltot = is.Ltot;
+  const int M0 = is.st[0].M0, M1 = is.st[0].M1, MF = M0 * M1;
+  const int n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF;
+  const long nn = (long)n4 * n4;
+  DevBuf<zc>& Gd = sol.Gd;
+  DevBuf<zc>& Gc = sol.Gc;
+  DevBuf<zc>& beta = sol.beta;
+  DevBuf<zc>& gam = sol.gam;
+  DevBuf<zc>& x = sol.x;
+  sol.opt = o;
+  sol.st = is.st[0];
+  sol.n4 = n4;
+  sol.NIF.alloc((size_t)Ltot * MF);
+  CWX_HIP(hipMemcpyAsync(sol.NIF.p, NI, sizeof(zc) * Ltot * MF, hipMemcpyDeviceToDevice, s));
+  sol.tabs.alloc((size_t)7 * MF);
+  CWX_HIP(hipMemcpyAsync(sol.tabs.p, is.tabs.p, sizeof(zc) * 7 * MF, hipMemcpyDeviceToDevice, s));
+  if (!sol.opt.external) {
+    sol.Qch.alloc((size_t)Ltot * n3 * n4);
+    DevBuf<zc> Yz((size_t)Ltot * MF * n4), Cn((size_t)Ltot * MF * MF);
+    hipLaunchKernelGGL(k_q_rows, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, Gd.p, Gc.p, beta.p, sol.Qch.p,
+                       Yz.p);
+    if (sol.opt.npz) {
+      hipLaunchKernelGGL(k_zero_rows, dim3(grid_for((long)MF * n4), Ltot), dim3(256), 0, s, n2, MF, n3, n4,
+                         sol.Qch.p, (long)n3 * n4);
+    } else {
+      hipLaunchKernelGGL(k_cn, dim3(grid_for((long)MF * MF), Ltot), dim3(256), 0, s, M0, M1, sol.NIF.p, Cn.p);
+      zgemm_batched(OpN, OpN, MF, n4, MF, mk(1.0), Cn.p, MF, (long)MF * MF, Yz.p, MF, (long)MF * n4, mk(0.0),
+                    sol.Qch.p + n2, n3, (long)n3 * n4, Ltot, s);
+    }
+    CWX_HIP(hipStreamSynchronize(s));  // Yz and Cn are released on return
+  }
+
+  // ---- 7. .cw records ----
+  const Setup& st = sol.st;
+  const int L = Ltot, ZL = L + 1;
+  std::vector<Seg> segs;
+  long off = 0;
+  auto add = [&](long cnt, int kind, int layer, double zb) {
+    segs.push_back(Seg{off, cnt, kind, layer, 0, zb});
+    sol.seg_off.push_back(off);
+    sol.seg_cnt.push_back(cnt);
+    off += cnt;
+  };
+  sol.seg_off.clear();
+  sol.seg_cnt.clear();
+  const long inner = sol.opt.external ? 0 : (long)n2 * MF;
+  add(MF, 0, 0, 0.0);  // Pi
+  for (int l = 0; l < ZL; ++l) {
+    if (l == 0) {
+      add(MF, 1, 0, st.z[0]);
+      add(inner, 3, 0, st.z[0]);
+    } else if (l == ZL - 1) {
+      add(inner, 4, l - 1, st.z[l]);
+      add(MF, 2, 0, st.z[l]);
+    } else {
+      add(inner, 4, l - 1, st.z[l]);
+      add(inner, 3, l, st.z[l]);
+    }
+  }
+  sol.nrec = off;
+  sol.rec.alloc((size_t)off * 6);
+  DevBuf<Seg> d_segs;
+  upload(d_segs, segs, s);
+  long maxcnt = 1;
+  for (auto& g : segs) maxcnt = std::max(maxcnt, g.cnt);
+  hipLaunchKernelGGL(k_records, dim3(grid_for(maxcnt), (unsigned)segs.size()), dim3(256), 0, s, ia, x.p, gam.p,
+                     sol.opt.external ? nullptr : sol.Qch.p, d_segs.p, sol.rec.p);
+  CWX_HIP(hipGetLastError());
+  CWX_HIP(hipStreamSynchronize(s));  // d_segs is released on return
+  }
+
 void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
                int s1, int s2, SetResult& res) {
   hipStream_t s = ctx.stream;
@@ -1028,69 +1104,7 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
       zgemm_set(OpN, OpN, n4, 1, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
                 lv.batch, s);
   }
-  if (full) {
-    Solution& sol = *full;
-    sol.opt = opts[0];
-    sol.st = is.st[0];
-    sol.n4 = n4;
-    sol.NIF.alloc((size_t)Ltot * MF);
-    CWX_HIP(hipMemcpyAsync(sol.NIF.p, NI, sizeof(zc) * Ltot * MF, hipMemcpyDeviceToDevice, s));
-    sol.tabs.alloc((size_t)7 * MF);
-    CWX_HIP(hipMemcpyAsync(sol.tabs.p, is.tabs.p, sizeof(zc) * 7 * MF, hipMemcpyDeviceToDevice, s));
-    if (!sol.opt.external) {
-      sol.Qch.alloc((size_t)Ltot * n3 * n4);
-      DevBuf<zc> Yz((size_t)Ltot * MF * n4), Cn((size_t)Ltot * MF * MF);
-      hipLaunchKernelGGL(k_q_rows, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, Gd.p, Gc.p, beta.p, sol.Qch.p,
-                         Yz.p);
-      if (sol.opt.npz) {
-        hipLaunchKernelGGL(k_zero_rows, dim3(grid_for((long)MF * n4), Ltot), dim3(256), 0, s, n2, MF, n3, n4,
-                           sol.Qch.p, (long)n3 * n4);
-      } else {
-        hipLaunchKernelGGL(k_cn, dim3(grid_for((long)MF * MF), Ltot), dim3(256), 0, s, M0, M1, sol.NIF.p, Cn.p);
-        zgemm_batched(OpN, OpN, MF, n4, MF, mk(1.0), Cn.p, MF, (long)MF * MF, Yz.p, MF, (long)MF * n4, mk(0.0),
-                      sol.Qch.p + n2, n3, (long)n3 * n4, Ltot, s);
-      }
-      CWX_HIP(hipStreamSynchronize(s));  // Yz and Cn are released on return
-    }
-
-    // ---- 7. .cw records ----
-    const Setup& st = sol.st;
-    const int L = Ltot, ZL = L + 1;
-    std::vector<Seg> segs;
-    long off = 0;
-    auto add = [&](long cnt, int kind, int layer, double zb) {
-      segs.push_back(Seg{off, cnt, kind, layer, 0, zb});
-      sol.seg_off.push_back(off);
-      sol.seg_cnt.push_back(cnt);
-      off += cnt;
-    };
-    sol.seg_off.clear();
-    sol.seg_cnt.clear();
-    const long inner = sol.opt.external ? 0 : (long)n2 * MF;
-    add(MF, 0, 0, 0.0);  // Pi
-    for (int l = 0; l < ZL; ++l) {
-      if (l == 0) {
-        add(MF, 1, 0, st.z[0]);
-        add(inner, 3, 0, st.z[0]);
-      } else if (l == ZL - 1) {
-        add(inner, 4, l - 1, st.z[l]);
-        add(MF, 2, 0, st.z[l]);
-      } else {
-        add(inner, 4, l - 1, st.z[l]);
-        add(inner, 3, l, st.z[l]);
-      }
-    }
-    sol.nrec = off;
-    sol.rec.alloc((size_t)off * 6);
-    DevBuf<Seg> d_segs;
-    upload(d_segs, segs, s);
-    long maxcnt = 1;
-    for (auto& g : segs) maxcnt = std::max(maxcnt, g.cnt);
-    hipLaunchKernelGGL(k_records, dim3(grid_for(maxcnt), (unsigned)segs.size()), dim3(256), 0, s, ia, x.p, gam.p,
-                       sol.opt.external ? nullptr : sol.Qch.p, d_segs.p, sol.rec.p);
-    CWX_HIP(hipGetLastError());
-    CWX_HIP(hipStreamSynchronize(s));  // d_segs is released on return
-  }
+  if (full) finish_solution(*full, opts[0], is, NI, s);
   CWX_HIP(hipGetLastError());
 
   // ---- status per instance ----
@@ -1220,6 +1234,199 @@ void cw_write_file(const Solution& sol, const std::string& path, hipStream_t s) 
     put_list(2 + 2 * l);
   }
   CWX_CHECK((bool)f, "write to " + path + " failed");
+}
+
+
+// ------------------------------------------------------------------------------------------------
+// Compact heterogeneous .cw (docs/cw_file_format.pptx slide 1; reader ScatterView.py:68-178):
+// is_volume = 1, precision 8, the incident waves, every boundary's z with only the external lists
+// (reflected at the first boundary, transmitted at the last; the interior waves follow from the mode
+// data), then M0 M1 (int32), the sample size (3 doubles), the layer count (uint32) and per layer
+// beta (4MF), gamma (4MF), gg (the 4MF x 4MF mode matrix row by row, gg[r 4MF + m] = V(r, m): rows
+// Ex, Ey and the two transverse magnetic blocks, columns the interleaved down / up modes, as
+// cpuEvaluator.cpp:171-205 reads it) and ft (the n^-2 coefficients in the solver's storage order:
+// the (M1 x M0) matrix column by column, which ScatterVolume.cpp:872 and cpuEvaluator.cpp:225 index
+// alike; as the sample is transposed on load (FourierWave.h:288), ScatterView.py:174's row-major
+// reshape gives square layers back as the .npy file holds them).  After the
+// layers comes a trailer this library adds (the reference reader stops before it): the magic
+// "CWXTRL01", then lambda, direction[3], ex[2], ey[2], ez[2], n[2], kappa, z (doubles), the solve's
+// grid and remaining options (s1, s2, ncoef, coef[2], psf, external, npz, no_cutoff, eig_mode: int32
+// each) and the boundary solution x (6MF complex): what rebuilding the solution needs beyond the
+// reference's layout.
+// ------------------------------------------------------------------------------------------------
+static const char kCw2Magic[8] = {'C', 'W', 'X', 'T', 'R', 'L', '0', '1'};
+
+void cw2_write_file(const Solution& sol, const std::string& path, hipStream_t s) {
+  const Setup& st = sol.st;
+  const int L = st.L, ZL = L + 1, MF = st.MF, M0 = st.M0, M1 = st.M1, n4 = 4 * MF, n6 = 6 * MF;
+  CWX_CHECK(sol.Gd.p && sol.Gc.p && sol.beta.p && sol.gam.p && sol.NIF.p && sol.x.p, "solution has no device data");
+  const size_t nn = (size_t)n4 * n4;
+  std::vector<zc> rec((size_t)sol.nrec * 6), gam((size_t)L * n4), beta((size_t)L * n4), Gd(nn * L), Gc(nn * L),
+      nif((size_t)L * MF), x(n6);
+  if (sol.nrec) CWX_HIP(hipMemcpyAsync(rec.data(), sol.rec.p, sizeof(zc) * rec.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(gam.data(), sol.gam.p, sizeof(zc) * gam.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(beta.data(), sol.beta.p, sizeof(zc) * beta.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(Gd.data(), sol.Gd.p, sizeof(zc) * Gd.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(Gc.data(), sol.Gc.p, sizeof(zc) * Gc.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(nif.data(), sol.NIF.p, sizeof(zc) * nif.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(x.data(), sol.x.p, sizeof(zc) * x.size(), hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipStreamSynchronize(s));
+  std::ofstream f(path, std::ios::binary);
+  CWX_CHECK((bool)f, "cannot open " + path + " for writing");
+  auto put = [&](const void* p, size_t n) { f.write((const char*)p, (std::streamsize)n); };
+  auto put_u64 = [&](uint64_t v) { put(&v, 8); };
+  auto put_list = [&](int seg) {
+    const uint64_t cnt = (uint64_t)sol.seg_cnt[seg];
+    put_u64(cnt);
+    put(rec.data() + sol.seg_off[seg] * 6, cnt * 96);
+  };
+  const unsigned char is_volume = 1;
+  put(&is_volume, 1);
+  put_u64(8);
+  put_list(0);
+  put_u64((uint64_t)ZL);
+  for (int l = 0; l < ZL; ++l) {
+    put(&st.z[l], 8);
+    if (l == 0) put_list(1);
+    else put_u64(0);
+    if (l == ZL - 1) put_list(2 + 2 * l);
+    else put_u64(0);
+  }
+  const int32_t m[2] = {M0, M1};
+  put(m, 8);
+  put(sol.opt.size, 24);
+  const uint32_t nl = (uint32_t)L;
+  put(&nl, 4);
+  std::vector<zc> gg(nn);
+  for (int l = 0; l < L; ++l) {
+    put(beta.data() + (size_t)l * n4, sizeof(zc) * n4);
+    put(gam.data() + (size_t)l * n4, sizeof(zc) * n4);
+    // even modes are the columns of Gd, odd modes those of Gc (k_build_gdgc)
+    for (int mo = 0; mo < n4; ++mo) {
+      const zc* col = ((mo & 1) ? Gc.data() : Gd.data()) + nn * l + (size_t)mo * n4;
+      for (int r = 0; r < n4; ++r) gg[(size_t)r * n4 + mo] = col[r];
+    }
+    put(gg.data(), sizeof(zc) * nn);
+    put(nif.data() + (size_t)l * MF, sizeof(zc) * MF);
+  }
+  put(kCw2Magic, 8);
+  const VolumeOptions& o = sol.opt;
+  const double d[13] = {o.lambda, o.direction[0], o.direction[1], o.direction[2], o.ex[0], o.ex[1], o.ey[0],
+                        o.ey[1], o.ez[0], o.ez[1], o.n[0], o.n[1], o.kappa};
+  put(d, sizeof(d));
+  put(&o.z, 8);
+  const int32_t iv[10] = {st.s1, st.s2, o.ncoef, o.coef[0], o.coef[1], o.psf ? 1 : 0, o.external ? 1 : 0,
+                          o.npz ? 1 : 0, o.no_cutoff ? 1 : 0, o.eig_mode};
+  put(iv, sizeof(iv));
+  put(x.data(), sizeof(zc) * n6);
+  CWX_CHECK((bool)f, "write to " + path + " failed");
+}
+
+void cw2_read_file(Context& ctx, const std::string& path, Solution& sol) {
+  hipStream_t s = ctx.stream;
+  std::ifstream f(path, std::ios::binary);
+  CWX_CHECK((bool)f, "cannot open " + path);
+  auto get = [&](void* p, size_t n) {
+    f.read((char*)p, (std::streamsize)n);
+    CWX_CHECK((bool)f, path + ": truncated compact .cw file");
+  };
+  auto get_u64 = [&]() {
+    uint64_t v;
+    get(&v, 8);
+    return v;
+  };
+  auto skip_list = [&]() {
+    const uint64_t cnt = get_u64();
+    CWX_CHECK(cnt < (1ull << 40), path + ": implausible wave count");
+    f.seekg((std::streamoff)(cnt * 96), std::ios::cur);
+  };
+  unsigned char is_volume = 0;
+  get(&is_volume, 1);
+  CWX_CHECK(is_volume == 1, path + ": not a heterogeneous (is_volume) .cw file");
+  CWX_CHECK(get_u64() == 8, path + ": only double-precision .cw files are supported");
+  skip_list();
+  const uint64_t ZL = get_u64();
+  CWX_CHECK(ZL >= 2 && ZL <= 65536, path + ": implausible boundary count");
+  for (uint64_t l = 0; l < ZL; ++l) {
+    double z;
+    get(&z, 8);
+    skip_list();
+    skip_list();
+  }
+  int32_t m[2];
+  get(m, 8);
+  double size[3];
+  get(size, 24);
+  uint32_t L;
+  get(&L, 4);
+  CWX_CHECK(m[0] >= 1 && m[1] >= 1 && (long)m[0] * m[1] * 4 <= 2048, path + ": unsupported Fourier orders");
+  CWX_CHECK(L >= 1 && L + 1 == ZL, path + ": layer count does not match the boundaries");
+  const int M0 = m[0], M1 = m[1], MF = M0 * M1, n4 = 4 * MF, n6 = 6 * MF;
+  const size_t nn = (size_t)n4 * n4;
+  std::vector<zc> beta((size_t)L * n4), gam((size_t)L * n4), V(nn * L), nif((size_t)L * MF), gg(nn);
+  for (uint32_t l = 0; l < L; ++l) {
+    get(beta.data() + (size_t)l * n4, sizeof(zc) * n4);
+    get(gam.data() + (size_t)l * n4, sizeof(zc) * n4);
+    get(gg.data(), sizeof(zc) * nn);
+    for (int r = 0; r < n4; ++r)
+      for (int mo = 0; mo < n4; ++mo) V[nn * l + (size_t)mo * n4 + r] = gg[(size_t)r * n4 + mo];
+    get(nif.data() + (size_t)l * MF, sizeof(zc) * MF);
+  }
+  char magic[8];
+  f.read(magic, 8);
+  CWX_CHECK((bool)f && std::memcmp(magic, kCw2Magic, 8) == 0,
+            path + ": no solver trailer (the reference's layout holds neither the wavelength nor the incident "
+                   "field, so the solution cannot be rebuilt from it)");
+  double d[13];
+  get(d, sizeof(d));
+  VolumeOptions o;
+  o.lambda = d[0];
+  for (int i = 0; i < 3; ++i) o.direction[i] = d[1 + i];
+  o.ex[0] = d[4]; o.ex[1] = d[5]; o.ey[0] = d[6]; o.ey[1] = d[7]; o.ez[0] = d[8]; o.ez[1] = d[9];
+  o.n[0] = d[10]; o.n[1] = d[11]; o.kappa = d[12];
+  get(&o.z, 8);
+  int32_t iv[10];
+  get(iv, sizeof(iv));
+  const int s1 = iv[0], s2 = iv[1];
+  CWX_CHECK(s1 >= 1 && s2 >= 1 && iv[2] >= 0 && iv[2] <= 2, path + ": bad solver trailer");
+  o.ncoef = iv[2];
+  o.coef[0] = iv[3];
+  o.coef[1] = iv[4];
+  o.psf = iv[5] != 0;
+  o.external = iv[6] != 0;
+  o.npz = iv[7] != 0;
+  o.no_cutoff = iv[8] != 0;
+  o.eig_mode = iv[9];
+  for (int i = 0; i < 3; ++i) o.size[i] = size[i];
+  std::vector<zc> x(n6);
+  get(x.data(), sizeof(zc) * n6);
+  // the solve's own options and grid give back its setup (orders, tables, cut-off flag) exactly
+  InstSet is;
+  make_instances({o}, {(int)L}, s1, s2, is, s);
+  CWX_CHECK(is.st[0].M0 == M0 && is.st[0].M1 == M1, path + ": orders do not match the solver trailer");
+  sol.gam.alloc(gam.size());
+  sol.beta.alloc(beta.size());
+  sol.Gd.alloc(V.size());
+  sol.Gc.alloc(V.size());
+  sol.x.alloc(x.size());
+  DevBuf<zc> NI(nif.size()), dV(V.size());
+  std::vector<int> ident((size_t)L * n4);
+  for (size_t i = 0; i < ident.size(); ++i) ident[i] = (int)(i % n4);
+  DevBuf<int> src;
+  upload(src, ident, s);
+  CWX_HIP(hipMemcpyAsync(sol.gam.p, gam.data(), sizeof(zc) * gam.size(), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(sol.beta.p, beta.data(), sizeof(zc) * beta.size(), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(dV.p, V.data(), sizeof(zc) * V.size(), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(sol.x.p, x.data(), sizeof(zc) * x.size(), hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(NI.p, nif.data(), sizeof(zc) * nif.size(), hipMemcpyHostToDevice, s));
+  // Gd / Gc: the mode matrix with the layer phases, by the solve's own kernel (the modes are sorted)
+  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for((long)nn), L), dim3(256), 0, s, is.args(), 0, dV.p, nullptr,
+                     sol.gam.p, src.p, is.dz.p, sol.Gd.p, sol.Gc.p);
+  CWX_HIP(hipGetLastError());
+  finish_solution(sol, o, is, NI.p, s);
+  CWX_HIP(hipStreamSynchronize(s));
+  sol.info = 0;
+  sol.info_msg.clear();
 }
 
 }  // namespace cwx
