@@ -3,7 +3,7 @@
     python -m scattersim_amd.build            # builds scattersim_amd/lib/libcwx.so and bin/
 
 Objects go to scattersim_amd/_build/, the shared library to scattersim_amd/lib/ and the tools
-(scattervolume, scatterthinvolume, scatterview) to scattersim_amd/bin/, all in-tree.
+(scattervolume, scatterthinvolume, scatterview, scatterviewsample) to scattersim_amd/bin/, all in-tree.
 """
 from __future__ import annotations
 
@@ -57,8 +57,9 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     host = ["g++", "-O2", "-std=c++17", f"-I{CSRC}", f"-L{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib"]
-    tools = [("cli_scattervolume.cpp", "scattervolume"), ("cli_scatterview.cpp", "scatterview")]
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+    tools = [("cli_scattervolume.cpp", "scattervolume"), ("cli_scatterview.cpp", "scatterview"),
+             ("cli_scatterviewsample.cpp", "scatterviewsample")]
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
         list(ex.map(lambda t: _run([*host, os.path.join(CSRC, t[0]), "-o", os.path.join(BINDIR, t[1]), "-lcwx"],
                                    verbose), tools))
     # scatterthinvolume is the same solver on a 2-D sample (readme.md:4,22)

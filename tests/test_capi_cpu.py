@@ -107,6 +107,12 @@ def test_cli_help_and_errors():
     assert r.returncode == 1 and "--nogui" in r.stderr
     r = run("scatterview", "--nogui", "--input", "/nonexistent.cw")
     assert r.returncode == 1 and "not found" in r.stdout
+    r = run("scatterviewsample", "--help")
+    assert r.returncode == 1 and "--extent" in r.stdout and "--nogui" in r.stdout
+    r = run("scatterviewsample", "c.cw")
+    assert r.returncode == 1 and "--nogui" in r.stderr
+    r = run("scatterviewsample", "--nogui", "--input", "/nonexistent.cw")
+    assert r.returncode == 1 and "not found" in r.stdout
 
 
 def test_cli_negative_values_parse(tmp_path):

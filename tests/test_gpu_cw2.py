@@ -245,7 +245,8 @@ def test_compact_file_without_trailer_is_refused(ctx):
 
 
 def test_cli_compact_file_renders_like_full_file(tmp_path):
-    """scattervolume --compact, then scatterview --nogui on it, against the same slice of the full file."""
+    """scattervolume --compact, then scatterview / scatterviewsample --nogui on it, against the same slices of
+    the full file (ScatterView.cpp:354-376, ScatterViewSample.cpp:354-376 slice geometry)."""
     import subprocess
     bin_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "scattersim_amd", "bin")
 
@@ -272,3 +273,19 @@ def test_cli_compact_file_renders_like_full_file(tmp_path):
     assert a.shape == b.shape == (32, 32, 3)
     # the full file is evaluated from FP32 waves (cw_unpack), the compact one from the FP64 records
     assert rel_err(a, b) < 1e-4
+    # scatterviewsample: the slices of all three axes from the compact file through the separable evaluator,
+    # and from the full file through the wave table, against scatterview on the full file
+    for axis in (0, 1, 2):
+        geo = ["--nogui", "--resolution", 5, "--axis", axis, "--slice", 0.21, "--center", 0.3, 0.2, 0.017]
+        r = run("scatterview", *geo, "--size", 4, "--input", tmp_path / "full.cw", "--output", tmp_path / "v.npy")
+        assert r.returncode == 1, r.stderr
+        r = run("scatterviewsample", *geo, "--extent", 4, "--input", tmp_path / "compact.cw", "--output",
+                tmp_path / "sc.npy")
+        assert r.returncode == 1 and "Visualization working." in r.stdout, r.stderr
+        r = run("scatterviewsample", *geo, "--extent", 4, "--input", tmp_path / "full.cw", "--output",
+                tmp_path / "sf.npy")
+        assert r.returncode == 1, r.stderr
+        v, sc, sf = (np.load(tmp_path / n) for n in ("v.npy", "sc.npy", "sf.npy"))
+        assert sc.shape == (32, 32, 3) and sc.dtype == np.complex64
+        assert np.array_equal(sf, v)
+        assert rel_err(sc, v) < 1e-4, (axis, rel_err(sc, v))
