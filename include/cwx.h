@@ -198,6 +198,44 @@ int cwx_solution_free(cwx_solution* sol);
 int cwx_cw_read_unpacked(const char* path, int64_t* nwaves, int* layers, cwx_wave_f64* W,
                          double* z_layers, int* waves_begin, int* waves_end);
 
+/* ---- homogeneous layer stacks (ScatterLayer.cpp:110-420) -------------------------------------- */
+/* The per-incident-wave loop of scatterlayer (ScatterLayer.cpp:390-420) for P incident waves at once:
+ * for each wave the 6L x 6L system of ScatterLayer.cpp:152-232 (incident amplitude in layer 0, no
+ * reflected wave in layer L-1, Gauss's law per wave, Ex Ey Hx Hy continuity per interface) is
+ * solved and turned into the reflected wave of layer l and the transmitted wave of layer l+1 at
+ * boundary l, wound to z[l] (mat2waves, ScatterLayer.cpp:240-266).  n: L real indices, kappa: L-1
+ * absorbances of layers 1..L-1, z: L-1 increasing boundaries (the expansion of the command-line
+ * lists, ScatterLayer.cpp:327-344, is the caller's).  incident: P records with real k;
+ * reflected / transmitted: P x (L-1) records, wave-major.  A positive return counts incident waves
+ * whose system had an exactly zero pivot. */
+int cwx_layer_solve(cwx_context* ctx, int L, const double* n, const double* kappa, const double* z, double lambda,
+                    int64_t P, const cwx_wave_f64* incident, cwx_wave_f64* reflected, cwx_wave_f64* transmitted);
+/* The same with incident / reflected / transmitted in device memory (n, kappa, z stay on the host). */
+int cwx_layer_solve_dev(cwx_context* ctx, int L, const double* n, const double* kappa, const double* z,
+                        double lambda, int64_t P, const cwx_wave_f64* d_incident, cwx_wave_f64* d_reflected,
+                        cwx_wave_f64* d_transmitted);
+/* CoupledWaveStructure::save (CoupledWaveStructure.h:63-92) of a layer solve as scatterlayer writes it
+ * (ScatterLayer.cpp:390-424): all P incident waves, then per boundary z[l] its P reflected and P
+ * transmitted waves; 96-byte records behind an is_volume byte of 0, as cwx_solution_write_cw. */
+int cwx_layer_write_cw(const char* path, int L, const double* z, int64_t P, const cwx_wave_f64* incident,
+                       const cwx_wave_f64* reflected, const cwx_wave_f64* transmitted);
+
+/* ---- focused-beam incidence (host only) ------------------------------------------------------ */
+/* tira::planewave<double>::SolidAnglePolar, an external dependency of the reference called at
+ * ScatterLayer.cpp:383-384 and ScatterPlane.cpp:240-241: the cone of half-angle alpha (obscured below
+ * beta) about k, sampled at theta_i = beta + i (alpha - beta) / N0 (i < N0) and phi_j = 2 pi j / N1
+ * (j < N1, the outer index: out[j N0 + i]).  Each wave has |k| of the axis, amplitude weight
+ * sin(theta) dtheta dphi on E's x and y components, and Ez from k . E0 = 0.  For an axis along +z
+ * and beta = 0 this reproduces the incident list of x64/a.cw; a tilted axis rotates the cone about
+ * z x k (Rodrigues), which no reference output covers. */
+int cwx_beam_polar(double alpha, double beta, const double k[3], const cwx_complex E[3], int N0, int N1,
+                   cwx_wave_f64* out);
+/* tira::planewave<double>::SolidAngleMC (ScatterLayer.cpp:381-382, ScatterPlane.cpp:238-239): N
+ * directions uniform in solid angle over the same cone (std::mt19937_64 seeded with seed), each with
+ * weight (cap solid angle) / N.  The reference's random sequence is tira's and is not reproduced. */
+int cwx_beam_montecarlo(double alpha, double beta, const double k[3], const cwx_complex E[3], int64_t N,
+                        uint64_t seed, cwx_wave_f64* out);
+
 /* ---- field evaluation (gpuEvaluator.h:14-17) ------------------------------------------------- */
 /* gpu_initialize (gpuEvaluator.cu:15-26): upload an unpacked FP32 wave table and region map. */
 int cwx_field_init_f32(cwx_context* ctx, const cwx_wave_f32* W, int64_t nwaves, const float* z_layers,

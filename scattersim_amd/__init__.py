@@ -22,7 +22,8 @@ from . import _capi
 from ._capi import check, lib
 
 __all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
-           "cw_read_unpacked", "read_cw2", "WAVE_F32", "WAVE_F64"]
+           "cw_read_unpacked", "read_cw2", "scatterlayer", "layer_stack", "layer_write_cw", "beam_polar",
+           "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
 WAVE_F64 = np.dtype([("E0", np.complex128, 3), ("k", np.complex128, 3)])
@@ -276,6 +277,68 @@ def scattervolume(ctx: Context, sample: np.ndarray, **opts) -> Solution:
             lib().cwx_solution_free(h)
         raise _capi.CwxError(st, msg)
     return Solution(ctx, h)
+
+
+def layer_stack(n=(1.0, 1.4, 1.4, 1.0), kappa=(0.05,), z=(-5.0, 0.0, 5.0)):
+    """The stack of scatterlayer's --n / --kappa / --z lists (ScatterLayer.cpp:327-344): L is the longest
+    of len(n), len(kappa) + 1 and len(z) + 1; n is padded with its last value, kappa with zeros and z in
+    steps of 10.  Returns (L, n[L], kappa[L-1], z[L-1])."""
+    n, kappa, z = [float(v) for v in n], [float(v) for v in kappa], [float(v) for v in z]
+    L = max(len(n), len(kappa) + 1, len(z) + 1)
+    n += [n[-1]] * (L - len(n))
+    kappa = (kappa + [0.0] * L)[:L - 1]
+    while len(z) + 1 < L:
+        z.append(z[-1] + 10.0)
+    return L, np.array(n), np.array(kappa), np.array(z[:L - 1])
+
+
+def _waves(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if a.ndim != 2 or a.shape[1] != 6:
+        raise ValueError("plane waves are (P, 6) records: E0.x, E0.y, E0.z, k.x, k.y, k.z")
+    return a
+
+
+def scatterlayer(ctx: Context, incident, n=(1.0, 1.4, 1.4, 1.0), kappa=(0.05,), z=(-5.0, 0.0, 5.0), lam=1.0):
+    """Solve a stack of homogeneous layers for every incident wave (cwx_layer_solve).
+
+    ``incident`` is (P, 6) complex records or a device pointer (int) with ``P`` given as ``(ptr, P)``.
+    Returns (reflected, transmitted), each (P, L-1, 6): the reflected wave of layer l and the transmitted
+    wave of layer l+1 at boundary l, wound to z[l] (ScatterLayer.cpp:390-420)."""
+    L, nn, kk, zz = layer_stack(n, kappa, z)
+    a = _waves(incident)
+    P = a.shape[0]
+    R = np.zeros((P, L - 1, 6), np.complex128)
+    T = np.zeros((P, L - 1, 6), np.complex128)
+    check(lib().cwx_layer_solve(ctx.h, L, _ptr(nn), _ptr(kk), _ptr(zz), float(lam), P, _ptr(a), _ptr(R), _ptr(T)))
+    return R, T
+
+
+def layer_write_cw(path: str, z, incident, reflected, transmitted):
+    """The .cw file scatterlayer writes (cwx_layer_write_cw)."""
+    z = np.ascontiguousarray(z, np.float64)
+    a = _waves(incident)
+    R = np.ascontiguousarray(reflected, np.complex128)
+    T = np.ascontiguousarray(transmitted, np.complex128)
+    check(lib().cwx_layer_write_cw(path.encode(), len(z) + 1, _ptr(z), a.shape[0], _ptr(a), _ptr(R), _ptr(T)))
+
+
+def beam_polar(alpha: float, k, E, N0: int, N1: int, beta: float = 0.0) -> np.ndarray:
+    """Focused-beam sampling of tira's SolidAnglePolar (cwx_beam_polar): (N1*N0, 6) records."""
+    kv = np.ascontiguousarray(k, np.float64)
+    Ev = np.ascontiguousarray(E, np.complex128)
+    out = np.zeros((N0 * N1, 6), np.complex128)
+    check(lib().cwx_beam_polar(float(alpha), float(beta), _ptr(kv), _ptr(Ev), int(N0), int(N1), _ptr(out)))
+    return out
+
+
+def beam_montecarlo(alpha: float, k, E, N: int, beta: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Monte-Carlo focused-beam sampling (cwx_beam_montecarlo): (N, 6) records."""
+    kv = np.ascontiguousarray(k, np.float64)
+    Ev = np.ascontiguousarray(E, np.complex128)
+    out = np.zeros((N, 6), np.complex128)
+    check(lib().cwx_beam_montecarlo(float(alpha), float(beta), _ptr(kv), _ptr(Ev), int(N), int(seed), _ptr(out)))
+    return out
 
 
 def batch_orders(params, Ls, s1: int, s2: int):

@@ -85,6 +85,11 @@ SIGNATURES = {
     "cwx_solution_write_cw": (I, [P, P, C.c_char_p]),
     "cwx_solution_write_cw2": (I, [P, P, C.c_char_p]),
     "cwx_solution_read_cw2": (I, [P, C.c_char_p, P]),
+    "cwx_layer_solve": (I, [P, I, P, P, P, C.c_double, C.c_int64, P, P, P]),
+    "cwx_layer_solve_dev": (I, [P, I, P, P, P, C.c_double, C.c_int64, P, P, P]),
+    "cwx_layer_write_cw": (I, [C.c_char_p, I, P, C.c_int64, P, P, P]),
+    "cwx_beam_polar": (I, [C.c_double, C.c_double, P, P, I, I, P]),
+    "cwx_beam_montecarlo": (I, [C.c_double, C.c_double, P, P, C.c_int64, C.c_uint64, P]),
     "cwx_solution_free": (I, [P]),
     "cwx_cw_read_unpacked": (I, [C.c_char_p, C.POINTER(I64), C.POINTER(I), P, P, P, P]),
     "cwx_field_init_f32": (I, [P, P, I64, P, I, P, P]),

@@ -3,7 +3,8 @@
     python -m scattersim_amd.build            # builds scattersim_amd/lib/libcwx.so and bin/
 
 Objects go to scattersim_amd/_build/, the shared library to scattersim_amd/lib/ and the tools
-(scattervolume, scatterthinvolume, scatterview, scatterviewsample) to scattersim_amd/bin/, all in-tree.
+(scattervolume, scatterthinvolume, scatterview, scatterviewsample,
+scatterlayer) to scattersim_amd/bin/, all in-tree.
 """
 from __future__ import annotations
 
@@ -21,7 +22,7 @@ BINDIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIBDIR, "libcwx.so")
 
 HIP_SOURCES = ["cwx_linalg.hip", "cwx_eig.hip", "cwx_field.hip", "cwx_fieldsep.hip", "cwx_solver.hip",
-               "cwx_capi.hip"]
+               "cwx_layer.hip", "cwx_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -58,8 +59,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     os.replace(tmp, LIB)
     host = ["g++", "-O2", "-std=c++17", f"-I{CSRC}", f"-L{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib"]
     tools = [("cli_scattervolume.cpp", "scattervolume"), ("cli_scatterview.cpp", "scatterview"),
-             ("cli_scatterviewsample.cpp", "scatterviewsample")]
-    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+             ("cli_scatterviewsample.cpp", "scatterviewsample"), ("cli_scatterlayer.cpp", "scatterlayer")]
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
         list(ex.map(lambda t: _run([*host, os.path.join(CSRC, t[0]), "-o", os.path.join(BINDIR, t[1]), "-lcwx"],
                                    verbose), tools))
     # scatterthinvolume is the same solver on a 2-D sample (readme.md:4,22)

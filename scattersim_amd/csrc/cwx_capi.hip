@@ -1,12 +1,16 @@
 // extern "C" entry points of libcwx (declared in include/cwx.h).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "../../include/cwx.h"
+#include "cwx_layer.h"
 #include "cwx_solver.h"
 
 struct cwx_context {
@@ -572,6 +576,177 @@ int cwx_solution_read_cw2(cwx_context* ctx, const char* path, cwx_solution** out
 int cwx_solution_free(cwx_solution* sol) {
   return guard([&]() -> int {
     delete sol;
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// the stack as the device kernels take it: L complex indices (the first real) and L - 1 boundaries
+void layer_tables(int L, const double* n, const double* kappa, const double* z, cwx::DevBuf<zc>& ri,
+                  cwx::DevBuf<double>& dz, hipStream_t s) {
+  CWX_CHECK(L >= 2 && n && z && (L == 1 || kappa), "a stack needs n[L], kappa[L-1] and z[L-1] with L >= 2");
+  std::vector<zc> h(L);
+  h[0] = cwx::mk(n[0], 0.0);
+  for (int l = 1; l < L; ++l) h[l] = cwx::mk(n[l], kappa[l - 1]);
+  for (int l = 1; l + 1 < L; ++l)
+    CWX_CHECK(z[l] >= z[l - 1], "layer boundaries must be in increasing z");
+  ri.alloc(L);
+  dz.alloc(L - 1);
+  CWX_HIP(hipMemcpyAsync(ri.p, h.data(), sizeof(zc) * L, hipMemcpyHostToDevice, s));
+  CWX_HIP(hipMemcpyAsync(dz.p, z, sizeof(double) * (L - 1), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+int cwx_layer_solve_dev(cwx_context* ctx, int L, const double* n, const double* kappa, const double* z,
+                        double lambda, int64_t P, const cwx_wave_f64* d_incident, cwx_wave_f64* d_reflected,
+                        cwx_wave_f64* d_transmitted) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && (P == 0 || (d_incident && d_reflected && d_transmitted)), "NULL argument");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    cwx::DevBuf<zc> ri;
+    cwx::DevBuf<double> dz;
+    layer_tables(L, n, kappa, z, ri, dz, s);
+    const int ns = cwx::layer_solve_dev(L, ri.p, dz.p, lambda, P, reinterpret_cast<const zc*>(d_incident),
+                                        reinterpret_cast<zc*>(d_reflected), reinterpret_cast<zc*>(d_transmitted), s);
+    if (ns) return fail(ns, std::to_string(ns) + " incident wave(s) gave an exactly singular system");
+    return CWX_OK;
+  });
+}
+
+int cwx_layer_solve(cwx_context* ctx, int L, const double* n, const double* kappa, const double* z, double lambda,
+                    int64_t P, const cwx_wave_f64* incident, cwx_wave_f64* reflected, cwx_wave_f64* transmitted) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && (P == 0 || (incident && reflected && transmitted)), "NULL argument");
+    CWX_CHECK(L >= 2, "a stack needs at least two layers");
+    set_device(ctx);
+    hipStream_t s = ctx->c.stream;
+    const size_t nout = (size_t)P * (L - 1);
+    cwx::DevBuf<zc> di((size_t)P * 6), dr(nout * 6), dt(nout * 6);
+    if (P) CWX_HIP(hipMemcpyAsync(di.p, incident, sizeof(cwx_wave_f64) * P, hipMemcpyHostToDevice, s));
+    const int st = cwx_layer_solve_dev(ctx, L, n, kappa, z, lambda, P, reinterpret_cast<cwx_wave_f64*>(di.p),
+                                       reinterpret_cast<cwx_wave_f64*>(dr.p), reinterpret_cast<cwx_wave_f64*>(dt.p));
+    if (st < 0) return st;
+    const std::string msg = g_err;
+    if (P) {
+      CWX_HIP(hipMemcpyAsync(reflected, dr.p, sizeof(cwx_wave_f64) * nout, hipMemcpyDeviceToHost, s));
+      CWX_HIP(hipMemcpyAsync(transmitted, dt.p, sizeof(cwx_wave_f64) * nout, hipMemcpyDeviceToHost, s));
+    }
+    CWX_HIP(hipStreamSynchronize(s));
+    if (st > 0) return fail(st, msg);
+    return CWX_OK;
+  });
+}
+
+int cwx_layer_write_cw(const char* path, int L, const double* z, int64_t P, const cwx_wave_f64* incident,
+                       const cwx_wave_f64* reflected, const cwx_wave_f64* transmitted) {
+  return guard([&]() -> int {
+    CWX_CHECK(path && z && (P == 0 || (incident && reflected && transmitted)) && L >= 2, "NULL argument");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return fail(CWX_ERR_IO, std::string("cannot open ") + path + " for writing");
+    auto put = [&](const void* p, size_t nb) { f.write(static_cast<const char*>(p), (std::streamsize)nb); };
+    auto put_u64 = [&](uint64_t v) { put(&v, 8); };
+    const unsigned char is_volume = 0;
+    put(&is_volume, 1);
+    put_u64(8);
+    put_u64((uint64_t)P);
+    put(incident, sizeof(cwx_wave_f64) * (size_t)P);
+    put_u64((uint64_t)(L - 1));
+    for (int l = 0; l < L - 1; ++l) {
+      put(&z[l], 8);
+      put_u64((uint64_t)P);
+      for (int64_t p = 0; p < P; ++p) put(&reflected[(size_t)p * (L - 1) + l], sizeof(cwx_wave_f64));
+      put_u64((uint64_t)P);
+      for (int64_t p = 0; p < P; ++p) put(&transmitted[(size_t)p * (L - 1) + l], sizeof(cwx_wave_f64));
+    }
+    if (!f) return fail(CWX_ERR_IO, std::string("write to ") + path + " failed");
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// unit vector d and the rotation that takes +z to it (Rodrigues), applied to v
+void rotate_from_z(const double d[3], const double v[3], double out[3]) {
+  const double c = d[2];
+  if (c > 1.0 - 1e-15) {
+    for (int i = 0; i < 3; ++i) out[i] = v[i];
+    return;
+  }
+  if (c < -1.0 + 1e-15) {  // half turn about x
+    out[0] = v[0];
+    out[1] = -v[1];
+    out[2] = -v[2];
+    return;
+  }
+  // axis a = z x d / |z x d|, angle acos(c)
+  const double ax = -d[1], ay = d[0], sn = std::sqrt(ax * ax + ay * ay);
+  const double ux = ax / sn, uy = ay / sn;  // uz = 0
+  const double dot = ux * v[0] + uy * v[1];
+  const double cr[3] = {uy * v[2], -ux * v[2], ux * v[1] - uy * v[0]};
+  for (int i = 0; i < 3; ++i) {
+    const double ui = i == 0 ? ux : (i == 1 ? uy : 0.0);
+    out[i] = v[i] * c + cr[i] * sn + ui * dot * (1.0 - c);
+  }
+}
+
+// one sample of a focused beam: direction (theta, phi) about the beam axis, amplitude weight w
+void beam_wave(const double k[3], double kmag, const cwx_complex E[3], double theta, double phi, double w,
+               cwx_wave_f64& out) {
+  const double local[3] = {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+  const double axis[3] = {k[0] / kmag, k[1] / kmag, k[2] / kmag};
+  double d[3];
+  rotate_from_z(axis, local, d);
+  const double kk[3] = {d[0] * kmag, d[1] * kmag, d[2] * kmag};
+  const cwx_complex ex{w * E[0].re, w * E[0].im}, ey{w * E[1].re, w * E[1].im};
+  cwx_complex ez{w * E[2].re, w * E[2].im};
+  if (kk[2] != 0.0)  // transverse: Ez from k . E0 = 0
+    ez = cwx_complex{-(kk[0] * ex.re + kk[1] * ey.re) / kk[2], -(kk[0] * ex.im + kk[1] * ey.im) / kk[2]};
+  out.E0[0] = ex;
+  out.E0[1] = ey;
+  out.E0[2] = ez;
+  for (int c = 0; c < 3; ++c) out.k[c] = cwx_complex{kk[c], 0.0};
+}
+
+}  // namespace
+
+int cwx_beam_polar(double alpha, double beta, const double k[3], const cwx_complex E[3], int N0, int N1,
+                   cwx_wave_f64* out) {
+  return guard([&]() -> int {
+    CWX_CHECK(k && E && out, "NULL argument");
+    CWX_CHECK(N0 >= 1 && N1 >= 1, "the beam needs at least one sample per angle");
+    CWX_CHECK(alpha > beta && beta >= 0.0, "need 0 <= beta < alpha");
+    const double kmag = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    CWX_CHECK(kmag > 0.0, "zero wave vector");
+    const double dth = (alpha - beta) / N0, dph = 2.0 * M_PI / N1;
+    for (int j = 0; j < N1; ++j)
+      for (int i = 0; i < N0; ++i) {
+        const double th = beta + i * dth, ph = j * dph;
+        beam_wave(k, kmag, E, th, ph, std::sin(th) * dth * dph, out[(size_t)j * N0 + i]);
+      }
+    return CWX_OK;
+  });
+}
+
+int cwx_beam_montecarlo(double alpha, double beta, const double k[3], const cwx_complex E[3], int64_t N,
+                        uint64_t seed, cwx_wave_f64* out) {
+  return guard([&]() -> int {
+    CWX_CHECK(k && E && (N == 0 || out), "NULL argument");
+    CWX_CHECK(N >= 0 && alpha > beta && beta >= 0.0, "need N >= 0 and 0 <= beta < alpha");
+    const double kmag = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    CWX_CHECK(kmag > 0.0, "zero wave vector");
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    const double ca = std::cos(alpha), cb = std::cos(beta);
+    const double w = 2.0 * M_PI * (cb - ca) / (double)std::max<int64_t>(N, 1);  // cap solid angle / N
+    for (int64_t i = 0; i < N; ++i) {
+      const double ct = cb - u(rng) * (cb - ca), ph = 2.0 * M_PI * u(rng);
+      beam_wave(k, kmag, E, std::acos(ct), ph, w, out[i]);
+    }
     return CWX_OK;
   });
 }

@@ -102,9 +102,11 @@ inline Array load(const std::string& path) {
   return a;
 }
 
-// Saves complex<float> data of the given shape in C order.
-inline void save_c64(const std::string& path, const std::vector<size_t>& shape, const void* data) {
-  std::string dict = "{'descr': '<c8', 'fortran_order': False, 'shape': (";
+// Saves data of the given shape in C order; descr names the element type ('<c8' complex<float>,
+// '<c16' complex<double>), esize its size in bytes.
+inline void save(const std::string& path, const std::vector<size_t>& shape, const void* data, const char* descr,
+                 size_t esize) {
+  std::string dict = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': (";
   size_t n = 1;
   for (size_t i = 0; i < shape.size(); ++i) {
     dict += std::to_string(shape[i]);
@@ -121,7 +123,13 @@ inline void save_c64(const std::string& path, const std::vector<size_t>& shape, 
   uint16_t hl = (uint16_t)dict.size();
   f.write((const char*)&hl, 2);
   f.write(dict.data(), dict.size());
-  f.write((const char*)data, 8 * n);
+  f.write((const char*)data, esize * n);
+}
+inline void save_c64(const std::string& path, const std::vector<size_t>& shape, const void* data) {
+  save(path, shape, data, "<c8", 8);
+}
+inline void save_c128(const std::string& path, const std::vector<size_t>& shape, const void* data) {
+  save(path, shape, data, "<c16", 16);
 }
 
 }  // namespace cwx_npy

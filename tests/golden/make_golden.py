@@ -6,6 +6,9 @@
                  n = 1 | 1.4+0.05i | 1.4 | 1, boundaries z = -3, 0, 3, lambda = 1): for each incident
                  wave its record and the reflected/transmitted record at every boundary, as
                  (E0.x, E0.y, E0.z = 0, k.x, k.y, k.z) complex128 rows.
+* acw_beam.npz -- all 4096 incident waves of x64/a.cw (the focused beam of scatterlayer's defaults,
+                 alpha = 1, 64 x 64 polar samples about +z) as real (E0.x, E0.y, k.x, k.y, k.z) rows;
+                 every imaginary part in the file is zero.
 * samples.npz -- the input grids data/layers.npy, data/circle.npy and 3_npy.npy.
 The reference's records hold only (E0.x, E0.y, k); for ky = 0 TE waves E0.y is the global Ey.
 """
@@ -52,6 +55,9 @@ def main(ref):
     te = np.nonzero((np.abs(Pi[:, 4]) == 0) & (np.abs(Pi[:, 1]) > 0))[0]
     np.savez_compressed(os.path.join(here, "acw_te.npz"), index=te, z=z, Pi=Pi[te],
                         Pr=np.stack([p[te] for p in Pr], 1), Pt=np.stack([p[te] for p in Pt], 1))
+    raw = np.concatenate([Pi[:, 0:2], Pi[:, 3:6]], axis=1)
+    assert np.all(raw.imag == 0)
+    np.savez_compressed(os.path.join(here, "acw_beam.npz"), Pi=raw.real.copy())
     np.savez_compressed(os.path.join(here, "samples.npz"),
                         layers=np.load(os.path.join(ref, "data", "layers.npy")),
                         circle=np.load(os.path.join(ref, "data", "circle.npy")),

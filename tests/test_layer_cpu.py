@@ -1,0 +1,181 @@
+"""CPU checks of the scatterlayer path: the layer oracle against x64/a.cw and closed forms, the focused-beam
+sampler against a.cw's incident list, the stack expansion of the command line, and the tool's options.
+
+Pins of ``oracle/layer_oracle.py``:
+
+- the 63 TE incidences of x64/a.cw (the reference's own scatterlayer output): every reflected and
+  transmitted record at all three boundaries;
+- Fresnel's single-interface coefficients, TE (E ratio) and TM (H_y ratio), absorbing substrate;
+- the Airy transmission of a slab (TE);
+- flux balance of lossless stacks for mixed polarisations and n0 != 1 excluded (the reference's Gauss rows
+  use the unit direction, which matches the physics only for n0 = 1).
+
+The other 4033 incidences of a.cw are not solutions of ScatterLayer.cpp:152-232's system for any incident
+field (the fit leaves residuals of 5e-5 against amplitudes of 8e-4 while the system's condition number is
+about 6); that binary predates the source, so those records pin nothing.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import scattersim_amd as sa
+from oracle import layer_oracle as lo
+from tests.helpers import golden, rel_err
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "scattersim_amd", "bin")
+
+ACW_STACK = lo.LayerStack(n=(1.0, 1.4, 1.4, 1.0), kappa=(0.05,), z=(-3.0, 0.0, 3.0), lam=1.0)
+
+
+def drop_ez(rec):
+    return np.concatenate([rec[..., 0:2], rec[..., 3:6]], axis=-1)
+
+
+def test_oracle_reproduces_acw_te_incidences():
+    g = golden("acw_te.npz")
+    inc = g["Pi"].copy()
+    R, T = lo.solve(ACW_STACK, inc)
+    for l in range(3):
+        assert rel_err(drop_ez(R[:, l]), drop_ez(g["Pr"][:, l])) < 1e-13
+        assert rel_err(drop_ez(T[:, l]), drop_ez(g["Pt"][:, l])) < 1e-13
+
+
+def plane(theta, n0, lam, pol):
+    """Incident record of unit amplitude at angle theta in the x-z plane; pol 'TE' (E along y) or 'TM'."""
+    k = 2 * math.pi / lam * n0
+    kv = np.array([k * math.sin(theta), 0.0, k * math.cos(theta)])
+    E = np.array([0, 1, 0], complex) if pol == "TE" else np.array([math.cos(theta), 0, -math.sin(theta)], complex)
+    return np.array([[E[0], E[1], E[2], kv[0], kv[1], kv[2]]])
+
+
+def field_at(rec, z):
+    """E of a record at (0, 0, z) and H = k x E (non-magnetic, common scale)."""
+    E = rec[0:3] * np.exp(1j * rec[5] * z)
+    H = np.cross(rec[3:6], E)
+    return E, H
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 0.9])
+def test_oracle_fresnel_single_interface(theta):
+    n1 = 1.5 + 0.1j
+    st = lo.LayerStack(n=(1.0, n1.real), kappa=(n1.imag,), z=(0.4,), lam=0.8)
+    k = 2 * math.pi / 0.8
+    kz0 = k * math.cos(theta)
+    kz1 = k * np.sqrt(n1 ** 2 - math.sin(theta) ** 2)
+    zb = 0.4
+    # TE: E_y ratios
+    inc = plane(theta, 1.0, 0.8, "TE")
+    R, T = lo.solve(st, inc)
+    Ei, _ = field_at(inc[0], zb)
+    Er, _ = field_at(R[0, 0], zb)
+    Et, _ = field_at(T[0, 0], zb)
+    assert abs(Er[1] / Ei[1] - (kz0 - kz1) / (kz0 + kz1)) < 1e-13
+    assert abs(Et[1] / Ei[1] - 2 * kz0 / (kz0 + kz1)) < 1e-13
+    # TM: H_y ratios (eps1 = 1, eps2 = n1^2)
+    inc = plane(theta, 1.0, 0.8, "TM")
+    R, T = lo.solve(st, inc)
+    _, Hi = field_at(inc[0], zb)
+    _, Hr = field_at(R[0, 0], zb)
+    _, Ht = field_at(T[0, 0], zb)
+    e2 = n1 ** 2
+    assert abs(Hr[1] / Hi[1] - (e2 * kz0 - kz1) / (e2 * kz0 + kz1)) < 1e-13
+    assert abs(Ht[1] / Hi[1] - 2 * e2 * kz0 / (e2 * kz0 + kz1)) < 1e-13
+
+
+def test_oracle_slab_airy_te():
+    n1, n2, d, lam, theta = 1.7 + 0.02j, 1.3, 0.9, 0.6, 0.4
+    st = lo.LayerStack(n=(1.0, n1.real, n2), kappa=(n1.imag, 0.0), z=(-0.2, -0.2 + d), lam=lam)
+    k = 2 * math.pi / lam
+    s = math.sin(theta)
+    kz = [k * math.cos(theta), k * np.sqrt(n1 ** 2 - s * s), k * np.sqrt(n2 ** 2 - s * s)]
+    r = lambda a, b: (kz[a] - kz[b]) / (kz[a] + kz[b])
+    t = lambda a, b: 2 * kz[a] / (kz[a] + kz[b])
+    ph = np.exp(1j * kz[1] * d)
+    t_slab = t(0, 1) * t(1, 2) * ph / (1 + r(0, 1) * r(1, 2) * ph * ph)
+    r_slab = (r(0, 1) + r(1, 2) * ph * ph) / (1 + r(0, 1) * r(1, 2) * ph * ph)
+    inc = plane(theta, 1.0, lam, "TE")
+    R, T = lo.solve(st, inc)
+    Ei0, _ = field_at(inc[0], -0.2)
+    Er0, _ = field_at(R[0, 0], -0.2)
+    Et1, _ = field_at(T[0, 1], -0.2 + d)
+    assert abs(Er0[1] / Ei0[1] - r_slab) < 1e-12
+    assert abs(Et1[1] / Ei0[1] - t_slab) < 1e-12
+
+
+def sz_flux(rec):
+    """z component of the time-averaged Poynting vector of a record at its phase origin (real k)."""
+    E = rec[0:3]
+    H = np.cross(rec[3:6], E)
+    return 0.5 * np.real(np.cross(E, np.conj(H))[2])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oracle_flux_balance_lossless_stack(seed):
+    rng = np.random.default_rng(seed)
+    L = 5
+    n = (1.0,) + tuple(1.2 + rng.random(L - 1))
+    z = tuple(np.cumsum(0.2 + rng.random(L - 1)) - 1.0)
+    st = lo.LayerStack(n=n, kappa=(0.0,) * (L - 1), z=z, lam=0.7)
+    th, ph = 0.6 * rng.random(), 2 * math.pi * rng.random()
+    k = 2 * math.pi / 0.7
+    kv = k * np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    E = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    E -= kv * (kv @ E) / (kv @ kv)
+    inc = np.concatenate([E, kv]).reshape(1, 6)
+    R, T = lo.solve(st, inc)
+    Si = sz_flux(inc[0])
+    Sr = -sz_flux(R[0, 0])
+    St = sz_flux(T[0, -1])
+    assert Si > 0 and abs(Sr + St - Si) < 1e-12 * Si
+
+
+def test_library_beam_matches_acw_incident_list():
+    """cwx_beam_polar with scatterlayer's defaults reproduces every incident wave of x64/a.cw."""
+    Pi = golden("acw_beam.npz")["Pi"]
+    k = 2 * math.pi
+    out = sa.beam_polar(1.0, (0.0, 0.0, k), (0, 1, 0), 64, 64)
+    assert out.shape == (4096, 6)
+    mine = np.stack([out[:, 0].real, out[:, 1].real, out[:, 3].real, out[:, 4].real, out[:, 5].real], 1)
+    assert np.max(np.abs(mine - Pi)) < 4 * np.finfo(float).eps * k
+    assert np.all(out[:, 0:2].imag == 0) and np.all(out[:, 3:6].imag == 0)
+    # transversality of every sample
+    assert np.max(np.abs(np.sum(out[:, 0:3] * out[:, 3:6], 1))) < 1e-18
+
+
+def test_library_beam_tilted_and_montecarlo():
+    k = 2 * math.pi * 1.3
+    d = np.array([0.3, -0.2, 0.9])
+    d /= np.linalg.norm(d)
+    out = sa.beam_polar(0.5, k * d, (1, 0.5j, 0), 8, 12, beta=0.1)
+    kk = out[:, 3:6].real
+    assert np.allclose(np.linalg.norm(kk, axis=1), k)
+    ang = np.arccos(np.clip(kk @ d / k, -1, 1))
+    assert ang.min() > 0.1 - 1e-12 and ang.max() < 0.5
+    mc = sa.beam_montecarlo(0.5, k * d, (1, 0, 0), 1000, seed=3)
+    ang = np.arccos(np.clip(mc[:, 3:6].real @ d / k, -1, 1))
+    assert ang.max() <= 0.5 + 1e-12
+    # the weights add up to the cap's solid angle times |E|
+    assert abs(np.sum(mc[:, 0].real) - 2 * math.pi * (1 - math.cos(0.5))) < 1e-12
+    assert np.array_equal(mc, sa.beam_montecarlo(0.5, k * d, (1, 0, 0), 1000, seed=3))
+
+
+def test_layer_stack_expansion():
+    """ScatterLayer.cpp:327-344."""
+    L, n, kap, z = sa.layer_stack((1, 1.4), (0.05, 0.1, 0.0), (-1.0,))
+    assert L == 4 and list(n) == [1, 1.4, 1.4, 1.4] and list(kap) == [0.05, 0.1, 0.0]
+    assert list(z) == [-1.0, 9.0, 19.0]
+    L, n, kap, z = sa.layer_stack()
+    assert L == 4 and list(z) == [-5.0, 0.0, 5.0] and list(kap) == [0.05, 0.0, 0.0]
+    assert lo.expand(lo.LayerStack())[0] == 4
+
+
+def test_cli_scatterlayer_help_and_errors():
+    r = subprocess.run([os.path.join(BIN, "scatterlayer"), "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--samples" in r.stdout and "--output_sample" in r.stdout
+    r = subprocess.run([os.path.join(BIN, "scatterlayer"), "--mode", "grid"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "--mode" in r.stderr
