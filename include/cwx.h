@@ -139,6 +139,19 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
                         const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
                         int* M1);
 
+/* Many incident fields on one sample, sharing every layer's eigen-decomposition, the transfer chain
+ * and the boundary factorisation.  The reference builds one incident field, the Fourier
+ * coefficients EF of the plane wave (ScatterVolume.cpp:660-674; with --psf every order carries E),
+ * and solves A x = [0; -f1 EF] for it (ScatterVolume.cpp:739-740); here K fields are solved with one
+ * factorisation of A.  Field k is K x 3MF complex in d_EF (device): its Ex, Ey and Ez
+ * coefficients, MF each, in the order of CWX_SOL_EF (order (r, c) of the M1 x M0 grid at r + c M1;
+ * a single non-zero order is a plane wave of transverse wave vector (Sx, Sy) k of that order).
+ * The wavelength, direction and sample come from p (its ex/ey/ez are not used).  d_x (device,
+ * K x 6MF) receives [R; T] per field.  With d_x == NULL only *M0, *M1 are set (ctx may be NULL).
+ * No counterpart in the reference, which runs one process per incident field. */
+int cwx_solve_fields_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
+                         int K, const void* d_EF, void* d_x, int* M0, int* M1);
+
 /* Replaces volume::CalculateD / phi (FourierWave.h:303-397): the property matrix of every layer,
  * Phi = [[0, P], [Q, 0]] (4MF x 4MF, column-major, blocks = 0, L matrices) or its blocks P and Q
  * (2MF x 2MF each, blocks = 1: the L P blocks, then the L Q blocks) into device memory d_out.

@@ -470,13 +470,16 @@ def eig_layers(s: Setup):
     return gammas, vectors, Nifs, phis
 
 
-def solve(sample: np.ndarray, p: VolumeParams, eig=None) -> Solution:
+def solve(sample: np.ndarray, p: VolumeParams, eig=None, EF=None) -> Solution:
     """The scattervolume solve, ScatterVolume.cpp:574-889.
 
     ``eig`` may replace the per-layer eigen step: a callable (Setup) -> (gammas, vectors, Nifs);
-    the default is :func:`eig_layers`.
+    the default is :func:`eig_layers`.  ``EF`` (3MF) replaces the incident field's Fourier
+    coefficients of ScatterVolume.cpp:660-674.
     """
     s = setup(sample, p)
+    if EF is not None:
+        s.EF = np.asarray(EF, np.complex128).copy()
     MF, L, k, z = s.MF, s.L, s.k, s.z
     ZL = L + 1
     if eig is None:

@@ -22,7 +22,7 @@ from . import _capi
 from ._capi import check, lib
 
 __all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
-           "cw_read_unpacked", "read_cw2", "scatterlayer", "layer_stack", "layer_write_cw", "beam_polar",
+           "cw_read_unpacked", "read_cw2", "solve_fields", "scatterlayer", "layer_stack", "layer_write_cw", "beam_polar",
            "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
@@ -338,6 +338,44 @@ def beam_montecarlo(alpha: float, k, E, N: int, beta: float = 0.0, seed: int = 0
     Ev = np.ascontiguousarray(E, np.complex128)
     out = np.zeros((N, 6), np.complex128)
     check(lib().cwx_beam_montecarlo(float(alpha), float(beta), _ptr(kv), _ptr(Ev), int(N), int(seed), _ptr(out)))
+    return out
+
+
+def solve_fields(ctx: Context, sample, EF, **opts):
+    """Solve one sample for K incident fields at once (cwx_solve_fields_dev).
+
+    ``EF`` is (K, 3*MF) complex: per field the Ex, Ey, Ez Fourier coefficients of the incident orders, in
+    the order of ``Solution.setup_array(SOL_EF)``.  ``sample`` is an (L, s1, s2) array (copied to the
+    device) or a device pointer with ``shape=(L, s1, s2)``.  Returns (K, 6*MF) [R; T] per field.
+    ``EF=None`` returns the orders (M0, M1) instead."""
+    shape = opts.pop("shape", None)
+    p = _params(**opts)
+    if isinstance(sample, int):
+        L, s1, s2 = shape
+        ptr, keep = sample, None
+    else:
+        a = np.asarray(sample, dtype=np.complex128)
+        if a.ndim == 2:
+            a = a[None]
+        L, s1, s2 = a.shape
+        keep = DeviceArray.from_numpy(ctx, np.ascontiguousarray(a))
+        ptr = keep.ptr
+    M0, M1 = C.c_int(), C.c_int()
+    check(lib().cwx_solve_fields_dev(None, C.byref(p), None, L, s1, s2, 0, None, None, C.byref(M0), C.byref(M1)))
+    if EF is None:
+        return M0.value, M1.value
+    E = np.ascontiguousarray(EF, np.complex128)
+    if E.ndim == 1:
+        E = E[None]
+    if E.ndim != 2 or E.shape[1] != 3 * M0.value * M1.value:
+        raise ValueError(f"EF must be (K, 3*MF) with 3*MF = {3 * M0.value * M1.value}")
+    K = E.shape[0]
+    dE = DeviceArray.from_numpy(ctx, E)
+    dx = DeviceArray(ctx, (K, 2 * E.shape[1]), np.complex128)
+    check(lib().cwx_solve_fields_dev(ctx.h, C.byref(p), C.c_void_p(ptr), L, s1, s2, K, C.c_void_p(dE.ptr),
+                                     C.c_void_p(dx.ptr), None, None))
+    out = dx.numpy()
+    del keep
     return out
 
 

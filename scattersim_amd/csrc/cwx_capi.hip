@@ -416,6 +416,26 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
   });
 }
 
+int cwx_solve_fields_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
+                         int K, const void* d_EF, void* d_x, int* M0, int* M1) {
+  return guard([&]() -> int {
+    CWX_CHECK(p, "NULL argument");
+    CWX_CHECK(L >= 1 && s1 >= 1 && s2 >= 1, "sample dimensions must be positive");
+    const cwx::VolumeOptions o = to_options(p);
+    const cwx::Setup st = cwx::make_setup(o, L, s1, s2);
+    if (M0) *M0 = st.M0;
+    if (M1) *M1 = st.M1;
+    if (!d_x) return CWX_OK;
+    CWX_CHECK(ctx && d_sample && d_EF, "NULL argument");
+    CWX_CHECK(K >= 1, "K must be positive");
+    set_device(ctx);
+    std::string msg;
+    const int code = cwx::solve_fields(ctx->c, o, (const zc*)d_sample, L, s1, s2, K, (const zc*)d_EF, (zc*)d_x, msg);
+    if (code) return fail(code, msg);
+    return CWX_OK;
+  });
+}
+
 int cwx_phi_build_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
                       int blocks, void* d_out, int* n_out) {
   return guard([&]() -> int {

@@ -78,6 +78,12 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
 int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
                 int s1, int s2, size_t max_bytes, zc* d_x, std::vector<int>& info, std::string& first_msg);
 
+// K incident fields on one sample with one eigen-decomposition, one transfer chain and one boundary
+// factorisation: d_EF holds K x 3MF incident Fourier coefficients (component-major, order index
+// as Setup::EF), d_x receives K x 6MF boundary solutions [R; T].  Returns the solve's status.
+int solve_fields(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2, int K,
+                 const zc* d_EF, zc* d_x, std::string& msg);
+
 // Compact heterogeneous .cw (mode data instead of interior plane waves) and its reverse: a device
 // solution rebuilt from such a file (see cwx_solver.hip for the layout).
 void cw2_write_file(const Solution& sol, const std::string& path, hipStream_t s);

@@ -416,6 +416,10 @@ __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V,
   }
 }
 
+__global__ void k_fill(long n, zc* __restrict__ a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = mk(0.0);
+}
+
 __global__ void k_copy(long n, const zc* __restrict__ a, zc* __restrict__ b) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -832,6 +836,9 @@ struct SetResult {
   std::vector<int> info;        // per instance
   std::vector<std::string> msg;
   Solution* full = nullptr;     // count == 1: keep every array and build the .cw records here
+  int K = 0;                    // count == 1: further incident fields solved with the same factors
+  const zc* EFs = nullptr;      // K x 3MF incident Fourier coefficients
+  zc* xs = nullptr;             // K x 6MF boundary solutions
 };
 
 // The arrays a full solution keeps besides gam, Gd, Gc, beta and x (already in sol): the n^-2
@@ -1095,6 +1102,21 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
   hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6), count), dim3(256), 0, s, ia, Abr.p, A.p, x.p);
   zgetrf_batched(n6, A.p, n6, (long)n6 * n6, pivA, n6, infoA.p, count, s);
   zgetrs_batched(n6, 1, A.p, n6, (long)n6 * n6, pivA, n6, x.p, n6, n6, count, s);
+  if (res.K > 0) {
+    // b_k = [0; -f1 EF_k] for every further field, then the same factors (ScatterVolume.cpp:739-740 per field)
+    CWX_CHECK(count == 1, "several incident fields are solved for one instance at a time");
+    const int K = res.K;
+    DevBuf<zc> F1((size_t)n4 * n3);
+    hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), 1), dim3(256), 0, s, ia, 1, F1.p);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((long)n6 * K)), dim3(256), 0, s, (long)n6 * K, res.xs);
+    for (int k0 = 0; k0 < K; k0 += 65535) {
+      const int kb = std::min(65535, K - k0);
+      zgemm_batched(OpN, OpN, n4, kb, n3, mk(-1.0), F1.p, n4, 0, res.EFs + (long)k0 * n3, n3, 0, mk(0.0),
+                    res.xs + (long)k0 * n6 + n2, n6, 0, 1, s);
+      zgetrs_batched(n6, kb, A.p, n6, (long)n6 * n6, pivA, n6, res.xs + (long)k0 * n6, n6, 0, 1, s);
+    }
+    CWX_HIP(hipStreamSynchronize(s));  // F1 is released on return
+  }
 
   // ---- 6. beta recurrence, Q_check / Q_hat, Ez rows ----
   if (beta.p) {
@@ -1201,6 +1223,21 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
     i0 = i1;
   }
   return worst;
+}
+
+int solve_fields(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2, int K,
+                 const zc* d_EF, zc* d_x, std::string& msg) {
+  CWX_CHECK(L >= 1 && L <= 65535, "bad layer count");
+  CWX_CHECK(K >= 1 && d_EF && d_x, "need at least one incident field");
+  DevBuf<zc> x0;
+  SetResult res;
+  res.x = &x0;
+  res.K = K;
+  res.EFs = d_EF;
+  res.xs = d_x;
+  solve_set(ctx, {o}, d_sample, {L}, s1, s2, res);
+  msg = res.msg[0];
+  return res.info[0];
 }
 
 // ------------------------------------------------------------------------------------------------
