@@ -205,6 +205,108 @@ def batched_leg(ctx, count: int, plain_count: int, dev):
             "batched_tflops_nominal": fl / tb / 1e12}
 
 
+def layer_flops(L: int) -> float:
+    """Algorithmic FP64 flops of one incident wave of a scatterlayer stack: Gaussian elimination of the
+    augmented 6L x (6L+1) system (complex multiply-add = 8 flops) and the back-substitution."""
+    n = 6 * L
+    return 8.0 * (n - 1) * n * (n + 1) / 3.0 + 8.0 * n * (n - 1) / 2.0
+
+
+def layer_leg(ctx, stream, dev, no_cpu: bool):
+    """scatterlayer (cwx_layer_solve_dev) on the a.cw stack (n = 1 | 1.4+0.05i | 1.4 | 1, z = -3 0 3,
+    lambda = 1): the tool's default 64 x 64 polar beam and a 1024 x 1024 beam, waves resident in HBM."""
+    import math
+    import torch
+    import scattersim_amd as sa
+    lib = sa.lib()
+    L, n, kap, z = sa.layer_stack((1.0, 1.4, 1.4, 1.0), (0.05,), (-3.0, 0.0, 3.0))
+    out = {"workload": "a.cw stack (4 layers), polar beam alpha 1 about +z, lambda 1"}
+    for name, N in (("beam_64x64", 64), ("beam_1024x1024", 1024)):
+        inc = sa.beam_polar(1.0, (0.0, 0.0, 2 * math.pi), (0, 1, 0), N, N)
+        P = len(inc)
+        d_inc = torch.from_numpy(inc).to(dev)
+        d_r = torch.zeros((P, L - 1, 6), dtype=torch.complex128, device=dev)
+        d_t = torch.zeros_like(d_r)
+
+        def call():
+            sa.check(lib.cwx_layer_solve_dev(ctx.h, L, n.ctypes.data, kap.ctypes.data, z.ctypes.data, 1.0, P,
+                                             d_inc.data_ptr(), d_r.data_ptr(), d_t.data_ptr()))
+        call()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            call()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        t = min(ts)
+        fl = P * layer_flops(L)
+        out[name] = {"waves": P, "ms": t * 1e3, "waves_per_s": P / t,
+                     "roofline": {"kernel": "k_layer_reg<4>", "bound": "fp64 vector", "achieved": fl / t / 1e12,
+                                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": fl / t / 1e12 / PEAK_FP64_TFLOPS, "flops_per_wave": layer_flops(L),
+                                  "bytes_per_wave": 96 * (1 + 2 * (L - 1))}}
+    if not no_cpu:
+        from oracle import layer_oracle as lo
+        inc = sa.beam_polar(1.0, (0.0, 0.0, 2 * math.pi), (0, 1, 0), 64, 64)[::8]
+        t0 = time.perf_counter()
+        lo.solve(lo.LayerStack(n=(1.0, 1.4, 1.4, 1.0), kappa=(0.05,), z=(-3.0, 0.0, 3.0), lam=1.0), inc)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(inc) / tc, "unit": "waves/s", "cores": 1, "kind": "port",
+                               "sample": f"{len(inc)} of the 4096 beam waves through the layer oracle"}
+    return out
+
+
+def fields_leg(ctx, d_sample, shape, opts, eig_mode: int, plain_ms: float):
+    """All 2MF unit transverse incident fields of the workload's cell in one cwx_solve_fields_dev call (the
+    order basis of its reflection and transmission matrices), against one plain solve per field."""
+    import torch
+    import scattersim_amd as sa
+    L, s1, s2 = shape
+    M0, M1 = sa.solve_fields(ctx, d_sample.data_ptr(), None, shape=shape, eig_mode=eig_mode, **opts)
+    MF = M0 * M1
+    K = 2 * MF
+    EF = torch.zeros((K, 3 * MF), dtype=torch.complex128, device=d_sample.device)
+    idx = torch.arange(MF, device=d_sample.device)
+    EF[idx, idx] = 1.0
+    EF[MF + idx, MF + idx] = 1.0
+    d_x = torch.zeros((K, 6 * MF), dtype=torch.complex128, device=d_sample.device)
+    p = sa._params(eig_mode=eig_mode, **opts)
+    import ctypes as C
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sa.check(sa.lib().cwx_solve_fields_dev(ctx.h, C.byref(p), C.c_void_p(d_sample.data_ptr()), L, s1, s2, K,
+                                           C.c_void_p(EF.data_ptr()), C.c_void_p(d_x.data_ptr()), None, None))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"workload": "order basis (2MF unit Ex / Ey fields) of the benchmark cell", "fields": K, "s": t,
+            "fields_per_s": K / t, "plain_solves_per_s": 1e3 / plain_ms,
+            "speedup_over_plain": (K / t) / (1e3 / plain_ms)}
+
+
+def compact_leg(sol):
+    """Full and compact .cw of the benchmark solution: sizes, write times, and the rebuild of a device
+    solution from the compact file (host I/O to the node's local disk)."""
+    import tempfile
+    import scattersim_amd as sa
+    with tempfile.TemporaryDirectory() as d:
+        full, comp = os.path.join(d, "c.cw"), os.path.join(d, "c2.cw")
+        t0 = time.perf_counter()
+        sol.write_cw(full)
+        t1 = time.perf_counter()
+        sol.write_cw2(comp)
+        t2 = time.perf_counter()
+        back = sa.read_cw2(sol.ctx, comp)
+        t3 = time.perf_counter()
+        same = bool(np.array_equal(back.x, sol.x))
+        back.free()
+        return {"full_bytes": os.path.getsize(full), "compact_bytes": os.path.getsize(comp),
+                "write_full_s": t1 - t0, "write_compact_s": t2 - t1, "rebuild_s": t3 - t2,
+                "rebuilt_x_identical": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +320,7 @@ def main():
     ap.add_argument("--sweep-count", type=int, default=512, help="instances of the batched sweep leg")
     ap.add_argument("--sweep-plain", type=int, default=24, help="of which solved one at a time")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the layer-stack, many-field and compact legs")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a blit of known traffic that scales the "
                          "profiler's FETCH_SIZE / WRITE_SIZE counters)")
@@ -310,6 +413,11 @@ def main():
     cpu_field = None
     if not args.no_field and not args.no_cpu:
         cpu_field = cpu_field_baseline(sol, grid)
+    nxt = None
+    if not args.no_next:
+        nxt = {"layer": layer_leg(ctx, stream, dev, args.no_cpu),
+               "fields": fields_leg(ctx, d_sample, (L, s1, s2), opts, args.eig_mode, ms),
+               "compact_cw": compact_leg(sol)}
     sol.free()
 
     batched = None
@@ -344,6 +452,7 @@ def main():
         "field": field,
         "field_c3": field_c3,
         "batched": batched,
+        "next": nxt,
         "roofline": {"kernel": "eigensolver stage (gehrd+unghr+hqr+trevc+gemm+normalize)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,

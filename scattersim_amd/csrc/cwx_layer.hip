@@ -8,9 +8,10 @@
 // reflected and a transmitted plane wave per interface, each wound to its boundary (240-266, 390-420).
 //
 // Stacks of up to 10 layers (6L <= 64) are solved by one wavefront per incident wave: lane r owns row r
-// of the augmented system in LDS, eliminates with partial pivoting (largest |re| + |im|, lowest row on
-// ties) and back-substitutes, then the lanes write the wound records.  Taller stacks assemble the
-// systems in HBM and go through the batched LU of this library.
+// of the augmented system, eliminates with partial pivoting (largest |re| + |im|, lowest row on ties) and
+// back-substitutes, then the lanes write the wound records.  Up to 8 layers the rows live in registers
+// (k_layer_reg, one instantiation per layer count); 9 and 10 layers keep them in LDS (k_layer_fused).
+// Taller stacks assemble the systems in HBM and go through the batched LU of this library.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -207,6 +208,122 @@ __global__ __launch_bounds__(64) void k_layer_fused(LayerArgs a) {
   }
 }
 
+// Broadcast of lane l's value to the whole wavefront (v_readlane: l is wave-uniform).
+__device__ inline double lane_d(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline zc lane_z(zc v, int l) { return zc{lane_d(v.re, l), lane_d(v.im, l)}; }
+
+// Rotation of a 16-lane row by s lanes (DPP row_ror: 0x120 + s).
+template <int S>
+__device__ inline double ror_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x120 + S, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x120 + S, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int S>
+__device__ inline int ror_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x120 + S, 0xF, 0xF, false);
+}
+template <int S>
+__device__ inline void row_best(double& v, int& i) {
+  const double ov = ror_d<S>(v);
+  const int oi = ror_i<S>(i);
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+// Stacks of up to 8 layers: one wavefront per incident wave, lane r holding row r of the augmented
+// system in registers.  The pivot (largest |re| + |im|, lowest row on ties) is found by DPP rotations
+// inside each 16-lane row and lane reads across the rows; the pivot row reaches every lane through
+// v_readlane, so an elimination step is register FMAs only.  The rows are assembled through LDS.
+template <int L>
+__global__ __launch_bounds__(64) void k_layer_reg(LayerArgs a) {
+  constexpr int n = 6 * L, rs = n + 1, nrows16 = (n + 15) / 16;
+  extern __shared__ zc S[];
+  __shared__ zc X[n];
+  const int lane = threadIdx.x;
+  for (long p = blockIdx.x; p < a.P; p += gridDim.x) {
+    const zc* rec = a.inc + p * 6;
+    double d[3];
+    inc_direction(rec, d);
+    for (int e = lane; e < n * rs; e += 64) S[e] = mk(0.0);
+    __syncthreads();
+    if (lane < n) {
+      zc* row = S + lane * rs;
+      layer_row(a, lane, d, rec, [&](int c, zc v) { row[c] = v; }, [&](zc v) { row[n] = v; });
+    }
+    __syncthreads();
+    zc row[n + 1];
+    const int mine = lane < n ? lane : 0;
+#pragma unroll
+    for (int c = 0; c <= n; ++c) row[c] = S[mine * rs + c];
+    bool used = lane >= n;
+    int rank = n;
+    int sing = 0;
+    int ord[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      double v = used ? -1.0 : abs1(row[k]);
+      int idx = lane;
+      row_best<1>(v, idx);
+      row_best<2>(v, idx);
+      row_best<4>(v, idx);
+      row_best<8>(v, idx);
+      double bv = lane_d(v, 0);
+      int pr = __builtin_amdgcn_readlane(idx, 0);
+#pragma unroll
+      for (int r = 1; r < nrows16; ++r) {
+        const double ov = lane_d(v, 16 * r);
+        const int oi = __builtin_amdgcn_readlane(idx, 16 * r);
+        if (ov > bv || (ov == bv && oi < pr)) {
+          bv = ov;
+          pr = oi;
+        }
+      }
+      pr = __builtin_amdgcn_readfirstlane(pr);
+      ord[k] = pr;
+      if (lane == pr) {
+        used = true;
+        rank = k;
+      }
+      if (bv == 0.0) sing = 1;
+      const bool act = !used && bv != 0.0;
+      const zc pk = lane_z(row[k], pr);
+      const zc f = act ? zdiv(row[k], pk) : mk(0.0);
+#pragma unroll
+      for (int j = k + 1; j <= n; ++j) {
+        const zc pj = lane_z(row[j], pr);
+        if (act) row[j] -= f * pj;
+      }
+    }
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      const int pr = ord[k];
+      const zc xk = zdiv(lane_z(row[n], pr), lane_z(row[k], pr));
+      if (lane == 0) X[k] = xk;
+      if (rank < k) row[n] -= row[k] * xk;
+    }
+    __syncthreads();
+    if (sing && lane == 0) atomicAdd(a.nsing, 1);
+    if (lane < 2 * (L - 1)) layer_record(a, p, lane, d, X);
+    __syncthreads();
+  }
+}
+
+template <int L>
+void launch_layer_reg(const LayerArgs& a, hipStream_t s) {
+  const size_t lds = sizeof(zc) * 6 * L * (6 * L + 1);
+  const unsigned grid = (unsigned)std::min<long>(a.P, 1L << 20);
+  hipLaunchKernelGGL(k_layer_reg<L>, dim3(grid), dim3(64), lds, s, a);
+}
+
 // Taller stacks: assemble A (column-major, n x n) and b of every incident wave in HBM.
 __global__ void k_layer_assemble(LayerArgs a, long p0, long np, zc* A, zc* B) {
   const int n = a.n;
@@ -260,7 +377,18 @@ int layer_solve_dev(int L, const zc* d_ri, const double* d_z, double lambda, lon
   DevBuf<int> nsing(1);
   CWX_HIP(hipMemsetAsync(nsing.p, 0, sizeof(int), s));
   a.nsing = nsing.p;
-  if (a.n <= LAYER_FUSED_MAXN) {
+  if (L <= 8) {
+    switch (L) {
+      case 2: launch_layer_reg<2>(a, s); break;
+      case 3: launch_layer_reg<3>(a, s); break;
+      case 4: launch_layer_reg<4>(a, s); break;
+      case 5: launch_layer_reg<5>(a, s); break;
+      case 6: launch_layer_reg<6>(a, s); break;
+      case 7: launch_layer_reg<7>(a, s); break;
+      default: launch_layer_reg<8>(a, s); break;
+    }
+    CWX_HIP(hipGetLastError());
+  } else if (a.n <= LAYER_FUSED_MAXN) {
     const size_t lds = sizeof(zc) * a.n * (a.n + 1);
     const unsigned grid = (unsigned)std::min<long>(P, 1L << 20);
     hipLaunchKernelGGL(k_layer_fused, dim3(grid), dim3(64), lds, s, a);
