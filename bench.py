@@ -32,6 +32,8 @@ PEAK_HBM_GBS = 8000.0
 # (run with --copy-calib 1024).  Counters cannot be read inside a timed run, so the profiled figure is
 # carried here and must be refreshed whenever the stage's kernels change.
 PROFILED_STAGE_TRAFFIC = {("volume32", 1): 57.5e9}
+# HBM bytes of one k_layer_reg<4> launch over the 1024 x 1024 beam, measured and scaled the same way.
+PROFILED_LAYER_TRAFFIC = {"beam_1024x1024": 807e6}
 
 
 def eig_flops(n: int) -> float:
@@ -247,6 +249,7 @@ def layer_leg(ctx, stream, dev, no_cpu: bool):
                      "roofline": {"kernel": "k_layer_reg<4>", "bound": "fp64 vector", "achieved": fl / t / 1e12,
                                   "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                                   "frac": fl / t / 1e12 / PEAK_FP64_TFLOPS, "flops_per_wave": layer_flops(L),
+                                  "traffic": PROFILED_LAYER_TRAFFIC.get(name),
                                   "bytes_per_wave": 96 * (1 + 2 * (L - 1))}}
     if not no_cpu:
         from oracle import layer_oracle as lo
