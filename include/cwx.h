@@ -238,9 +238,9 @@ int cwx_layer_write_cw(const char* path, int L, const double* z, int64_t P, cons
  * ScatterLayer.cpp:383-384 and ScatterPlane.cpp:240-241: the cone of half-angle alpha (obscured below
  * beta) about k, sampled at theta_i = beta + i (alpha - beta) / N0 (i < N0) and phi_j = 2 pi j / N1
  * (j < N1, the outer index: out[j N0 + i]).  Each wave has |k| of the axis, amplitude weight
- * sin(theta) dtheta dphi on E's x and y components, and Ez from k . E0 = 0.  For an axis along +z
- * and beta = 0 this reproduces the incident list of x64/a.cw; a tilted axis rotates the cone about
- * z x k (Rodrigues), which no reference output covers. */
+ * sin(theta) dtheta dphi on E's x and y components, and Ez from k . E0 = 0.  A tilted axis rotates
+ * the cone about z x k (Rodrigues).  This reproduces the incident lists of x64/a.cw (axis +z) and
+ * data/psf.cw (axis (1, 0, 1)); an obscuration beta > 0 is covered by no reference output. */
 int cwx_beam_polar(double alpha, double beta, const double k[3], const cwx_complex E[3], int N0, int N1,
                    cwx_wave_f64* out);
 /* tira::planewave<double>::SolidAngleMC (ScatterLayer.cpp:381-382, ScatterPlane.cpp:238-239): N

@@ -9,6 +9,10 @@
 * acw_beam.npz -- all 4096 incident waves of x64/a.cw (the focused beam of scatterlayer's defaults,
                  alpha = 1, 64 x 64 polar samples about +z) as real (E0.x, E0.y, k.x, k.y, k.z) rows;
                  every imaginary part in the file is zero.
+* psf.npz     -- data/psf.cw (scatterplane's defaults: a 64 x 64 polar beam of half-angle 0.5 about
+                 (1, 0, 1), lambda = 1, one interface at z = 0 into 1.4+0.05i): all 4096 incident waves as
+                 real (E0.x, E0.y, k.x, k.y, k.z) rows, and for the waves with k.y = 0 and E0.y != 0 (TE)
+                 their reflected and transmitted records.
 * samples.npz -- the input grids data/layers.npy, data/circle.npy and 3_npy.npy.
 The reference's records hold only (E0.x, E0.y, k); for ky = 0 TE waves E0.y is the global Ey.
 """
@@ -58,6 +62,12 @@ def main(ref):
     raw = np.concatenate([Pi[:, 0:2], Pi[:, 3:6]], axis=1)
     assert np.all(raw.imag == 0)
     np.savez_compressed(os.path.join(here, "acw_beam.npz"), Pi=raw.real.copy())
+    Pi, z, Pr, Pt = read_cw80(os.path.join(ref, "data", "psf.cw"))
+    raw = np.concatenate([Pi[:, 0:2], Pi[:, 3:6]], axis=1)
+    assert np.all(raw.imag == 0) and len(z) == 1
+    te = np.nonzero((np.abs(Pi[:, 4]) == 0) & (np.abs(Pi[:, 1]) > 0))[0]
+    np.savez_compressed(os.path.join(here, "psf.npz"), Pi=raw.real.copy(), z=z, index=te, Pr=Pr[0][te],
+                        Pt=Pt[0][te])
     np.savez_compressed(os.path.join(here, "samples.npz"),
                         layers=np.load(os.path.join(ref, "data", "layers.npy")),
                         circle=np.load(os.path.join(ref, "data", "circle.npy")),

@@ -114,3 +114,16 @@ def test_cli_single_wave_renders(tmp_path):
     pts = o.slice_points(1, 32, 3.0, (0, 0, 0), (0, 0, 0))
     ref = o.field_points(W, zl, b, e, pts, tol=0.001).reshape(32, 32, 3)
     assert rel_err(v, ref) < 1e-5
+
+
+def test_layer_solve_psf_te_known_answer(ctx):
+    """One interface (L = 2): the TE waves of data/psf.cw, scatterplane's default focus."""
+    g = golden("psf.npz")
+    Pi = g["Pi"][g["index"]]
+    inc = np.zeros((len(Pi), 6), np.complex128)
+    inc[:, 0:2] = Pi[:, 0:2]
+    inc[:, 3:6] = Pi[:, 2:5]
+    R, T = sa.scatterlayer(ctx, inc, n=(1.0, 1.4), kappa=(0.05,), z=(0.0,), lam=1.0)
+    drop = lambda a: np.concatenate([a[..., 0:2], a[..., 3:6]], -1)
+    assert rel_err(drop(R[:, 0]), drop(g["Pr"])) < 1e-12
+    assert rel_err(drop(T[:, 0]), drop(g["Pt"])) < 1e-12

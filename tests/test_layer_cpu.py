@@ -12,7 +12,11 @@ Pins of ``oracle/layer_oracle.py``:
 
 The other 4033 incidences of a.cw are not solutions of ScatterLayer.cpp:152-232's system for any incident
 field (the fit leaves residuals of 5e-5 against amplitudes of 8e-4 while the system's condition number is
-about 6); that binary predates the source, so those records pin nothing.
+about 6); that binary predates the source, so those records pin nothing.  data/psf.cw (scatterplane, tira's
+single-interface scatter) shows the same split: its TE waves agree with the layer oracle to rounding, its
+waves off the beam's plane do not (20 % on the transmitted amplitudes).
+
+Both files pin the focused-beam sampler exactly: a.cw's cone about +z and psf.cw's cone about (1, 0, 1).
 """
 import math
 import os
@@ -144,6 +148,29 @@ def test_library_beam_matches_acw_incident_list():
     assert np.all(out[:, 0:2].imag == 0) and np.all(out[:, 3:6].imag == 0)
     # transversality of every sample
     assert np.max(np.abs(np.sum(out[:, 0:3] * out[:, 3:6], 1))) < 1e-18
+
+
+def test_library_beam_matches_psf_incident_list():
+    """The tilted cone: scatterplane's default beam (half-angle 0.5 about (1, 0, 1)) reproduces all 4096
+    incident waves of data/psf.cw."""
+    Pi = golden("psf.npz")["Pi"]
+    k = 2 * math.pi
+    d = np.array([1.0, 0.0, 1.0]) / math.sqrt(2.0)
+    out = sa.beam_polar(0.5, k * d, (0, 1, 0), 64, 64)
+    mine = np.stack([out[:, 0].real, out[:, 1].real, out[:, 3].real, out[:, 4].real, out[:, 5].real], 1)
+    assert np.max(np.abs(mine - Pi)) < 4 * np.finfo(float).eps * k
+
+
+def test_oracle_single_interface_psf_te():
+    """The TE waves of data/psf.cw (tira's single-interface scatter) through the layer oracle with L = 2."""
+    g = golden("psf.npz")
+    Pi = g["Pi"][g["index"]]
+    inc = np.zeros((len(Pi), 6), np.complex128)
+    inc[:, 0:2] = Pi[:, 0:2]
+    inc[:, 3:6] = Pi[:, 2:5]
+    R, T = lo.solve(lo.LayerStack(n=(1.0, 1.4), kappa=(0.05,), z=(0.0,), lam=1.0), inc)
+    assert rel_err(drop_ez(R[:, 0]), drop_ez(g["Pr"])) < 1e-13
+    assert rel_err(drop_ez(T[:, 0]), drop_ez(g["Pt"])) < 1e-13
 
 
 def test_library_beam_tilted_and_montecarlo():
