@@ -310,6 +310,73 @@ def compact_leg(sol):
                 "rebuilt_x_identical": same}
 
 
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def learned_leg(ctx, no_cpu: bool):
+    """The learned closure (c4: 7x7 orders plus permittivity coefficients in, 21x21 R/T out) on 256 fresh
+    random-disk layers: its error and the low-order solution's error against the 21x21 judge solve, and
+    the throughput of low-order solve plus network against the judge.  The network was trained on 20 000
+    solver-generated pairs (python -m scattersim_amd.closure) and ships as scattersim_amd/data/closure_c4.npz."""
+    import scattersim_amd as sa
+    from scattersim_amd import closure as cl
+    net = cl.load(os.path.join(ROOT, "scattersim_amd", "data", "closure_c4.npz"))
+    cfg = net.cfg
+    count = 256
+    layers = cl.disk_layers(cfg, count, seed=987654)
+    cl.solve_orders(ctx, cfg, layers[:16], cfg.low)            # warm-up
+    t0 = time.perf_counter()
+    x_low = cl.solve_orders(ctx, cfg, layers, cfg.low)
+    t_low = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x_high = cl.solve_orders(ctx, cfg, layers, cfg.high, chunk=256)
+    t_high = time.perf_counter() - t0
+    X = np.concatenate([cl._ri(x_low), cl._ri(cl.eps_features(layers, cfg.eps_orders))], axis=1).astype(np.float32)
+    net.predict(ctx, X[:16], x_low[:16])                       # warm-up
+    t0 = time.perf_counter()
+    pred = net.predict(ctx, X, x_low)
+    t_inf = time.perf_counter() - t0
+    e_net = cl.rel_err_rows(pred, x_high)
+    e_base = cl.rel_err_rows(net.baseline(x_low), x_high)
+    # the network's forward pass at a large batch (the library's stream, synchronised around the timed loop)
+    dims = [cfg.d_in, *cfg.hidden, cfg.d_out]
+    B = 16384
+    mlp = cl.MLP(ctx, dims, B, weights=net.weights)
+    Xn = cl._bf16_bits(np.tile((X - net.x_mean) / net.x_std, (B // count, 1)).astype(np.float32))
+    dX = sa.DeviceArray.from_numpy(ctx, Xn)
+    mlp.forward(dX, 0, B)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        mlp.forward(dX, 0, B)
+    ctx.synchronize()
+    t_fw = (time.perf_counter() - t0) / reps
+    fl = 2.0 * B * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    out = {"workload": "c4: random-disk layers (128x128, 8 disks, n 1.3-1.7, 4 um cell, 1 um thick, lambda 0.5), "
+                       "7x7 -> 21x21 orders", "samples": count,
+           "rel_err_closure": float(e_net.mean()), "rel_err_low_order": float(e_base.mean()),
+           "judge_solves_per_s": count / t_high, "low_order_solves_per_s": count / t_low,
+           "closure_samples_per_s": count / (t_low + t_inf),
+           "closure_over_judge": (count / (t_low + t_inf)) / (count / t_high),
+           "params": int(sum(int(np.prod(w.shape)) for w in net.weights)),
+           "forward_roofline": {"kernel": "k_gemm_bf16 (3 layers, batch 16384)", "bound": "mfma bf16",
+                                "achieved": fl / t_fw / 1e12, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                                "frac": fl / t_fw / 1e12 / PEAK_BF16_TFLOPS, "ms": t_fw * 1e3}}
+    if not no_cpu:
+        from oracle import closure_oracle as co
+        params = [(net.weights[2 * i].astype(np.float64), net.weights[2 * i + 1].astype(np.float64))
+                  for i in range(len(dims) - 1)]
+        Xc = ((X - net.x_mean) / net.x_std).astype(np.float32)
+        t0 = time.perf_counter()
+        co.forward(params, Xc)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": count / tc, "unit": "network forwards/s", "kind": "port",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "sample": f"{count} forwards through the closure oracle (numpy)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,7 +487,8 @@ def main():
     if not args.no_next:
         nxt = {"layer": layer_leg(ctx, stream, dev, args.no_cpu),
                "fields": fields_leg(ctx, d_sample, (L, s1, s2), opts, args.eig_mode, ms),
-               "compact_cw": compact_leg(sol)}
+               "compact_cw": compact_leg(sol),
+               "learned": learned_leg(ctx, args.no_cpu)}
     sol.free()
 
     batched = None

@@ -249,6 +249,31 @@ int cwx_beam_polar(double alpha, double beta, const double k[3], const cwx_compl
 int cwx_beam_montecarlo(double alpha, double beta, const double k[3], const cwx_complex E[3], int64_t N,
                         uint64_t seed, cwx_wave_f64* out);
 
+/* ---- learned closure (no counterpart in the reference) ------------------------------------------ */
+/* Kernels of a small multilayer perceptron that maps a low-order solution to high-order reflection and
+ * transmission coefficients, trained on the solver's own solutions.  Device pointers throughout;
+ * BF16 arrays are uint16 bit patterns, FP32 accumulation, row-major activations (rows = samples).
+ * GEMM: C (M x N, row stride ldc) = alpha op(A) op(B) + beta C with A(m, k) = A[m sam + k sak] and
+ * B(k, n) = B[k sbk + n sbn] in BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16). */
+int cwx_bf16_gemm(cwx_context* ctx, int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
+                  int64_t sbk, int64_t sbn, float alpha, float beta, float* C, int64_t ldc);
+/* Z = Zacc + bias (bias per column, may be NULL); H = act(Z) as BF16 (H_bf16) and/or FP32 (H_f32);
+ * act 0 is the identity, 1 the tanh form of GELU.  Z, H_bf16 and H_f32 may be NULL. */
+int cwx_mlp_act_fwd(cwx_context* ctx, int M, int N, int act, const float* Zacc, const float* bias, float* Z,
+                    void* H_bf16, float* H_f32);
+/* dZ = dH act'(Z) as FP32 (dZ) and/or BF16 (dZ_bf16); dbias (may be NULL) = column sums of dZ. */
+int cwx_mlp_act_bwd(cwx_context* ctx, int M, int N, int act, const float* Z, const float* dH, float* dZ,
+                    void* dZ_bf16, float* dbias);
+/* *loss (device, may be NULL) = mean((Y - T)^2) over M x N; dY = 2 (Y - T) / (M N) (FP32 and/or BF16). */
+int cwx_mlp_mse(cwx_context* ctx, int M, int N, const float* Y, const float* T, float* dY, void* dY_bf16,
+                float* loss);
+/* One bias-corrected Adam step (step >= 1) on n FP32 weights w with gradient g and moments m, v;
+ * w_bf16 (may be NULL) receives the rounded copy used by the GEMMs. */
+int cwx_mlp_adam(cwx_context* ctx, int64_t n, float lr, float beta1, float beta2, float eps, int step, float* w,
+                 const float* g, float* m, float* v, void* w_bf16);
+/* y = x rounded to BF16 (to nearest, ties to even). */
+int cwx_f32_to_bf16(cwx_context* ctx, int64_t n, const float* x, void* y);
+
 /* ---- field evaluation (gpuEvaluator.h:14-17) ------------------------------------------------- */
 /* gpu_initialize (gpuEvaluator.cu:15-26): upload an unpacked FP32 wave table and region map. */
 int cwx_field_init_f32(cwx_context* ctx, const cwx_wave_f32* W, int64_t nwaves, const float* z_layers,

@@ -90,6 +90,13 @@ SIGNATURES = {
     "cwx_layer_solve": (I, [P, I, P, P, P, C.c_double, C.c_int64, P, P, P]),
     "cwx_layer_solve_dev": (I, [P, I, P, P, P, C.c_double, C.c_int64, P, P, P]),
     "cwx_layer_write_cw": (I, [C.c_char_p, I, P, C.c_int64, P, P, P]),
+    "cwx_bf16_gemm": (I, [P, I, I, I, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int64, C.c_float, C.c_float, P,
+                          C.c_int64]),
+    "cwx_mlp_act_fwd": (I, [P, I, I, I, P, P, P, P, P]),
+    "cwx_mlp_act_bwd": (I, [P, I, I, I, P, P, P, P, P]),
+    "cwx_mlp_mse": (I, [P, I, I, P, P, P, P, P]),
+    "cwx_mlp_adam": (I, [P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, I, P, P, P, P, P]),
+    "cwx_f32_to_bf16": (I, [P, C.c_int64, P, P]),
     "cwx_beam_polar": (I, [C.c_double, C.c_double, P, P, I, I, P]),
     "cwx_beam_montecarlo": (I, [C.c_double, C.c_double, P, P, C.c_int64, C.c_uint64, P]),
     "cwx_solution_free": (I, [P]),

@@ -22,7 +22,7 @@ BINDIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIBDIR, "libcwx.so")
 
 HIP_SOURCES = ["cwx_linalg.hip", "cwx_eig.hip", "cwx_field.hip", "cwx_fieldsep.hip", "cwx_solver.hip",
-               "cwx_layer.hip", "cwx_capi.hip"]
+               "cwx_layer.hip", "cwx_mlp.hip", "cwx_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [

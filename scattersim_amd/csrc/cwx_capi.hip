@@ -11,6 +11,7 @@
 
 #include "../../include/cwx.h"
 #include "cwx_layer.h"
+#include "cwx_mlp.h"
 #include "cwx_solver.h"
 
 struct cwx_context {
@@ -767,6 +768,73 @@ int cwx_beam_montecarlo(double alpha, double beta, const double k[3], const cwx_
       const double ct = cb - u(rng) * (cb - ca), ph = 2.0 * M_PI * u(rng);
       beam_wave(k, kmag, E, std::acos(ct), ph, w, out[i]);
     }
+    return CWX_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+int cwx_bf16_gemm(cwx_context* ctx, int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
+                  int64_t sbk, int64_t sbn, float alpha, float beta, float* C, int64_t ldc) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && (M == 0 || N == 0 || (A && B && C)), "NULL argument");
+    CWX_CHECK(M >= 0 && N >= 0 && K >= 0 && ldc >= N, "bad GEMM dimensions");
+    set_device(ctx);
+    cwx::gemm_bf16(M, N, K, (const uint16_t*)A, sam, sak, (const uint16_t*)B, sbk, sbn, alpha, beta, C, ldc,
+                   ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_mlp_act_fwd(cwx_context* ctx, int M, int N, int act, const float* Zacc, const float* bias, float* Z,
+                    void* H_bf16, float* H_f32) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && Zacc, "NULL argument");
+    CWX_CHECK(M >= 0 && N >= 1 && (act == 0 || act == 1), "bad activation arguments");
+    set_device(ctx);
+    cwx::mlp_act_fwd(M, N, act, Zacc, bias, Z, (uint16_t*)H_bf16, H_f32, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_mlp_act_bwd(cwx_context* ctx, int M, int N, int act, const float* Z, const float* dH, float* dZ,
+                    void* dZ_bf16, float* dbias) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dH && (act == 0 || Z), "NULL argument");
+    CWX_CHECK(M >= 0 && N >= 1 && (act == 0 || act == 1), "bad activation arguments");
+    set_device(ctx);
+    cwx::mlp_act_bwd(M, N, act, Z, dH, dZ, (uint16_t*)dZ_bf16, dbias, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_mlp_mse(cwx_context* ctx, int M, int N, const float* Y, const float* T, float* dY, void* dY_bf16,
+                float* loss) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && Y && T, "NULL argument");
+    CWX_CHECK(M >= 1 && N >= 1, "bad loss dimensions");
+    set_device(ctx);
+    if (loss) CWX_HIP(hipMemsetAsync(loss, 0, sizeof(float), ctx->c.stream));
+    cwx::mlp_mse(M, N, Y, T, dY, (uint16_t*)dY_bf16, loss, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_mlp_adam(cwx_context* ctx, int64_t n, float lr, float beta1, float beta2, float eps, int step, float* w,
+                 const float* g, float* m, float* v, void* w_bf16) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && w && g && m && v, "NULL argument");
+    CWX_CHECK(n >= 0, "negative size");
+    set_device(ctx);
+    if (n) cwx::mlp_adam(n, lr, beta1, beta2, eps, step, w, g, m, v, (uint16_t*)w_bf16, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_f32_to_bf16(cwx_context* ctx, int64_t n, const float* x, void* y) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && (n == 0 || (x && y)), "NULL argument");
+    set_device(ctx);
+    if (n) cwx::to_bf16(n, x, (uint16_t*)y, ctx->c.stream);
     return CWX_OK;
   });
 }

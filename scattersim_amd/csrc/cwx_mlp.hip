@@ -1,0 +1,216 @@
+// Kernels of the learned closure: a small multilayer perceptron trained and run in BF16 with FP32
+// accumulation on the matrix cores.
+//
+// GEMM: C (FP32) = alpha op(A) op(B) + beta C with A and B in BF16 under arbitrary element strides, so one
+// kernel serves the forward product (X W^T), the data gradient (dY W) and the weight gradient (dY^T X).
+// A 256-thread workgroup computes a 64 x 64 tile of C: 64 x 16 and 16 x 64 slices of A and B are staged
+// through LDS per step of k, and each of the four wavefronts accumulates a 32 x 32 block as 2 x 2
+// v_mfma_f32_16x16x16_bf16 products.  For that instruction lane l supplies A[l % 16][4(l/16) + 0..3] and
+// B[4(l/16) + 0..3][l % 16], and holds D[4(l/16) + q][l % 16] in register q.
+// The element-wise kernels: bias + GELU (forward and backward, with the bias gradient), the mean-squared
+// error and its gradient, Adam on FP32 master weights with BF16 copies, and FP32 -> BF16 rounding (to
+// nearest even).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "cwx_common.h"
+#include "cwx_mlp.h"
+
+namespace cwx {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int GT = 64;   // C tile
+constexpr int GK = 16;   // k step
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, long sam,
+                                                   long sak, const uint16_t* __restrict__ B, long sbk, long sbn,
+                                                   float alpha, float beta, float* __restrict__ C, long ldc) {
+  __shared__ uint16_t As[GT][GK + 2];
+  __shared__ uint16_t Bs[GK][GT + 2];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    for (int e = tid; e < GT * GK; e += 256) {
+      const int r = e / GK, kk = e % GK, gm = m0 + r, gk = k0 + kk;
+      As[r][kk] = (gm < M && gk < K) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
+    }
+    for (int e = tid; e < GT * GK; e += 256) {
+      const int kk = e / GT, c = e % GT, gk = k0 + kk, gn = n0 + c;
+      Bs[kk][c] = (gk < K && gn < N) ? B[(long)gk * sbk + (long)gn * sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int ar = wm + 16 * i + li;
+      const bf16x4 a = {(short)As[ar][lk], (short)As[ar][lk + 1], (short)As[ar][lk + 2], (short)As[ar][lk + 3]};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int bc = wn + 16 * j + li;
+        const bf16x4 b = {(short)Bs[lk][bc], (short)Bs[lk + 1][bc], (short)Bs[lk + 2][bc], (short)Bs[lk + 3][bc]};
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm + 16 * i + lk + q, col = n0 + wn + 16 * j + li;
+        if (row < M && col < N) {
+          float* c = C + (long)row * ldc + col;
+          const float v = alpha * acc[i][j][q];
+          *c = beta == 0.f ? v : v + beta * *c;
+        }
+      }
+}
+
+__device__ inline float gelu(float z, float& dg) {
+  const float c = 0.7978845608028654f, a = 0.044715f;
+  const float u = c * (z + a * z * z * z);
+  const float t = tanhf(u);
+  dg = 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * c * (1.f + 3.f * a * z * z);
+  return 0.5f * z * (1.f + t);
+}
+
+__global__ void k_act_fwd(int M, int N, int act, const float* __restrict__ Zacc, const float* __restrict__ bias,
+                          float* __restrict__ Z, uint16_t* __restrict__ H, float* __restrict__ Hf) {
+  const long n = (long)M * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float z = Zacc[e] + (bias ? bias[e % N] : 0.f);
+    float dg;
+    const float h = act ? gelu(z, dg) : z;
+    if (Z) Z[e] = z;
+    if (H) H[e] = to_bf16(h);
+    if (Hf) Hf[e] = h;
+  }
+}
+
+// dZ = dH * act'(Z); one thread per column also sums the bias gradient over the rows.
+__global__ void k_act_bwd(int M, int N, int act, const float* __restrict__ Z, const float* __restrict__ dH,
+                          float* __restrict__ dZ, uint16_t* __restrict__ dZb, float* __restrict__ dbias) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int r = 0; r < M; ++r) {
+      const long e = (long)r * N + c;
+      float dg = 1.f;
+      if (act) (void)gelu(Z[e], dg);
+      const float g = dH[e] * dg;
+      if (dZ) dZ[e] = g;
+      if (dZb) dZb[e] = to_bf16(g);
+      s += g;
+    }
+    if (dbias) dbias[c] = s;
+  }
+}
+
+// Mean squared error over M x N; dY = 2 (Y - T) / (M N); loss accumulated per block.
+__global__ void k_mse(int M, int N, const float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
+                      uint16_t* __restrict__ dYb, float* __restrict__ loss) {
+  __shared__ float part[256];
+  const long n = (long)M * N;
+  const float sc = 2.f / (float)n;
+  float s = 0.f;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float d = Y[e] - T[e];
+    s += d * d;
+    if (dY) dY[e] = sc * d;
+    if (dYb) dYb[e] = to_bf16(sc * d);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, part[0] / (float)n);
+}
+
+__global__ void k_adam(long n, float lr, float b1, float b2, float eps, float c1, float c2, float* __restrict__ w,
+                       const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       uint16_t* __restrict__ wb) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.f - b1) * gi;
+    const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    const float wi = w[e] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[e] = wi;
+    if (wb) wb[e] = to_bf16(wi);
+  }
+}
+
+__global__ void k_to_bf16(long n, const float* __restrict__ x, uint16_t* __restrict__ y) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    y[e] = to_bf16(x[e]);
+}
+
+unsigned grid1(long n) { return (unsigned)std::min<long>(65535, std::max<long>(1, (n + 255) / 256)); }
+
+}  // namespace
+
+void gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn,
+               float alpha, float beta, float* C, long ldc, hipStream_t s) {
+  CWX_CHECK(M >= 0 && N >= 0 && K >= 0, "negative GEMM dimension");
+  if (M == 0 || N == 0) return;
+  CWX_CHECK((M + GT - 1) / GT <= 65535, "GEMM: too many row tiles");
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((N + GT - 1) / GT, (M + GT - 1) / GT), dim3(256), 0, s, M, N, K, A, sam, sak,
+                     B, sbk, sbn, alpha, beta, C, ldc);
+  CWX_HIP(hipGetLastError());
+}
+
+void mlp_act_fwd(int M, int N, int act, const float* Zacc, const float* bias, float* Z, uint16_t* H, float* Hf,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_act_fwd, dim3(grid1((long)M * N)), dim3(256), 0, s, M, N, act, Zacc, bias, Z, H, Hf);
+  CWX_HIP(hipGetLastError());
+}
+
+void mlp_act_bwd(int M, int N, int act, const float* Z, const float* dH, float* dZ, uint16_t* dZb, float* dbias,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_act_bwd, dim3(grid1(N)), dim3(256), 0, s, M, N, act, Z, dH, dZ, dZb, dbias);
+  CWX_HIP(hipGetLastError());
+}
+
+void mlp_mse(int M, int N, const float* Y, const float* T, float* dY, uint16_t* dYb, float* loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(std::min<unsigned>(grid1((long)M * N), 1024)), dim3(256), 0, s, M, N, Y, T, dY, dYb,
+                     loss);
+  CWX_HIP(hipGetLastError());
+}
+
+void mlp_adam(long n, float lr, float b1, float b2, float eps, int step, float* w, const float* g, float* m, float* v,
+              uint16_t* wb, hipStream_t s) {
+  CWX_CHECK(step >= 1, "Adam steps count from 1");
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(256), 0, s, n, lr, b1, b2, eps, c1, c2, w, g, m, v, wb);
+  CWX_HIP(hipGetLastError());
+}
+
+void to_bf16(long n, const float* x, uint16_t* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid1(n)), dim3(256), 0, s, n, x, y);
+  CWX_HIP(hipGetLastError());
+}
+
+}  // namespace cwx
