@@ -257,6 +257,11 @@ int cwx_beam_montecarlo(double alpha, double beta, const double k[3], const cwx_
  * B(k, n) = B[k sbk + n sbn] in BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16). */
 int cwx_bf16_gemm(cwx_context* ctx, int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
                   int64_t sbk, int64_t sbn, float alpha, float beta, float* C, int64_t ldc);
+/* One dense layer in one launch: Z = X W^T + bias and H = act(Z), with X (M x K, row stride ldx) and W
+ * (N x K, row stride ldw) in BF16; K, ldx and ldw multiples of 4, X and W 8-byte aligned.  Outputs as
+ * cwx_mlp_act_fwd (row stride N). */
+int cwx_mlp_dense_fwd(cwx_context* ctx, int M, int N, int K, const void* X, int64_t ldx, const void* W, int64_t ldw,
+                      const float* bias, int act, float* Z, void* H_bf16, float* H_f32);
 /* Z = Zacc + bias (bias per column, may be NULL); H = act(Z) as BF16 (H_bf16) and/or FP32 (H_f32);
  * act 0 is the identity, 1 the tanh form of GELU.  Z, H_bf16 and H_f32 may be NULL. */
 int cwx_mlp_act_fwd(cwx_context* ctx, int M, int N, int act, const float* Zacc, const float* bias, float* Z,

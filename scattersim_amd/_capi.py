@@ -92,6 +92,7 @@ SIGNATURES = {
     "cwx_layer_write_cw": (I, [C.c_char_p, I, P, C.c_int64, P, P, P]),
     "cwx_bf16_gemm": (I, [P, I, I, I, P, C.c_int64, C.c_int64, P, C.c_int64, C.c_int64, C.c_float, C.c_float, P,
                           C.c_int64]),
+    "cwx_mlp_dense_fwd": (I, [P, I, I, I, P, C.c_int64, P, C.c_int64, P, I, P, P, P]),
     "cwx_mlp_act_fwd": (I, [P, I, I, I, P, P, P, P, P]),
     "cwx_mlp_act_bwd": (I, [P, I, I, I, P, P, P, P, P]),
     "cwx_mlp_mse": (I, [P, I, I, P, P, P, P, P]),

@@ -33,6 +33,19 @@ def _p(a: DeviceArray, offset_elems: int = 0) -> C.c_void_p:
     return C.c_void_p(a.ptr + offset_elems * a.dtype.itemsize)
 
 
+def _pad4(n: int) -> int:
+    """Input width rounded up to a multiple of 4, so the forward product reads 8-byte BF16 fragments."""
+    return (n + 3) // 4 * 4
+
+
+def _pad_cols(a: np.ndarray, n: int) -> np.ndarray:
+    if a.shape[-1] == n:
+        return a
+    out = np.zeros(a.shape[:-1] + (n,), a.dtype)
+    out[..., :a.shape[-1]] = a
+    return out
+
+
 class MLP:
     """Layers dims[0] -> dims[1] -> ... -> dims[-1], GELU between them, on the device.
 
@@ -79,11 +92,16 @@ class MLP:
         Hin, ld_in = _p(X, row0 * self.dims[0]), self.dims[0]
         for i in range(len(self.dims) - 1):
             nin, nout = self.dims[i], self.dims[i + 1]
-            check(L.cwx_bf16_gemm(h, rows, nout, nin, Hin, ld_in, 1, _p(self.Wb[i]), 1, nin, 1.0, 0.0, _p(self.acc),
-                                  nout))
             last = i == len(self.dims) - 2
-            check(L.cwx_mlp_act_fwd(h, rows, nout, 0 if last else 1, _p(self.acc), _p(self.b[i]), _p(self.Z[i]),
-                                    None if last else _p(self.H[i]), _p(self.Y) if last else None))
+            outs = (_p(self.Z[i]), None if last else _p(self.H[i]), _p(self.Y) if last else None)
+            if nin % 4 == 0:
+                # one launch: product, bias and activation
+                check(L.cwx_mlp_dense_fwd(h, rows, nout, nin, Hin, ld_in, _p(self.Wb[i]), nin, _p(self.b[i]),
+                                          0 if last else 1, *outs))
+            else:
+                check(L.cwx_bf16_gemm(h, rows, nout, nin, Hin, ld_in, 1, _p(self.Wb[i]), 1, nin, 1.0, 0.0,
+                                      _p(self.acc), nout))
+                check(L.cwx_mlp_act_fwd(h, rows, nout, 0 if last else 1, _p(self.acc), _p(self.b[i]), *outs))
             if not last:
                 Hin, ld_in = _p(self.H[i]), nout
         return self.Y
@@ -234,6 +252,11 @@ def _ri(z: np.ndarray) -> np.ndarray:
     return np.concatenate([z.real, z.imag], axis=-1)
 
 
+def features(cfg: ClosureConfig, layers: np.ndarray, x_low: np.ndarray) -> np.ndarray:
+    """Network inputs: the low-order [R; T] and the central n^2 Fourier coefficients, real and imaginary."""
+    return np.concatenate([_ri(x_low), _ri(eps_features(layers, cfg.eps_orders))], axis=1).astype(np.float32)
+
+
 @dataclass
 class Dataset:
     X: np.ndarray        # (count, d_in) float32 features (unnormalised)
@@ -247,7 +270,7 @@ def make_dataset(ctx, cfg: ClosureConfig, count: int, seed: int, layers=None, lo
         layers = disk_layers(cfg, count, seed)
     x_low = solve_orders(ctx, cfg, layers, cfg.low, log=log)
     x_high = solve_orders(ctx, cfg, layers, cfg.high, log=log)
-    X = np.concatenate([_ri(x_low), _ri(eps_features(layers, cfg.eps_orders))], axis=1).astype(np.float32)
+    X = features(cfg, layers, x_low)
     corr = take(x_high, cfg.high, cfg.band) - place(x_low, cfg.low, cfg.band)
     return Dataset(X=X, T=_ri(corr).astype(np.float32), x_low=x_low, x_high=x_high)
 
@@ -287,8 +310,11 @@ class Closure:
     def predict(self, ctx, X: np.ndarray, x_low: np.ndarray, batch: int = 1024) -> np.ndarray:
         """High-order [R; T] (count, 6 high^2): the low-order solution plus the network's correction."""
         cfg = self.cfg
-        net = MLP(ctx, [cfg.d_in, *cfg.hidden, cfg.d_out], batch, weights=self.weights)
-        Xn = _bf16_bits(((X - self.x_mean) / self.x_std).astype(np.float32))
+        d0 = _pad4(cfg.d_in)
+        w = list(self.weights)
+        w[0] = _pad_cols(np.asarray(w[0], np.float32), d0)
+        net = MLP(ctx, [d0, *cfg.hidden, cfg.d_out], batch, weights=w)
+        Xn = _bf16_bits(_pad_cols(((X - self.x_mean) / self.x_std).astype(np.float32), d0))
         dX = DeviceArray.from_numpy(ctx, Xn)
         out = np.zeros((len(X), cfg.d_out), np.float32)
         for r0 in range(0, len(X), batch):
@@ -311,9 +337,10 @@ def train(ctx, cfg: ClosureConfig, data: Dataset, epochs: int, batch: int = 256,
     x_mean = data.X.mean(axis=0)
     x_std = data.X.std(axis=0) + 1e-12
     t_std = np.full(data.T.shape[1], data.T.std() + 1e-30, np.float32)
-    Xn = ((data.X - x_mean) / x_std).astype(np.float32)
+    d0 = _pad4(cfg.d_in)
+    Xn = _pad_cols(((data.X - x_mean) / x_std).astype(np.float32), d0)
     Tn = (data.T / t_std).astype(np.float32)
-    net = MLP(ctx, [cfg.d_in, *cfg.hidden, cfg.d_out], batch, seed=seed)
+    net = MLP(ctx, [d0, *cfg.hidden, cfg.d_out], batch, seed=seed)
     rng = np.random.default_rng(seed)
     n = len(Xn)
     hist = []
@@ -334,7 +361,9 @@ def train(ctx, cfg: ClosureConfig, data: Dataset, epochs: int, batch: int = 256,
             log(f"epoch {ep}: loss {hist[-1]:.4e}")
         dX.free()
         dT.free()
-    return Closure(cfg=cfg, weights=net.weights(), x_mean=x_mean.astype(np.float32), x_std=x_std.astype(np.float32),
+    w = net.weights()
+    w[0] = np.ascontiguousarray(w[0][:, :cfg.d_in])   # the padding columns only ever meet zero inputs
+    return Closure(cfg=cfg, weights=w, x_mean=x_mean.astype(np.float32), x_std=x_std.astype(np.float32),
                    t_std=t_std, history=hist)
 
 

@@ -785,6 +785,18 @@ int cwx_bf16_gemm(cwx_context* ctx, int M, int N, int K, const void* A, int64_t 
   });
 }
 
+int cwx_mlp_dense_fwd(cwx_context* ctx, int M, int N, int K, const void* X, int64_t ldx, const void* W, int64_t ldw,
+                      const float* bias, int act, float* Z, void* H_bf16, float* H_f32) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && (M == 0 || (X && W)), "NULL argument");
+    CWX_CHECK(ldx >= K && ldw >= K, "bad dense-layer strides");
+    set_device(ctx);
+    cwx::mlp_dense_fwd(M, N, K, (const uint16_t*)X, ldx, (const uint16_t*)W, ldw, bias, act, Z, (uint16_t*)H_bf16,
+                       H_f32, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
 int cwx_mlp_act_fwd(cwx_context* ctx, int M, int N, int act, const float* Zacc, const float* bias, float* Z,
                     void* H_bf16, float* H_f32) {
   return guard([&]() -> int {

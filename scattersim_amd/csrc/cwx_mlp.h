@@ -11,6 +11,10 @@ namespace cwx {
 // C (M x N, row stride ldc) = alpha op(A) op(B) + beta C; A(m, k) = A[m sam + k sak], B(k, n) = B[k sbk + n sbn].
 void gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn,
                float alpha, float beta, float* C, long ldc, hipStream_t s);
+// One dense layer in one launch: Z = X W^T + bias, H = act(Z) (X: M x K rows of stride ldx, W: N x K rows of
+// stride ldw, both BF16 and contiguous along K; K, ldx, ldw multiples of 4).  Z, H (BF16), Hf (FP32) may be null.
+void mlp_dense_fwd(int M, int N, int K, const uint16_t* X, long ldx, const uint16_t* W, long ldw, const float* bias,
+                   int act, float* Z, uint16_t* H, float* Hf, hipStream_t s);
 // Z = Zacc + bias (per column); H = bf16(act(Z)), Hf = act(Z) (act 0: identity, 1: GELU); outputs may be null.
 void mlp_act_fwd(int M, int N, int act, const float* Zacc, const float* bias, float* Z, uint16_t* H, float* Hf,
                  hipStream_t s);

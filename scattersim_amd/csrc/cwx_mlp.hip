@@ -7,6 +7,8 @@
 // through LDS per step of k, and each of the four wavefronts accumulates a 32 x 32 block as 2 x 2
 // v_mfma_f32_16x16x16_bf16 products.  For that instruction lane l supplies A[l % 16][4(l/16) + 0..3] and
 // B[4(l/16) + 0..3][l % 16], and holds D[4(l/16) + q][l % 16] in register q.
+// When both operands are contiguous along k (the forward product), k_gemm_bf16_kc reads the fragments
+// straight from memory instead.
 // The element-wise kernels: bias + GELU (forward and backward, with the bias gradient), the mean-squared
 // error and its gradient, Adam on FP32 master weights with BF16 copies, and FP32 -> BF16 rounding (to
 // nearest even).
@@ -83,6 +85,86 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
           float* c = C + (long)row * ldc + col;
           const float v = alpha * acc[i][j][q];
           *c = beta == 0.f ? v : v + beta * *c;
+        }
+      }
+}
+
+// Both operands contiguous along k (A(m, k) = A[m lda + k], B(k, n) = B[n ldb + k]; k, lda, ldb multiples of 4,
+// 8-byte aligned bases): every lane reads its MFMA fragments straight from memory as 8-byte loads, no LDS.
+// Each wavefront accumulates a 64 x 64 block as 4 x 4 products per step of k (8 loads per 16 MFMAs); four
+// wavefronts make a 128 x 128 tile, whose rows and columns they share through the L1 cache.
+constexpr int KC_W = 64;    // per-wavefront block
+constexpr int KC_T = 128;   // workgroup tile
+__device__ inline float gelu(float z, float& dg);
+
+// Epilogue of the fast GEMM: with bias or an activation it writes Z = acc + bias (FP32, may be null),
+// H = act(Z) as BF16 and/or FP32 (row stride N) instead of alpha acc + beta C.
+struct DenseOut {
+  const float* bias = nullptr;
+  int act = -1;                 // -1: plain GEMM epilogue; 0: identity; 1: GELU
+  float* Z = nullptr;
+  uint16_t* H = nullptr;
+  float* Hf = nullptr;
+};
+
+__global__ __launch_bounds__(256) void k_gemm_bf16_kc(int M, int N, int K, const uint16_t* __restrict__ A, long lda,
+                                                      const uint16_t* __restrict__ B, long ldb, float alpha,
+                                                      float beta, float* __restrict__ C, long ldc, DenseOut d) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.y * KC_T + (w >> 1) * KC_W, n0 = blockIdx.x * KC_T + (w & 1) * KC_W;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x4 zero = {0, 0, 0, 0};
+  const uint16_t* arow[4];
+  const uint16_t* bcol[4];
+  bool aok[4], bok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = m0 + 16 * i + li, c = n0 + 16 * i + li;
+    aok[i] = r < M;
+    bok[i] = c < N;
+    arow[i] = A + (long)(aok[i] ? r : 0) * lda + lk;
+    bcol[i] = B + (long)(bok[i] ? c : 0) * ldb + lk;
+  }
+  for (int k = 0; k < K; k += GK) {
+    const bool kin = k + lk < K;
+    bf16x4 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = (aok[i] && kin) ? *reinterpret_cast<const bf16x4*>(arow[i] + k) : zero;
+      b[i] = (bok[i] && kin) ? *reinterpret_cast<const bf16x4*>(bcol[i] + k) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + 16 * i + lk + q, col = n0 + 16 * j + li;
+        if (row < M && col < N) {
+          if (d.act < 0) {
+            float* c = C + (long)row * ldc + col;
+            const float v = alpha * acc[i][j][q];
+            *c = beta == 0.f ? v : v + beta * *c;
+          } else {
+            const long e = (long)row * N + col;
+            const float z = alpha * acc[i][j][q] + (d.bias ? d.bias[col] : 0.f);
+            float dg;
+            const float h = d.act ? gelu(z, dg) : z;
+            if (d.Z) d.Z[e] = z;
+            if (d.H) d.H[e] = to_bf16(h);
+            if (d.Hf) d.Hf[e] = h;
+          }
         }
       }
 }
@@ -177,8 +259,32 @@ void gemm_bf16(int M, int N, int K, const uint16_t* A, long sam, long sak, const
   CWX_CHECK(M >= 0 && N >= 0 && K >= 0, "negative GEMM dimension");
   if (M == 0 || N == 0) return;
   CWX_CHECK((M + GT - 1) / GT <= 65535, "GEMM: too many row tiles");
-  hipLaunchKernelGGL(k_gemm_bf16, dim3((N + GT - 1) / GT, (M + GT - 1) / GT), dim3(256), 0, s, M, N, K, A, sam, sak,
-                     B, sbk, sbn, alpha, beta, C, ldc);
+  const bool kc = sak == 1 && sbk == 1 && K % 4 == 0 && sam % 4 == 0 && sbn % 4 == 0 &&
+                  ((uintptr_t)A & 7) == 0 && ((uintptr_t)B & 7) == 0;
+  if (kc)
+    hipLaunchKernelGGL(k_gemm_bf16_kc, dim3((N + KC_T - 1) / KC_T, (M + KC_T - 1) / KC_T), dim3(256), 0, s, M, N, K,
+                       A, sam, B, sbn, alpha, beta, C, ldc, DenseOut{});
+  else
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + GT - 1) / GT, (M + GT - 1) / GT), dim3(256), 0, s, M, N, K, A, sam,
+                       sak, B, sbk, sbn, alpha, beta, C, ldc);
+  CWX_HIP(hipGetLastError());
+}
+
+void mlp_dense_fwd(int M, int N, int K, const uint16_t* X, long ldx, const uint16_t* W, long ldw, const float* bias,
+                   int act, float* Z, uint16_t* H, float* Hf, hipStream_t s) {
+  CWX_CHECK(M >= 0 && N >= 1 && K >= 0 && (act == 0 || act == 1), "bad dense-layer arguments");
+  CWX_CHECK(K % 4 == 0 && ldx % 4 == 0 && ldw % 4 == 0 && ((uintptr_t)X & 7) == 0 && ((uintptr_t)W & 7) == 0,
+            "dense layer: the input width and both row strides must be multiples of 4 BF16 values, 8-byte aligned");
+  if (M == 0) return;
+  CWX_CHECK((M + KC_T - 1) / KC_T <= 65535, "dense layer: too many row tiles");
+  DenseOut d;
+  d.bias = bias;
+  d.act = act;
+  d.Z = Z;
+  d.H = H;
+  d.Hf = Hf;
+  hipLaunchKernelGGL(k_gemm_bf16_kc, dim3((N + KC_T - 1) / KC_T, (M + KC_T - 1) / KC_T), dim3(256), 0, s, M, N, K, X,
+                     ldx, W, ldw, 1.f, 0.f, nullptr, (long)N, d);
   CWX_HIP(hipGetLastError());
 }
 
