@@ -22,8 +22,8 @@ from . import _capi
 from ._capi import check, lib
 
 __all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
-           "cw_read_unpacked", "read_cw2", "solve_fields", "scatterlayer", "layer_stack", "layer_write_cw", "beam_polar",
-           "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
+           "cw_read_unpacked", "read_cw2", "solve_fields", "scatterlayer", "layer_stack", "layer_write_cw",
+           "beam_polar", "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
 WAVE_F64 = np.dtype([("E0", np.complex128, 3), ("k", np.complex128, 3)])

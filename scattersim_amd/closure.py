@@ -406,7 +406,8 @@ def main(argv=None):
     t0 = time.time()
     data = make_dataset(ctx, cfg, a.count + a.test, seed=a.seed, log=lambda s: print(s, flush=True))
     t_data = time.time() - t0
-    print(f"{a.count + a.test} pairs ({cfg.low}x{cfg.low} and {cfg.high}x{cfg.high} orders) in {t_data:.1f} s", flush=True)
+    print(f"{a.count + a.test} pairs ({cfg.low}x{cfg.low} and {cfg.high}x{cfg.high} orders) in {t_data:.1f} s",
+          flush=True)
     n = a.count
     tr = Dataset(X=data.X[:n], T=data.T[:n], x_low=data.x_low[:n], x_high=data.x_high[:n])
     t0 = time.time()
@@ -421,7 +422,8 @@ def main(argv=None):
     save_bf16(net, a.out)
     print(json.dumps({"pairs": n, "test": a.test, "data_s": t_data, "train_s": t_train, "epochs": a.epochs,
                       "params": sum(int(np.prod(w.shape)) for w in net.weights),
-                      "held_out_rel_err_low_order": float(e_base.mean()), "held_out_rel_err_closure": float(e_net.mean()),
+                      "held_out_rel_err_low_order": float(e_base.mean()),
+                      "held_out_rel_err_closure": float(e_net.mean()),
                       "loss_first": net.history[0], "loss_last": net.history[-1]}), flush=True)
     ctx.close()
 

@@ -214,7 +214,8 @@ int main(int argc, char** argv) {
   if (st > 0) std::cerr << "warning: " << cwx_last_error() << std::endl;
   if (logfile) logfile << "solved " << P << " incident waves in " << std::chrono::duration<double>(t1 - t0).count()
                        << " s" << std::endl;
-  if (!output.empty() && cwx_layer_write_cw(output.c_str(), (int)L, z.data(), P, I.data(), R.data(), T.data()) != CWX_OK) {
+  if (!output.empty() &&
+      cwx_layer_write_cw(output.c_str(), (int)L, z.data(), P, I.data(), R.data(), T.data()) != CWX_OK) {
     std::cerr << "error: " << cwx_last_error() << std::endl;
     cwx_destroy(ctx);
     return 2;
