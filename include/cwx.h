@@ -139,6 +139,16 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
                         const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
                         int* M1);
 
+/* cwx_solve_batch_dev for instances whose Fourier orders (or eigen modes) differ, as wavelength and angle
+ * sweeps with the order cut-off produce: the instances are grouped by orders and each group is one
+ * batched solve.  x_offsets (host, count + 1) receives the start of every instance's [R; T] in d_x
+ * (6 M0_i M1_i complex each, instances in input order; x_offsets[count] is the total); M0s, M1s (host,
+ * count, may be NULL) the orders.  With d_x == NULL only these are computed (ctx and d_samples may be
+ * NULL), so the caller can size d_x first.  No counterpart in the reference. */
+int cwx_solve_batch_mixed_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                              const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int64_t* x_offsets,
+                              int* h_info, int* M0s, int* M1s);
+
 /* Many incident fields on one sample, sharing every layer's eigen-decomposition, the transfer chain
  * and the boundary factorisation.  The reference builds one incident field, the Fourier
  * coefficients EF of the plane wave (ScatterVolume.cpp:660-674; with --psf every order carries E),

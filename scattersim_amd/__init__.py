@@ -22,8 +22,8 @@ from . import _capi
 from ._capi import check, lib
 
 __all__ = ["Context", "DeviceArray", "Solution", "scattervolume", "zgeev", "zgesv", "zinverse", "zgemm",
-           "cw_read_unpacked", "read_cw2", "solve_fields", "scatterlayer", "layer_stack", "layer_write_cw",
-           "beam_polar", "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
+           "cw_read_unpacked", "read_cw2", "solve_fields", "solve_batch_mixed", "scatterlayer", "layer_stack",
+           "layer_write_cw", "beam_polar", "beam_montecarlo", "WAVE_F32", "WAVE_F64"]
 
 WAVE_F32 = np.dtype([("E0", np.complex64, 3), ("k", np.complex64, 3)])   # cpuEvaluator.h:6-10
 WAVE_F64 = np.dtype([("E0", np.complex128, 3), ("k", np.complex128, 3)])
@@ -339,6 +339,39 @@ def beam_montecarlo(alpha: float, k, E, N: int, beta: float = 0.0, seed: int = 0
     out = np.zeros((N, 6), np.complex128)
     check(lib().cwx_beam_montecarlo(float(alpha), float(beta), _ptr(kv), _ptr(Ev), int(N), int(seed), _ptr(out)))
     return out
+
+
+def solve_batch_mixed(ctx: Context, samples, params, max_bytes: int = 0):
+    """Solve independent samples whose Fourier orders may differ (cwx_solve_batch_mixed_dev).
+
+    ``samples`` is a list of (L_i, s1, s2) complex arrays on one grid, ``params`` one option dict per
+    instance.  Returns (xs, info, orders): xs[i] is instance i's [R; T] (6 M0_i M1_i complex128)."""
+    count = len(params)
+    pa = (_capi.cwx_volume_params * count)()
+    for i, o in enumerate(params):
+        pa[i] = _params(**o)
+    arrs = [np.asarray(s_, dtype=np.complex128) for s_ in samples]
+    arrs = [a[None] if a.ndim == 2 else a for a in arrs]
+    s1, s2 = arrs[0].shape[1:]
+    if any(a.shape[1:] != (s1, s2) for a in arrs) or len(arrs) != count:
+        raise ValueError("one sample per instance, all on one grid")
+    Ls = np.array([a.shape[0] for a in arrs], np.int32)
+    off = np.zeros(count + 1, np.int64)
+    M0 = np.zeros(count, np.int32)
+    M1 = np.zeros(count, np.int32)
+    check(lib().cwx_solve_batch_mixed_dev(None, count, pa, _ptr(Ls), None, int(s1), int(s2), int(max_bytes), None,
+                                          _ptr(off), None, _ptr(M0), _ptr(M1)))
+    dS = DeviceArray.from_numpy(ctx, np.ascontiguousarray(np.concatenate(arrs, axis=0)))
+    dX = DeviceArray(ctx, (int(off[-1]),), np.complex128)
+    info = np.zeros(count, np.int32)
+    st = lib().cwx_solve_batch_mixed_dev(ctx.h, count, pa, _ptr(Ls), C.c_void_p(dS.ptr), int(s1), int(s2),
+                                         int(max_bytes), C.c_void_p(dX.ptr), _ptr(off), _ptr(info), None, None)
+    if st < 0:
+        raise _capi.CwxError(st, lib().cwx_last_error().decode())
+    x = dX.numpy()
+    dS.free()
+    dX.free()
+    return [x[off[i]:off[i + 1]] for i in range(count)], info, list(zip(M0.tolist(), M1.tolist()))
 
 
 def solve_fields(ctx: Context, sample, EF, **opts):

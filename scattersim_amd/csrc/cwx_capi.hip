@@ -417,6 +417,37 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
   });
 }
 
+int cwx_solve_batch_mixed_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                              const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int64_t* x_offsets,
+                              int* h_info, int* M0s, int* M1s) {
+  return guard([&]() -> int {
+    CWX_CHECK(params && L && x_offsets, "NULL argument");
+    CWX_CHECK(count >= 1 && s1 >= 1 && s2 >= 1 && max_bytes >= 0, "bad batch arguments");
+    std::vector<cwx::VolumeOptions> opts(count);
+    std::vector<int> Ls(L, L + count);
+    std::vector<long> off(count + 1, 0);
+    for (int i = 0; i < count; ++i) {
+      CWX_CHECK(Ls[i] >= 1, "instance " + std::to_string(i) + " has no layers");
+      opts[i] = to_options(params + i);
+      const cwx::Setup st = cwx::make_setup(opts[i], Ls[i], s1, s2);
+      if (M0s) M0s[i] = st.M0;
+      if (M1s) M1s[i] = st.M1;
+      off[i + 1] = off[i] + 6L * st.MF;
+    }
+    for (int i = 0; i <= count; ++i) x_offsets[i] = off[i];
+    if (!d_x) return CWX_OK;
+    CWX_CHECK(ctx && d_samples, "NULL argument");
+    set_device(ctx);
+    std::vector<int> info;
+    std::string msg;
+    const int code = cwx::solve_batch_mixed(ctx->c, opts, (const zc*)d_samples, Ls, s1, s2, (size_t)max_bytes,
+                                            (zc*)d_x, off, info, msg);
+    if (h_info) std::copy(info.begin(), info.end(), h_info);
+    if (code) return fail(code, msg);
+    return CWX_OK;
+  });
+}
+
 int cwx_solve_fields_dev(cwx_context* ctx, const cwx_volume_params* p, const void* d_sample, int L, int s1, int s2,
                          int K, const void* d_EF, void* d_x, int* M0, int* M1) {
   return guard([&]() -> int {

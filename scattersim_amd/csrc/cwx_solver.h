@@ -78,6 +78,12 @@ void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int 
 int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
                 int s1, int s2, size_t max_bytes, zc* d_x, std::vector<int>& info, std::string& first_msg);
 
+// Like solve_batch, for instances whose orders (and eigen modes) differ: instances are grouped by orders,
+// each group is one batched solve, and instance i's [R; T] (6 M0_i M1_i) lands at d_x + x_off[i].
+int solve_batch_mixed(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples,
+                      const std::vector<int>& Ls, int s1, int s2, size_t max_bytes, zc* d_x,
+                      const std::vector<long>& x_off, std::vector<int>& info, std::string& first_msg);
+
 // K incident fields on one sample with one eigen-decomposition, one transfer chain and one boundary
 // factorisation: d_EF holds K x 3MF incident Fourier coefficients (component-major, order index
 // as Setup::EF), d_x receives K x 6MF boundary solutions [R; T].  Returns the solve's status.

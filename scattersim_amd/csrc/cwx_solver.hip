@@ -7,6 +7,7 @@
 #include "cwx_solver.h"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -1221,6 +1222,69 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
     }
     layer0 += layers;
     i0 = i1;
+  }
+  return worst;
+}
+
+int solve_batch_mixed(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples,
+                      const std::vector<int>& Ls, int s1, int s2, size_t max_bytes, zc* d_x,
+                      const std::vector<long>& x_off, std::vector<int>& info, std::string& first_msg) {
+  const int count = (int)opts.size();
+  CWX_CHECK(count >= 1 && (int)Ls.size() == count && (int)x_off.size() == count + 1, "batch is empty");
+  // group the instances by their orders and eigen mode; each group is one batched solve
+  std::vector<long> lay0(count + 1, 0);
+  for (int i = 0; i < count; ++i) lay0[i + 1] = lay0[i] + Ls[i];
+  std::vector<std::array<int, 3>> key(count);
+  for (int i = 0; i < count; ++i) {
+    const Setup st = make_setup(opts[i], Ls[i], s1, s2);
+    key[i] = {st.M0, st.M1, opts[i].eig_mode};
+  }
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  info.assign(count, 0);
+  first_msg.clear();
+  int worst = 0;
+  const long npix = (long)s1 * s2;
+  for (size_t g0 = 0; g0 < order.size();) {
+    size_t g1 = g0;
+    while (g1 < order.size() && key[order[g1]] == key[order[g0]]) ++g1;
+    std::vector<VolumeOptions> o;
+    std::vector<int> l;
+    long layers = 0;
+    for (size_t t = g0; t < g1; ++t) {
+      o.push_back(opts[order[t]]);
+      l.push_back(Ls[order[t]]);
+      layers += Ls[order[t]];
+    }
+    // the group's layers back to back, and its [R; T] rows
+    DevBuf<zc> samples((size_t)layers * npix);
+    long at = 0;
+    for (size_t t = g0; t < g1; ++t) {
+      const int i = order[t];
+      CWX_HIP(hipMemcpyAsync(samples.p + at * npix, d_samples + lay0[i] * npix, sizeof(zc) * Ls[i] * npix,
+                             hipMemcpyDeviceToDevice, ctx.stream));
+      at += Ls[i];
+    }
+    const long n6 = x_off[order[g0] + 1] - x_off[order[g0]];
+    DevBuf<zc> xs((size_t)n6 * (g1 - g0));
+    std::vector<int> ginfo;
+    std::string msg;
+    solve_batch(ctx, o, samples.p, l, s1, s2, max_bytes, xs.p, ginfo, msg);
+    for (size_t t = g0; t < g1; ++t) {
+      const int i = order[t];
+      CWX_CHECK(x_off[i + 1] - x_off[i] == n6, "x offsets do not match the orders");
+      CWX_HIP(hipMemcpyAsync(d_x + x_off[i], xs.p + (long)(t - g0) * n6, sizeof(zc) * n6, hipMemcpyDeviceToDevice,
+                             ctx.stream));
+      info[i] = ginfo[t - g0];
+    }
+    CWX_HIP(hipStreamSynchronize(ctx.stream));  // the group buffers are released on return
+    for (int i = 0; i < count; ++i)
+      if (info[i] && !worst) {
+        worst = info[i];
+        first_msg = "instance " + std::to_string(i) + " failed";
+      }
+    g0 = g1;
   }
   return worst;
 }

@@ -66,3 +66,21 @@ def test_batch_device_input_and_errors(ctx):
     with pytest.raises(sa._capi.CwxError):
         sa.solve_batch(ctx, samples[:2], bad)
     d.free()
+
+
+def test_batch_with_mixed_orders(ctx):
+    """cwx_solve_batch_mixed_dev: instances at different orders in one call (grouped by orders), each equal bit
+    for bit to its own solve; the cut-off picks the orders of one instance itself."""
+    samples, params = instances(11, 7, M=(3, 3), grid=12)
+    coefs = [(3, 3), (5, 5), (3, 3), (5, 3), (5, 5), (3, 5), (5, 3)]
+    for p, c in zip(params, coefs):
+        p["coef"] = c
+    params[6]["no_cutoff"] = False
+    xs, info, orders = sa.solve_batch_mixed(ctx, samples, params)
+    assert np.all(info == 0) and len(xs) == 7
+    for s, p, x, (M0, M1) in zip(samples, params, xs, orders):
+        sol = sa.scattervolume(ctx, s, **p)
+        assert (sol.M0, sol.M1) == (M0, M1) and x.shape == (6 * M0 * M1,)
+        assert np.array_equal(sol.x, x)
+        sol.free()
+    assert orders[1] == (5, 5) and orders[5] == (3, 5)
