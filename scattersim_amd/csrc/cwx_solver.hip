@@ -1464,6 +1464,7 @@ void cw2_read_file(Context& ctx, const std::string& path, Solution& sol) {
   CWX_CHECK(L >= 1 && L + 1 == ZL, path + ": layer count does not match the boundaries");
   const int M0 = m[0], M1 = m[1], MF = M0 * M1, n4 = 4 * MF, n6 = 6 * MF;
   const size_t nn = (size_t)n4 * n4;
+  CWX_CHECK(sizeof(zc) * nn * L <= ((size_t)64 << 30), path + ": mode data larger than 64 GiB");
   std::vector<zc> beta((size_t)L * n4), gam((size_t)L * n4), V(nn * L), nif((size_t)L * MF), gg(nn);
   for (uint32_t l = 0; l < L; ++l) {
     get(beta.data() + (size_t)l * n4, sizeof(zc) * n4);
