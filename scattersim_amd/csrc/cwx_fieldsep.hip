@@ -172,12 +172,14 @@ struct PlaneGroup {
 };
 constexpr int SEP_G = 8;  // planes per group
 
-// 64 coefficients per workgroup; the SEP_W wavefronts take slices of the 4MF-term sums and their
-// partial sums are added in LDS in a fixed order.
+// 64 coefficients per workgroup over one slice of the 4MF modes (blockIdx.z of gridDim.z slices, so that a few
+// layers still give the chip enough workgroups to stream Q at full rate); the SEP_W wavefronts split the slice
+// and their partial sums are added in LDS in a fixed order.  Slice s writes C + s cstride; the rows pass adds
+// the slices in order.
 constexpr int SEP_W = 8;
 __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec g, const PlaneGroup* __restrict__ groups,
-                                                      zc* __restrict__ C) {
-  extern __shared__ zc ph[];  // count x 4MF mode phases
+                                                      zc* __restrict__ C, long cstride) {
+  extern __shared__ zc ph[];  // count x slice mode phases
   __shared__ zc part[SEP_W][SEP_G][64];
   const PlaneGroup gr = groups[blockIdx.y];
   const int MF = a.MF, n3 = 3 * MF, n4 = 4 * MF;
@@ -186,9 +188,12 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
   const int i = l - 1;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int e = blockIdx.x * 64 + lane;
+  const int S = gridDim.z, sl = blockIdx.z;
+  const int msl = (n4 + S - 1) / S, mlo = min(n4, sl * msl), mhi = min(n4, mlo + msl), ns = mhi - mlo;
+  C += sl * cstride;
   if (layer && !a.external) {
-    for (int t = threadIdx.x; t < gr.count * n4; t += blockDim.x) {
-      const int pl = t / n4, m = t % n4;
+    for (int t = threadIdx.x; t < gr.count * ns; t += blockDim.x) {
+      const int pl = t / ns, m = mlo + t % ns;
       const double z = (double)coord(g.z0, g.dz, gr.iz0 + pl);
       const bool up = m >= 2 * MF;
       const int mode = up ? 2 * (m - 2 * MF) + 1 : 2 * m;
@@ -200,14 +205,14 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
 #pragma unroll
     for (int t = 0; t < SEP_G; ++t) acc[t] = mk(0.0);
     if (e < n3) {
-      const int mq = (n4 + SEP_W - 1) / SEP_W, m0 = wv * mq, m1 = min(n4, m0 + mq);
-      const zc* Q = a.Qch + (long)i * n3 * n4 + e;
+      const int mq = (ns + SEP_W - 1) / SEP_W, m0 = wv * mq, m1 = min(ns, m0 + mq);
+      const zc* Q = a.Qch + (long)i * n3 * n4 + (long)mlo * n3 + e;
 #pragma unroll 4
       for (int m = m0; m < m1; ++m) {
         const zc q = Q[(long)m * n3];
 #pragma unroll
         for (int t = 0; t < SEP_G; ++t)
-          if (t < gr.count) fma_acc(acc[t], q, ph[t * n4 + m]);
+          if (t < gr.count) fma_acc(acc[t], q, ph[t * ns + m]);
       }
     }
 #pragma unroll
@@ -227,11 +232,11 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
   for (int t = 0; t < gr.count; ++t) {
     const double z = (double)coord(g.z0, g.dz, gr.iz0 + t);
     zc v = mk(0.0);
-    if (l == 0) {
+    if (sl == 0 && l == 0) {
       const zc kzi = a.Sz0[p] * a.k;
       const zc kzr = -(a.Sz0[p] * a.k) * a.n0;
       v = a.EF[c * MF + p] * cexpi(kzi, z) + a.x[c * MF + p] * cexpi(kzr, z - a.zl[0]);
-    } else if (l == a.L + 1) {
+    } else if (sl == 0 && l == a.L + 1) {
       const zc kzt = (a.Sz1[p] * a.k) * a.n1;
       v = a.x[n3 + c * MF + p] * cexpi(kzt, z - a.zl[a.L]);
     }
@@ -241,6 +246,15 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
 
 // T[iz][c][b][ix] = sum_a exp(i kx_a x_ix) C[iz][c][a M1 + b] as complex64, b padded to K4 with zeros;
 // Yf[b][iy] = exp(i ky_b y_iy) as complex64 (same padding).
+// C[e] = sum of the S slices of the coefficient pass, added in slice order (into slice 0).
+__global__ void k_sep_reduce(long n, int S, long cstride, zc* __restrict__ C) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    zc v = C[e];
+    for (int sl = 1; sl < S; ++sl) v = v + C[sl * cstride + e];
+    C[e] = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, const zc* __restrict__ C,
                                                   const zc* __restrict__ X, int K4, float2* __restrict__ T) {
   extern __shared__ zc sCp[];  // 3 MF of this plane
@@ -379,10 +393,18 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
     CWX_CHECK(K4 <= 64, "more than 64 orders along y in the complex64 evaluator");
     w.Tf.ensure((size_t)g.nz * 3 * K4 * g.nx);
     w.Yf.ensure((size_t)K4 * g.ny);
-    hipLaunchKernelGGL(k_sep_coeffs_g, dim3(cdiv(3 * MF, 64), (unsigned)groups.size()), dim3(64 * SEP_W),
-                       sizeof(zc) * gmax * n4, s, a, g, (const PlaneGroup*)w.groups.p, w.C.p);
+    // mode slices: enough workgroups to keep HBM busy when there are few plane groups
+    const long base = (long)cdiv(3 * MF, 64) * (long)groups.size();
+    const int S = (int)std::max<long>(1, std::min<long>({16L, (4096 + base - 1) / base, (long)n4 / 64}));
+    const long cstride = (long)g.nz * 3 * MF;
+    w.C.ensure((size_t)S * cstride);
+    const int msl = (n4 + S - 1) / S;
+    hipLaunchKernelGGL(k_sep_coeffs_g, dim3(cdiv(3 * MF, 64), (unsigned)groups.size(), S), dim3(64 * SEP_W),
+                       sizeof(zc) * gmax * msl, s, a, g, (const PlaneGroup*)w.groups.p, w.C.p, cstride);
     hipLaunchKernelGGL(k_sep_tables, dim3(cdiv(g.nx * st.M0, 256)), dim3(256), 0, s, a, g, w.X.p, w.Y.p);
     const long per = (long)3 * K4 * g.nx;
+    if (S > 1)
+      hipLaunchKernelGGL(k_sep_reduce, dim3(cdiv(cstride, 256)), dim3(256), 0, s, cstride, S, cstride, w.C.p);
     hipLaunchKernelGGL(k_sep_rows, dim3((unsigned)cdiv((int)per, 256), g.nz), dim3(256), sizeof(zc) * 3 * MF, s, a,
                        g, w.C.p, w.X.p, K4, w.Tf.p);
     hipLaunchKernelGGL(k_sep_ytab, dim3(cdiv(K4 * g.ny, 256)), dim3(256), 0, s, a, g, K4, w.Yf.p);
