@@ -554,7 +554,7 @@ __device__ inline bool negligible_c(const zc* S, int ld, int k, int lo, int hi, 
 
 // Eigenvalues of the m x m upper Hessenberg S (LDS, column-major, m <= 64) by single-shift QR
 // without Schur form or vectors (zlahqr with wantt = wantz = false), run by one wavefront.
-__device__ bool small_eigvals_wave(zc* S, int m, zc* w, double smlnum) {
+__device__ __attribute__((always_inline)) inline bool small_eigvals_wave(zc* S, int m, zc* w, double smlnum) {
   const int lane = threadIdx.x & 63;
   auto Sp = [&](int r, int c) -> zc& { return S[c * m + r]; };
   int i = m - 1, kdefl = 0, its = 0;
@@ -672,6 +672,332 @@ __device__ void real_subdiag(zc* H, int ldh, zc* Z, int ldz, int n, int L, int I
     z = z * d[c - f];
   }
   __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Aggressive early deflation (the zlaqr3 algorithm of Braman, Byers and Mathias).  Before a
+// multishift sweep, the trailing AED_W x AED_W window of the active block is brought to Schur form
+// T = V^H W V.  Its coupling to the rest of H is the single spike s = H(kw, kw-1), which becomes
+// the vector s conj(V(0, :)) in the new basis: every bottom entry of T whose spike component is
+// negligible deflates at once; the others are moved to the top of T by swaps of adjacent diagonal
+// entries.  When something deflated, the spike is reflected back onto its first entry, the
+// undeflated part is reduced to Hessenberg form again, and V is applied to the rest of H and to Z.
+// The undeflated eigenvalues of the window are the next sweep's shifts.
+// ------------------------------------------------------------------------------------------------
+constexpr int AED_W = 16;          // window rows (at most 32: lanes 0..31 take T's lines, 32..63 V's rows)
+constexpr int AED_LD = AED_W + 1;  // leading dimension of T and V in LDS
+
+// Schur form of the m x m upper Hessenberg S (LDS, column-major, leading dimension ld, m <= 32), with
+// the Schur vectors accumulated into V (initially the identity): zlahqr with wantt = wantz = true on
+// one wavefront.  Left products touch S's columns (one per lane), right products S's rows (lanes
+// 0..31) and V's rows (lanes 32..63) in the same phase.  Returns false at the iteration limit.
+__device__ __attribute__((always_inline)) inline bool schur_wave(zc* S, zc* V, int m, int ld, double smlnum) {
+  const int lane = threadIdx.x & 63;
+  auto Sp = [&](int r, int c) -> zc& { return S[c * ld + r]; };
+  int i = m - 1, kdefl = 0, its = 0;
+  const int itmax = 30 * max(10, m) * m;
+  while (i >= 0) {
+    if (++its > itmax) return false;
+    const int k = lane + 1;
+    const bool neg = (k <= i) && negligible_c(S, ld, k, 0, m - 1, smlnum);
+    const unsigned long long mask = __ballot(neg);
+    const int l = mask ? (64 - __clzll(mask)) : 0;
+    wave_sync();
+    if (l > 0 && lane == 0) Sp(l, l - 1) = mk(0.0);
+    wave_sync();
+    if (l >= i) {
+      i = l - 1;
+      kdefl = 0;
+      continue;
+    }
+    ++kdefl;
+    zc t;
+    if (kdefl % 20 == 0) {
+      t = mk(0.75 * abs1(Sp(i, i - 1))) + Sp(i, i);
+    } else if (kdefl % 10 == 0) {
+      t = mk(0.75 * abs1(Sp(l + 1, l))) + Sp(l, l);
+    } else {
+      t = Sp(i, i);
+      const zc u = zsqrt(Sp(i - 1, i)) * zsqrt(Sp(i, i - 1));
+      double s = abs1(u);
+      if (s != 0.0) {
+        const zc x = 0.5 * (Sp(i - 1, i - 1) - t);
+        const double sx = abs1(x);
+        s = fmax(s, sx);
+        const zc xs = x * (1.0 / s), us = u * (1.0 / s);
+        zc y = s * zsqrt(xs * xs + us * us);
+        if (sx > 0.0) {
+          const zc xn = x * (1.0 / sx);
+          if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
+        }
+        t = t - u * zdiv(u, x + y);
+      }
+    }
+    for (int kk = l; kk < i; ++kk) {
+      zc v1, v2;
+      if (kk == l) {
+        const zc h11s = Sp(l, l) - t, h21 = Sp(l + 1, l);
+        const double sc = abs1(h11s) + abs1(h21);
+        v1 = h11s * (1.0 / sc);
+        v2 = h21 * (1.0 / sc);
+      } else {
+        v1 = Sp(kk, kk - 1);
+        v2 = Sp(kk + 1, kk - 1);
+      }
+      const Rot2 g = make_reflector2(v1, v2);
+      wave_sync();
+      if (kk + lane < m) rot2_left(g, Sp(kk, kk + lane), Sp(kk + 1, kk + lane));
+      if (kk > l && lane == 0) {
+        Sp(kk, kk - 1) = v1;
+        Sp(kk + 1, kk - 1) = mk(0.0);
+      }
+      wave_sync();
+      {
+        // one code path for T's rows (lanes 0..31) and V's rows (lanes 32..63)
+        const bool isv = lane >= 32;
+        const int r = isv ? lane - 32 : lane;
+        zc* const base = isv ? V : S;
+        if (r <= (isv ? m - 1 : min(kk + 2, i))) rot2_right(g, base[kk * ld + r], base[(kk + 1) * ld + r]);
+      }
+      wave_sync();
+    }
+  }
+  return true;
+}
+
+// Plane rotation [c s; -conj(s) c] with c real that maps (f, g) to (r, 0) (the zlartg contract):
+// c = |f| / sqrt(|f|^2 + |g|^2), s = (f / |f|) conj(g) / sqrt(|f|^2 + |g|^2).  f and g are first scaled by
+// the same (approximate) reciprocal of max(|f|, |g|)_1, which leaves c and s unchanged; then one
+// reciprocal square root gives both, c = |f|^2 q and s = f conj(g) q with q = (|f|^2 (|f|^2 + |g|^2))^-1/2.
+__device__ inline void lartg(zc f, zc g, double& c, zc& s) {
+  const double af = abs1(f), ag = abs1(g);
+  if (ag == 0.0) {
+    c = 1.0;
+    s = mk(0.0);
+    return;
+  }
+  const double r = __builtin_amdgcn_rcp(fmax(af, ag));
+  const zc fs = f * r, gs = g * r;
+  const double f2 = norm2(fs), g2 = norm2(gs);
+  if (af == 0.0 || f2 < 1e-200) {
+    // |f| negligible against |g| (or zero): the general form of zlartg
+    const double ga = absz(g);
+    if (af == 0.0) {
+      c = 0.0;
+      s = conj(g) * (1.0 / ga);
+      return;
+    }
+    const double fa = absz(f), nrm = hypot(fa, ga);
+    c = fa / nrm;
+    s = (f * (1.0 / fa)) * conj(g) * (1.0 / nrm);
+    return;
+  }
+  const double q = rsqrt(f2 * (f2 + g2));
+  c = f2 * q;
+  s = fs * conj(gs) * q;
+}
+
+// Householder reflector I - tau u u^H (u_0 = 1) with (I - tau u u^H)^H (alpha, x) = (beta, 0) for the
+// vector held by lanes lo..hi-1 of one wavefront (x0 on lane lo): the zlarfg algorithm.  Every lane
+// receives tau and its own entry of u (zero outside lo..hi-1); returns beta.
+__device__ inline double wave_larfg(zc xi, int lo, int hi, zc alpha, zc& tau, zc& ui) {
+  const int lane = threadIdx.x & 63;
+  const double xn2 = wave_sum((lane > lo && lane < hi) ? norm2(xi) : 0.0);
+  if (xn2 == 0.0 && alpha.im == 0.0) {
+    tau = mk(0.0);
+    ui = (lane == lo) ? mk(1.0) : mk(0.0);
+    return alpha.re;
+  }
+  const double beta = -copysign(sqrt(norm2(alpha) + xn2), alpha.re);
+  tau = (mk(beta) - alpha) * (1.0 / beta);
+  const zc sc = zdiv(mk(1.0), alpha - mk(beta));
+  ui = (lane == lo) ? mk(1.0) : (lane > lo && lane < hi) ? xi * sc : mk(0.0);
+  return beta;
+}
+
+// One wavefront, after schur_wave: the deflation test and reordering of zlaqr3 (ztrexc swaps), then,
+// when something deflated and more than one entry did not, the spike reflection and the Hessenberg
+// reduction (zgehrd) of the undeflated block, all accumulated into V.  hv: LDS vector of AED_W.
+// Returns the number ns of undeflated eigenvalues, now T(0..ns-1) before the reduction (their values
+// are written to w[0..ns-1]).
+__device__ __attribute__((always_inline)) inline int aed_deflate_wave(zc* T, zc* V, zc s, zc* hv, zc* w, double smlnum) {
+  const int lane = threadIdx.x & 63, m = AED_W, ld = AED_LD;
+  auto Tp = [&](int r, int c) -> zc& { return T[c * ld + r]; };
+  auto Vp = [&](int r, int c) -> zc& { return V[c * ld + r]; };
+  int ns = m, ilst = 0;
+  const double as = abs1(s);
+  for (int knt = 0; knt < m; ++knt) {
+    const int k = ns - 1;
+    double foo = abs1(Tp(k, k));
+    if (foo == 0.0) foo = as;
+    if (as * abs1(Vp(0, k)) <= fmax(smlnum, kUlp * foo)) {
+      --ns;
+      continue;
+    }
+    for (int p = k - 1; p >= ilst; --p) {
+      // swap T(p, p) and T(p+1, p+1)
+      const zc t11 = Tp(p, p), t22 = Tp(p + 1, p + 1);
+      double c;
+      zc sn;
+      lartg(Tp(p, p + 1), t22 - t11, c, sn);
+      // rows p, p+1 of T's columns p+2..m-1 take (c, s); columns p, p+1 of T's rows 0..p-1 and of V's
+      // rows take (c, conj(s)): one code path, the pair and the sine chosen per lane
+      const bool isv = lane >= 32, rowop = !isv && lane >= p + 2;
+      const int q = isv ? lane - 32 : lane;
+      const bool act = isv ? q < m : (rowop ? lane < m : lane < p);
+      zc* const base = isv ? V : T;
+      const int o1 = rowop ? q * ld + p : p * ld + q, o2 = rowop ? o1 + 1 : o1 + ld;
+      const zc sr = rowop ? sn : conj(sn);
+      if (act) {
+        const zc x = base[o1], y = base[o2];
+        base[o1] = c * x + sr * y;
+        base[o2] = c * y - conj(sr) * x;
+      }
+      if (lane == 0) {
+        Tp(p, p) = t22;
+        Tp(p + 1, p + 1) = t11;
+      }
+      wave_sync();
+    }
+    ++ilst;
+  }
+  for (int j = lane; j < ns; j += 64) w[j] = Tp(j, j);
+  wave_sync();
+  if (ns == m || ns <= 1 || (s.re == 0.0 && s.im == 0.0)) return ns;
+
+  // reflect the spike s conj(V(0, 0..ns-1)) onto its first entry: T <- P^H T P, V <- V P
+  auto reflect = [&](int lo, int hi, zc tau, zc ui) {
+    // P = I - tau u u^H on indices lo..hi-1 (u in hv).  Left: T(lo..hi-1, lo..m-1); right: T(0..hi-1,
+    // lo..hi-1) and V(0..m-1, lo..hi-1).
+    if (lane < hi && lane >= lo) hv[lane] = ui;
+    wave_sync();
+    const zc tc = conj(tau);
+    if (lane < 32 && lane >= lo && lane < m) {
+      const int c = lane;
+      zc d = mk(0.0);
+      for (int r = lo; r < hi; ++r) fma_acc(d, conj(hv[r]), Tp(r, c));
+      d = tc * d;
+      for (int r = lo; r < hi; ++r) Tp(r, c) -= hv[r] * d;
+    }
+    wave_sync();
+    zc* rowp = nullptr;
+    int q = lane;
+    if (lane < 32) {
+      if (lane < hi) rowp = T;
+    } else if (lane - 32 < m) {
+      rowp = V;
+      q = lane - 32;
+    }
+    if (rowp) {
+      zc d = mk(0.0);
+      for (int c = lo; c < hi; ++c) fma_acc(d, rowp[c * ld + q], hv[c]);
+      d = tau * d;
+      for (int c = lo; c < hi; ++c) rowp[c * ld + q] -= d * conj(hv[c]);
+    }
+    wave_sync();
+  };
+  {
+    const zc wi = (lane < ns) ? conj(Vp(0, lane)) : mk(0.0);
+    const zc alpha = conj(Vp(0, 0));
+    zc tau, ui;
+    wave_larfg(wi, 0, ns, alpha, tau, ui);
+    wave_sync();
+    if (tau.re != 0.0 || tau.im != 0.0) reflect(0, ns, tau, ui);
+  }
+  // Hessenberg reduction of T(0..ns-1, 0..ns-1); the left products also reach columns ns..m-1
+  for (int j = 0; j + 2 < ns; ++j) {
+    const zc xi = (lane < 32) ? Tp(lane, j) : mk(0.0);
+    const zc alpha = Tp(j + 1, j);
+    zc tau, ui;
+    const double beta = wave_larfg(xi, j + 1, ns, alpha, tau, ui);
+    wave_sync();
+    if (lane == j + 1) Tp(j + 1, j) = mk(beta);
+    if (lane > j + 1 && lane < ns) Tp(lane, j) = mk(0.0);
+    wave_sync();
+    if (tau.re != 0.0 || tau.im != 0.0) reflect(j + 1, ns, tau, ui);
+  }
+  return ns;
+}
+
+// The trailing window H(kw..I, kw..I) with kw = I - AED_W + 1 (kw - 1 >= L, so the spike lies inside the
+// active block).  Called by the whole workgroup; T and V occupy 2 AED_LD AED_W entries of the LDS
+// scratch `buf`.  Returns the number of deflated eigenvalues (the window's bottom rows, now with a zero
+// subdiagonal above them), or -1 when the window's QR reached its iteration limit (H and Z unchanged).
+// When at least MS_NB / 2 undeflated eigenvalues remain, the last min(ns, MS_NB) of them are written to
+// shifts and sh[1] is their count, else sh[1] = 0.
+__device__ __attribute__((always_inline)) inline int aed_window(zc* H, int ldh, zc* Z, int ldz, int n, int I, zc* buf, zc* hv, zc* shifts, int* sh,
+                          double smlnum) {
+  const int tid = threadIdx.x, nt = blockDim.x, kw = I - AED_W + 1;
+  zc* T = buf;
+  zc* V = buf + AED_LD * AED_W;
+  auto Hp = [&](int r, int c) -> zc& { return H[(long)c * ldh + r]; };
+  for (int e = tid; e < AED_W * AED_W; e += nt) {
+    const int r = e % AED_W, c = e / AED_W;
+    T[c * AED_LD + r] = (r <= c + 1) ? Hp(kw + r, kw + c) : mk(0.0);
+    V[c * AED_LD + r] = mk(r == c ? 1.0 : 0.0);
+  }
+  const zc s = Hp(kw, kw - 1);
+  __syncthreads();
+  if (tid < 64) {
+    const bool ok = schur_wave(T, V, AED_W, AED_LD, smlnum);
+    int ns = -1;
+    if (ok) ns = aed_deflate_wave(T, V, s, hv, hv + AED_W, smlnum);
+    const int nsh = (ns >= MS_NB / 2) ? min(ns, MS_NB) : 0;
+    if (tid == 0) {
+      sh[0] = ns;
+      sh[1] = nsh;
+    }
+    if (tid < nsh) shifts[tid] = hv[AED_W + ns - nsh + tid];
+  }
+  __syncthreads();
+  const int ns = sh[0];
+  if (ns < 0 || ns == AED_W) return ns < 0 ? -1 : 0;
+  // write the window back, the new spike, and apply V to the rest of H and to Z
+  for (int e = tid; e < AED_W * AED_W; e += nt) {
+    const int r = e % AED_W, c = e / AED_W;
+    Hp(kw + r, kw + c) = T[c * AED_LD + r];
+  }
+  if (tid == 0) Hp(kw, kw - 1) = s * conj(V[0]);
+  // lines: rows 0..kw-1 of H (H(r, kw..I) V), columns I+1..n-1 of H (V^H H(kw..I, c), taken as the
+  // conjugate row times V), rows of Z.  Two lanes of a wavefront share a line, each forming half of
+  // its outputs while streaming the same inputs; the stores follow the last load of both.
+  constexpr int HALF = AED_W / 2;
+  const int nfa = kw, nfr = n - 1 - I, lines = nfa + nfr + n;
+  for (int task = tid; task < 2 * lines; task += nt) {
+    const int ln = task >> 1, j0 = (task & 1) * HALF;
+    zc* p;
+    long st;
+    bool cj;
+    if (ln < nfa) {
+      p = H + (long)kw * ldh + ln;
+      st = ldh;
+      cj = false;
+    } else if (ln < nfa + nfr) {
+      p = H + (long)(I + 1 + ln - nfa) * ldh + kw;
+      st = 1;
+      cj = true;
+    } else {
+      p = Z + (long)kw * ldz + (ln - nfa - nfr);
+      st = ldz;
+      cj = false;
+    }
+    zc acc[HALF];
+#pragma unroll
+    for (int j = 0; j < HALF; ++j) acc[j] = mk(0.0);
+    for (int k = 0; k < AED_W; ++k) {
+      zc x = p[k * st];
+      if (cj) x.im = -x.im;
+#pragma unroll
+      for (int j = 0; j < HALF; ++j) fma_acc(acc[j], x, V[(j0 + j) * AED_LD + k]);
+    }
+#pragma unroll
+    for (int j = 0; j < HALF; ++j) {
+      if (cj) acc[j].im = -acc[j].im;
+      p[(j0 + j) * st] = acc[j];
+    }
+  }
+  __syncthreads();
+  return AED_W - ns;
 }
 
 // The deferred products read a window's rotations as scalars: every lane of a wave applies the same
@@ -802,6 +1128,10 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
   __shared__ Rot rots[QR_W];
   __shared__ Rot2 rots2[(MS_S + 1) * MS_NB];  // row s: step s of the window; row MS_S: next window's step 0
   __shared__ zc shifts[MS_NB], betas[MS_NB];
+  __shared__ zc aed_hv[2 * AED_W];
+  __shared__ int aed_sh[2];
+  static_assert(2 * AED_LD * AED_W <= MS_LD * (MS_LEN + 1), "the deflation window fits the window buffer");
+  static_assert(AED_W + 1 < MS_NMIN, "the deflation window's spike lies inside the active block");
   __shared__ int s_k, s_m;
   __shared__ zc s_t, s_v1;
   __shared__ double s_v2;
@@ -847,6 +1177,8 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
 
   const int itmax = 30 * max(10, ihi - ilo + 1);
   int kdefl = 0, its = 0, ms_fail = 0, lastL = -1, lastI = -1;
+  bool aed_ready = false;  // the shifts of a deflation window taken at I = aed_I await their sweep
+  int aed_I = -1, aed_nsh = 0;
   int I = ihi, L = ilo;
   while (I >= ilo) {
     // ---- look for a single small subdiagonal element (largest k in [L+1, I]) ----
@@ -877,12 +1209,25 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
     const int m = I - L + 1;
     if (L != lastL || I != lastI) ms_fail = 0;
     if (m >= MS_NMIN && n <= kScratch && ms_fail < 3) {
+      if (!(aed_ready && aed_I == I)) {
+        // ================= aggressive early deflation =================
+        const int nd = aed_window(H, ldh, Z, ldz, n, I, MB, aed_hv, shifts, aed_sh, smlnum);
+        aed_nsh = (nd >= 0) ? aed_sh[1] : 0;
+        // when many deflated, the next window is examined before a sweep is spent (zlaqr0's rule:
+        // sweep only when at most 14 % of the window deflated)
+        aed_ready = nd * 100 <= AED_W * 14;
+        aed_I = I - max(nd, 0);
+        if (nd > 0) real_subdiag(H, ldh, Z, ldz, n, L, I, MB);
+        continue;
+      }
+      aed_ready = false;
       // ================= multishift sweep =================
       lastL = L;
       lastI = I;
       ++ms_fail;
-      const int nb = MS_NB;
-      if (tid < 64) {
+      // the window's undeflated eigenvalues, else those of the trailing MS_NB x MS_NB block
+      const int nb = (aed_nsh > 0) ? aed_nsh : MS_NB;
+      if (tid < 64 && aed_nsh == 0) {
         zc* S = MB;
         for (int e = tid; e < nb * nb; e += 64) {
           const int r = e % nb, c = e / nb;
@@ -895,6 +1240,7 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
         }
       }
       __syncthreads();
+      aed_nsh = 0;
       const int T_total = (I - 1 - L) + 2 * (nb - 1) + 1;  // steps t = 0 .. T_total - 1
       for (int t0 = 0; t0 < T_total; t0 += MS_S) {
         const int kbase = L + t0 - 2 * (MS_NB - 1);

@@ -65,7 +65,7 @@ def check_eig(A, w, V, tol_res=None):
         assert np.min(np.abs(w - x)) <= 1e-10 * max(nA, 1.0)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 10, 33, 50, 98, 196])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 33, 50, 98, 196, 450])
 def test_zgeev_random(ctx, n):
     rng = np.random.default_rng(n)
     A = crand(rng, n, n)
@@ -82,6 +82,19 @@ def test_zgeev_structured(ctx):
     w, V = sa.zgeev(ctx, D)
     check_eig(D, w, V)
     assert np.linalg.matrix_rank(V) == 5
+
+
+def test_zgeev_repeated_clusters(ctx):
+    # a normal matrix with 12 eigenvalues of multiplicity 25 each (n = 300): the deflation window sees
+    # whole clusters converge together, and the Schur basis inside a cluster is arbitrary
+    rng = np.random.default_rng(4)
+    n = 300
+    vals = np.repeat(crand(rng, 12), 25)
+    Q, _ = np.linalg.qr(crand(rng, n, n))
+    A = (Q * vals[None, :]) @ Q.conj().T
+    w, V = sa.zgeev(ctx, A)
+    check_eig(A, w, V, tol_res=1e-12)
+    assert np.max(np.abs(np.sort_complex(w) - np.sort_complex(vals))) < 1e-10
 
 
 def test_zgeev_uniform_layer_phi(ctx):
