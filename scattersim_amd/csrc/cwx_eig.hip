@@ -484,7 +484,7 @@ __device__ inline bool two_small(const zc* H, int ldh, int m, zc t) {
 constexpr int MS_NB = 16;                               // bulges (shifts) per sweep
 constexpr int MS_S = 16;                                // chase steps per window
 constexpr int MS_LEN = MS_S + 2 * (MS_NB - 1) + 2;      // rows spanned by one window (48)
-constexpr int MS_NMIN = 72;                             // smallest active block swept this way
+constexpr int MS_NMIN = 48;                             // smallest active block swept this way
 // Leading dimensions of the LDS window blocks: a column stride of 4 (mod 32) dwords puts the 16-byte
 // accesses of 8 consecutive lanes walking along a row into distinct bank groups.
 constexpr int MS_LD = MS_LEN + 1;                       // 49 complex = 196 dwords

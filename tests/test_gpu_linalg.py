@@ -153,7 +153,7 @@ def test_batched_dev_entry_points(ctx):
 def test_zgeev_large_batch_matches_small_batch(ctx):
     # A batch of at least two matrices per CU runs the QR with two smaller workgroups per CU; the
     # arithmetic of every matrix is the same either way, so results are bit-identical to solving
-    # the matrices a few at a time.  n = 100 takes the multishift sweeps (active block >= 72).
+    # the matrices a few at a time.  n = 100 takes the deflation windows and multishift sweeps (active block >= 48).
     rng = np.random.default_rng(21)
     n, b = 100, 640
     A = crand(rng, b, n, n)
