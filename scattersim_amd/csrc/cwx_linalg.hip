@@ -641,7 +641,8 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
 // The interchanges compose to B'(i, :) = B(src(i), :); src is formed in LDS by one lane (n dependent
 // steps, once per matrix), then columns pass through LDS in chunks of ch: read along the column and
 // written back permuted, both coalesced, instead of n dependent swaps of scattered 16-byte elements
-// per column.  One workgroup per matrix; dynamic LDS: n ints, then ch x n complex.
+// per column.  Workgroups (matrix, y) take the chunks y, y + gridDim.y, ... of their matrix, each forming
+// the map itself; dynamic LDS: ch x n complex, then n ints.
 constexpr int SWP_THREADS = 256;
 __global__ __launch_bounds__(SWP_THREADS) void k_laswp(int n, int nrhs, int ch, MatSet Bset, int ldb, IntSet pset) {
   extern __shared__ __align__(16) unsigned char swp_lds[];
@@ -663,7 +664,7 @@ __global__ __launch_bounds__(SWP_THREADS) void k_laswp(int n, int nrhs, int ch, 
     }
   }
   __syncthreads();
-  for (int c0 = 0; c0 < nrhs; c0 += ch) {
+  for (int c0 = blockIdx.y * ch; c0 < nrhs; c0 += gridDim.y * ch) {
     const int cn = min(ch, nrhs - c0);
     for (int e = threadIdx.x; e < cn * n; e += blockDim.x) {
       const int c = e / n, r = e % n;
@@ -777,7 +778,10 @@ void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int 
     CWX_CHECK(sizeof(int) * (size_t)n + sizeof(zc) * (size_t)n <= 64 * 1024, "zgetrs: n too large for the LDS row map");
     const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
     const size_t lds = sizeof(zc) * (size_t)ch * n + sizeof(int) * (size_t)n;
-    hipLaunchKernelGGL(k_laswp, dim3(batch), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
+    // enough workgroups to spread the column chunks over the device (about 2048 in all)
+    const int chunks = (nrhs + ch - 1) / ch;
+    const int gy = std::max(1, std::min({chunks, 65535, 2048 / batch}));
+    hipLaunchKernelGGL(k_laswp, dim3(batch, gy), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
   }
   // Blocks of TS_NB rows: inside a block, diagonal LU_NB blocks and updates of the block's own rows;
   // the rows below (above) are updated once per block, which bounds the passes over B.
