@@ -113,6 +113,30 @@ def test_zgeev_uniform_layer_phi(ctx):
     assert np.max(np.abs(np.sort_complex(w) - expect)) < 1e-12
 
 
+def test_zgeev_degenerate_layer_phi_large(ctx):
+    # Phi of a uniform layer at 9x9 orders (n = 324): every eigenvalue is exactly doubly degenerate, so whole
+    # pairs meet in the deflation windows and the reordering swaps equal diagonal entries
+    p = o.VolumeParams(lam=0.5, size=(4, 4, 1), coef=(9, 9), n=(1.0, 1.0))
+    s = o.setup(np.full((1, 16, 16), 1.4 + 0.01j), p)
+    Nf, Nif = o.layer_fourier(s.layers[0], s.M0, s.M1)
+    Phi = o.phi_matrix(Nf, Nif, s.up, s.wq, s.k)
+    w, V = sa.zgeev(ctx, Phi)
+    check_eig(Phi, w, V)
+    assert np.linalg.matrix_rank(V, tol=1e-8) == Phi.shape[0]
+
+
+def test_zgeev_split_hessenberg(ctx):
+    # a block upper-triangular matrix (three diagonal blocks of 60 coupled only upwards) keeps exact zeros on
+    # its Hessenberg subdiagonal, so the active block splits while deflation windows and sweeps run
+    rng = np.random.default_rng(9)
+    n = 180
+    A = np.triu(crand(rng, n, n))
+    for b0 in (0, 60, 120):
+        A[b0:b0 + 60, b0:b0 + 60] = crand(rng, 60, 60)
+    w, V = sa.zgeev(ctx, A)
+    check_eig(A, w, V)
+
+
 def test_batched_dev_entry_points(ctx):
     rng = np.random.default_rng(11)
     n, b = 40, 5
