@@ -390,7 +390,7 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-field", action="store_true")
-    ap.add_argument("--sweep-count", type=int, default=512, help="instances of the batched sweep leg")
+    ap.add_argument("--sweep-count", type=int, default=2048, help="instances of the batched sweep leg (2048: the whole sweep)")
     ap.add_argument("--sweep-plain", type=int, default=24, help="of which solved one at a time")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the layer-stack, many-field and compact legs")
