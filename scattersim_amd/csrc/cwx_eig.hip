@@ -4,10 +4,13 @@
 // reference calls once per layer (ScatterVolume.cpp:255-262).  Steps, one workgroup per matrix:
 //   1. Householder reduction to upper Hessenberg form          (zgehd2 algorithm)
 //   2. explicit formation of the unitary factor Q                (zung2r algorithm)
-//   3. complex single-shift QR with Wilkinson / exceptional shifts and the Ahues-Kahan deflation test
-//      (zlahqr algorithm) giving the Schur form T and Schur vectors Z = Q * (QR rotations).
-//      The bulge is chased through a (W+2) x (W+3) window of H held in LDS; the W rotations of a
-//      window are then applied to the rest of H and to Z from registers, one matrix line per lane.
+//   3. Schur form T and Schur vectors Z = Q * (QR transformations): on active blocks of 48 rows and more,
+//      aggressive early deflation on a trailing 16 x 16 window (zlaqr3 algorithm) alternating with
+//      sweeps of 16 chained single-shift bulges whose shifts are the window's undeflated eigenvalues
+//      (zlaqr5-style multishift); below that, single-shift QR with Wilkinson / exceptional shifts and
+//      the Ahues-Kahan deflation test (zlahqr algorithm).  Bulges are chased through windows of H held
+//      in LDS; each window's rotations are then applied to the rest of H and to Z from registers, one
+//      matrix line per lane.
 //   4. eigenvectors of T by back-substitution (ztrevc algorithm), one wavefront per eigenvector
 //   5. V = Z X on the GEMM kernel, then zgeev's normalisation (unit 2-norm, largest component real).
 #include "cwx_common.h"
