@@ -596,11 +596,15 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
 }
 
 // Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
-// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.
+// solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.  The interchanges
+// (applied in order) are composed once per workgroup into a gather over the rows they touch: rows
+// j0..j0+nb-1 and the pivot rows below the panel, at most 2 nb.  Each lane then loads all of them at
+// once and stores them permuted, instead of nb dependent load/store round trips.
 __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatSet Aset, int lda,
                                                      IntSet pset) {
   __shared__ zc L11[LU_NB][LU_NB];
-  __shared__ int piv[LU_NB];
+  __shared__ int trow[2 * LU_NB], tsrc[2 * LU_NB];
+  __shared__ int s_nt;
   const int b = blockIdx.y;
   zc* A = Aset.at(b);
   const int* ipiv = pset.at(b);
@@ -608,21 +612,57 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
     int r = e % nb, c = e / nb;
     L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
   }
-  for (int e = threadIdx.x; e < nb; e += blockDim.x) piv[e] = ipiv[j0 + e];
+  {
+    // Lane t holds touched row t (my_row) and the original row whose value ends in it (my_src): the
+    // panel's rows first, then each pivot row below the panel when first met (pivots are >= their
+    // row).  A lookup is a ballot and an interchange two lane reads, so the composition is cheap.
+    const int lane = threadIdx.x;
+    int my_row = (lane < nb) ? j0 + lane : -1, my_src = my_row;
+    const int pv = (lane < nb) ? ipiv[j0 + lane] : 0;
+    int nt = nb;
+    for (int jj = 0; jj < nb; ++jj) {
+      const int p = __builtin_amdgcn_readlane(pv, jj);
+      if (p == j0 + jj) continue;
+      int tp = p - j0;
+      if (p >= j0 + nb) {
+        const unsigned long long hit = __ballot(my_row == p);
+        if (hit) {
+          tp = __ffsll((long long)hit) - 1;
+        } else {
+          tp = nt++;
+          if (lane == tp) my_row = my_src = p;
+        }
+      }
+      const int sj = __builtin_amdgcn_readlane(my_src, jj), sp = __builtin_amdgcn_readlane(my_src, tp);
+      if (lane == jj) my_src = sp;
+      else if (lane == tp) my_src = sj;
+    }
+    trow[lane] = my_row;
+    tsrc[lane] = my_src;
+    if (lane == 0) s_nt = nt;
+  }
   __syncthreads();
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   // columns [0, j0) and [j0+nb, n)
   if (c >= j0) c += nb;
   if (c >= n) return;
   zc* col = A + (long)c * lda;
-  for (int jj = 0; jj < nb; ++jj) {
-    int p = piv[jj], j = j0 + jj;
-    if (p != j) { zc t = col[j]; col[j] = col[p]; col[p] = t; }
-  }
-  if (c < j0) return;
-  zc x[LU_NB];
+  const int ne = s_nt - nb;  // pivot rows below the panel
+  zc x[LU_NB], w[LU_NB];      // the panel's rows, then those pivot rows
 #pragma unroll
-  for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[j0 + r] : mk(0.0);
+  for (int t = 0; t < LU_NB; ++t) {
+    x[t] = (t < nb) ? col[tsrc[t]] : mk(0.0);
+    w[t] = (t < ne) ? col[tsrc[nb + t]] : mk(0.0);
+  }
+#pragma unroll
+  for (int t = 0; t < LU_NB; ++t)
+    if (t < ne) col[trow[nb + t]] = w[t];
+  if (c < j0) {
+#pragma unroll
+    for (int t = 0; t < LU_NB; ++t)
+      if (t < nb) col[j0 + t] = x[t];
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < LU_NB; ++r) {
 #pragma unroll
