@@ -760,6 +760,87 @@ __global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, Ma
     if (r < nb) col[r] = x[r];
 }
 
+// Forward and back substitution with the LU factors for one right-hand side (its row interchanges already
+// applied), one workgroup per matrix, x held in LDS.  Blocks of 32 rows: one wavefront solves the
+// diagonal block with its rows in registers (lane r holds row r), then all lanes update the rows still
+// to come with the block's 32 values.  Replaces two launches per block of the many-column path.
+constexpr int GS_THREADS = 512, GS_MAXN = 4000;
+// Lane src's value in every lane (src uniform): scalar reads, no LDS round trip.
+__device__ inline zc shfl_z(zc v, int src) {
+  const double re = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.re), src),
+                                     __builtin_amdgcn_readlane(__double2loint(v.re), src));
+  const double im = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.im), src),
+                                     __builtin_amdgcn_readlane(__double2loint(v.im), src));
+  return mk(re, im);
+}
+
+__global__ __launch_bounds__(GS_THREADS) void k_getrs1(int n, MatSet LUset, int lda, MatSet Bset) {
+  extern __shared__ __align__(16) unsigned char gs_lds[];
+  zc* xs = reinterpret_cast<zc*>(gs_lds);
+  const zc* LU = LUset.at(blockIdx.x);
+  zc* B = Bset.at(blockIdx.x);
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int r = tid; r < n; r += GS_THREADS) xs[r] = B[r];
+  __syncthreads();
+  // L (unit lower), blocks from the top
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int nb = min(LU_NB, n - j0);
+    if (tid < 64) {
+      zc lrow[LU_NB];
+#pragma unroll
+      for (int q = 0; q < LU_NB; ++q) lrow[q] = (q < lane && lane < nb) ? LU[(long)(j0 + q) * lda + j0 + lane] : mk(0.0);
+      zc xr = (lane < nb) ? xs[j0 + lane] : mk(0.0);
+#pragma unroll
+      for (int q = 0; q < LU_NB - 1; ++q)
+        if (q < nb - 1) xr = xr - lrow[q] * shfl_z(xr, q);
+      if (lane < nb) xs[j0 + lane] = xr;
+    }
+    __syncthreads();
+    for (int r = j0 + nb + tid; r < n; r += GS_THREADS) {
+      // all loads of a row are independent and in flight together; two partial sums
+      zc acc0 = mk(0.0), acc1 = mk(0.0);
+#pragma unroll
+      for (int q = 0; q < LU_NB; q += 2) {
+        if (q < nb) fma_acc(acc0, LU[(long)(j0 + q) * lda + r], xs[j0 + q]);
+        if (q + 1 < nb) fma_acc(acc1, LU[(long)(j0 + q + 1) * lda + r], xs[j0 + q + 1]);
+      }
+      xs[r] = xs[r] - (acc0 + acc1);
+    }
+    __syncthreads();
+  }
+  // U (upper, with its diagonal), blocks from the bottom
+  for (int j1 = n; j1 > 0; j1 -= LU_NB) {
+    const int j0 = max(0, j1 - LU_NB), nb = j1 - j0;
+    if (tid < 64) {
+      zc urow[LU_NB];
+#pragma unroll
+      for (int q = 0; q < LU_NB; ++q) urow[q] = (lane <= q && q < nb) ? LU[(long)(j0 + q) * lda + j0 + lane] : mk(0.0);
+      zc xr = (lane < nb) ? xs[j0 + lane] : mk(0.0);
+#pragma unroll
+      for (int q = LU_NB - 1; q >= 0; --q) {
+        if (q < nb) {
+          if (lane == q) xr = zdiv(xr, urow[q]);
+          const zc xq = shfl_z(xr, q);
+          if (lane < q) xr = xr - urow[q] * xq;
+        }
+      }
+      if (lane < nb) xs[j0 + lane] = xr;
+    }
+    __syncthreads();
+    for (int r = tid; r < j0; r += GS_THREADS) {
+      zc acc0 = mk(0.0), acc1 = mk(0.0);
+#pragma unroll
+      for (int q = 0; q < LU_NB; q += 2) {
+        if (q < nb) fma_acc(acc0, LU[(long)(j0 + q) * lda + r], xs[j0 + q]);
+        if (q + 1 < nb) fma_acc(acc1, LU[(long)(j0 + q + 1) * lda + r], xs[j0 + q + 1]);
+      }
+      xs[r] = xs[r] - (acc0 + acc1);
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += GS_THREADS) B[r] = xs[r];
+}
+
 __global__ void k_fill_int(int n, int* p, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -822,6 +903,11 @@ void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int 
     const int chunks = (nrhs + ch - 1) / ch;
     const int gy = std::max(1, std::min({chunks, 65535, 2048 / batch}));
     hipLaunchKernelGGL(k_laswp, dim3(batch, gy), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
+  }
+  if (nrhs == 1 && n <= GS_MAXN) {
+    hipLaunchKernelGGL(k_getrs1, dim3(batch), dim3(GS_THREADS), sizeof(zc) * (size_t)n, s, n, LU, lda, B);
+    CWX_HIP(hipGetLastError());
+    return;
   }
   // Blocks of TS_NB rows: inside a block, diagonal LU_NB blocks and updates of the block's own rows;
   // the rows below (above) are updated once per block, which bounds the passes over B.
