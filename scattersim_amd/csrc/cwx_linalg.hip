@@ -307,6 +307,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
                                                          IntSet pset, int* info) {
   __shared__ double sv[LU_THREADS / 64];
   __shared__ int si[LU_THREADS / 64];
+  __shared__ zc urow[LU_NB];
   const int b = blockIdx.x;
   zc* A = Aset.at(b);
   int* ipiv = pset.at(b);
@@ -341,16 +342,21 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_panel(int n, int j0, int nb, 
       for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) A[(long)j * lda + r] = A[(long)j * lda + r] * rinv;
     }
     __syncthreads();
-    // rank-1 update of the remaining panel columns
+    // rank-1 update of the remaining panel columns: the pivot row's entries are staged in LDS, and each
+    // row's entries are loaded together, updated and stored together (no load waits on a store)
     const int ncols = j0 + nb - (j + 1);
     if (ncols > 0) {
+      for (int c = threadIdx.x; c < ncols; c += blockDim.x) urow[c] = A[(long)(j + 1 + c) * lda + j];
+      __syncthreads();
       for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
         const zc l = A[(long)j * lda + r];
-        for (int c = j + 1; c < j0 + nb; ++c) {
-          const zc u = A[(long)c * lda + j];
-          zc& a = A[(long)c * lda + r];
-          a = a - l * u;
-        }
+        zc av[LU_NB];
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c)
+          if (c < ncols) av[c] = A[(long)(j + 1 + c) * lda + r];
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c)
+          if (c < ncols) A[(long)(j + 1 + c) * lda + r] = av[c] - l * urow[c];
       }
     }
     __syncthreads();
