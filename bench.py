@@ -324,7 +324,7 @@ def learned_leg(ctx, no_cpu: bool):
     cfg = net.cfg
     count = 256
     layers = cl.disk_layers(cfg, count, seed=987654)
-    cl.solve_orders(ctx, cfg, layers[:16], cfg.low)            # warm-up
+    cl.solve_orders(ctx, cfg, layers, cfg.low)                 # warm-up at full size: workspaces sized
     t0 = time.perf_counter()
     x_low = cl.solve_orders(ctx, cfg, layers, cfg.low)
     t_low = time.perf_counter() - t0
@@ -332,7 +332,7 @@ def learned_leg(ctx, no_cpu: bool):
     x_high = cl.solve_orders(ctx, cfg, layers, cfg.high, chunk=256)
     t_high = time.perf_counter() - t0
     X = cl.features(cfg, layers, x_low)
-    net.predict(ctx, X[:16], x_low[:16])                       # warm-up
+    net.predict(ctx, X, x_low)                                 # warm-up at full size
     t0 = time.perf_counter()
     pred = net.predict(ctx, X, x_low)
     t_inf = time.perf_counter() - t0
