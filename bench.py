@@ -187,7 +187,9 @@ def batched_leg(ctx, count: int, plain_count: int, dev):
         sa.solve_batch(ctx, ptr, [params[i]], shape=([int(Ls[i])], N, N), d_x=d_xp.data_ptr() + i * n6 * 16)
 
     plain(0)                                   # warm-up
-    sa.solve_batch(ctx, d_s.data_ptr(), params[:16], shape=(Ls[:16], N, N), d_x=d_x.data_ptr())
+    # warm-up over several full chunks, so that the timed call finds its workspaces already grown
+    wu = min(count, 512)
+    sa.solve_batch(ctx, d_s.data_ptr(), params[:wu], shape=(Ls[:wu], N, N), d_x=d_x.data_ptr())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sa.solve_batch(ctx, d_s.data_ptr(), params, shape=(Ls, N, N), d_x=d_x.data_ptr())
