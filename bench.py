@@ -317,8 +317,8 @@ PEAK_BF16_TFLOPS = 2500.0
 
 def learned_leg(ctx, no_cpu: bool):
     """The learned closure (c4: 7x7 orders plus permittivity coefficients in, 21x21 R/T out) on 256 fresh
-    random-disk layers: its error and the low-order solution's error against the 21x21 judge solve, and
-    the throughput of low-order solve plus network against the judge.  The network was trained on 20 000
+    random-disk layers: its error and the low-order solution's error against the 21x21 reference solve, and
+    the throughput of low-order solve plus network against that solve.  The network was trained on 20 000
     solver-generated pairs (python -m scattersim_amd.closure) and ships as scattersim_amd/data/closure_c4.npz."""
     import scattersim_amd as sa
     from scattersim_amd import closure as cl
@@ -361,9 +361,9 @@ def learned_leg(ctx, no_cpu: bool):
     out = {"workload": "c4: random-disk layers (128x128, 8 disks, n 1.3-1.7, 4 um cell, 1 um thick, lambda 0.5), "
                        "7x7 -> 21x21 orders", "samples": count,
            "rel_err_closure": float(e_net.mean()), "rel_err_low_order": float(e_base.mean()),
-           "judge_solves_per_s": count / t_high, "low_order_solves_per_s": count / t_low,
+           "high_order_solves_per_s": count / t_high, "low_order_solves_per_s": count / t_low,
            "closure_samples_per_s": count / (t_low + t_inf),
-           "closure_over_judge": (count / (t_low + t_inf)) / (count / t_high),
+           "closure_over_high_order": (count / (t_low + t_inf)) / (count / t_high),
            "params": int(sum(int(np.prod(w.shape)) for w in net.weights)),
            "forward_roofline": {"kernel": "k_gemm_bf16_kc (3 layers, batch 16384)", "bound": "mfma bf16",
                                 "achieved": fl / t_fw / 1e12, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",

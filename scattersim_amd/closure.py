@@ -2,7 +2,7 @@
 permittivity Fourier coefficients to the high-order reflection and transmission coefficients.
 
 No counterpart in the reference, which only has the plain solve.  The network is trained on pairs the
-solver generates (low orders in, the high-order solve as target and judge) and runs entirely in the HIP
+solver generates (low orders in, the high-order solve as target and reference) and runs entirely in the HIP
 kernels of libcwx: BF16 GEMMs with FP32 accumulation on the matrix cores, GELU, the mean-squared error
 and Adam (``cwx_bf16_gemm``, ``cwx_mlp_*``).  Host code only prepares features and moves data.
 
@@ -162,7 +162,7 @@ class ClosureConfig:
     r_range: tuple = (0.1, 0.3)     # um
     n_range: tuple = (1.3, 1.7)
     low: int = 7                    # orders per axis of the input solve
-    high: int = 21                  # orders per axis of the judge
+    high: int = 21                  # orders per axis of the high-order reference solve
     band: int = 15                  # central orders per axis the network predicts
     eps_orders: int = 13            # permittivity Fourier coefficients per axis in the input
     hidden: tuple = (256, 256)

@@ -112,8 +112,8 @@ def test_network_gradients_match_oracle(ctx, dims):
 
 
 def test_trained_closure_beats_low_order_solution(ctx):
-    """A small closure (3x3 orders in, 9x9 judge, 32x32 grids) trained on 3000 solves: on 1000 held-out
-    layers its high-order R/T are closer to the judge than the low-order solution alone (what the method
+    """A small closure (3x3 orders in, 9x9 reference solve, 32x32 grids) trained on 3000 solves: on 1000 held-out
+    layers its high-order R/T are closer to the 9x9 solve than the low-order solution alone (what the method
     gains from the network)."""
     cfg = cl.ClosureConfig(grid=32, cell=2.0, thickness=0.5, lam=0.5, disks=4, low=3, high=9, band=7, eps_orders=5,
                            hidden=(256, 256))
