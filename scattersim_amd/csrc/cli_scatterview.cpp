@@ -1,7 +1,7 @@
 // scatterview --nogui: load a .cw, evaluate the xy / xz / yz slices on the GPU and save one of them
 // as an (N, N, 3) complex64 .npy (ScatterView.cpp:678-795).  Options and defaults follow
 // ScatterView.cpp:681-697; like the reference, --nogui exits with status 1 after saving
-// (ScatterView.cpp:794).  The GUI and the CPU evaluator (--cuda -1) are not part of this build.
+// (ScatterView.cpp:794).  The GUI and the CPU evaluator (--cuda -1) are not part of this implementation.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,7 +21,7 @@ int main(int argc, char** argv) {
     std::cout << "Allowed options:\n"
                  "  --input arg (=c.cw)     coupled wave structure to evaluate (also positional)\n"
                  "  --help                  produce help message\n"
-                 "  -c [ --cuda ] arg (=0)  device number (-1 is CPU-only, not provided by this build)\n"
+                 "  -c [ --cuda ] arg (=0)  device number (-1 is CPU-only, not provided by this implementation)\n"
                  "  --nogui                 save an output file without loading the GUI\n"
                  "  -v [ --verbose ]        produce verbose output\n"
                  "  --size arg (=100)       size of the sample being visualized\n"
@@ -50,11 +50,11 @@ int main(int argc, char** argv) {
   }
   const bool verbose = a.has("verbose");
   if (!a.has("nogui")) {
-    std::cerr << "error: the interactive viewer is not part of this build; use --nogui" << std::endl;
+    std::cerr << "error: the interactive viewer is not part of this implementation; use --nogui" << std::endl;
     return 1;
   }
   if (device < 0) {
-    std::cerr << "error: --cuda -1 (CPU evaluation) is not provided by this build" << std::endl;
+    std::cerr << "error: --cuda -1 (CPU evaluation) is not provided by this implementation" << std::endl;
     return 2;
   }
   float plane[3] = {0, 0, 0};

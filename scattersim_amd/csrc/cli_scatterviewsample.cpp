@@ -31,7 +31,7 @@ int main(int argc, char** argv) {
     std::cout << "Allowed options:\n"
                  "  --input arg (=c.cw)       coupled wave structure to evaluate (also positional)\n"
                  "  --help                    produce help message\n"
-                 "  -c [ --cuda ] arg (=0)    device number (-1 is CPU-only, not provided by this build)\n"
+                 "  -c [ --cuda ] arg (=0)    device number (-1 is CPU-only, not provided by this implementation)\n"
                  "  --nogui                   save an output file without loading the GUI\n"
                  "  -v [ --verbose ]          produce verbose output\n"
                  "  --sample                  load a 3D sample stored as a grid (*.npy)\n"
@@ -62,11 +62,11 @@ int main(int argc, char** argv) {
   const bool verbose = a.has("verbose");
   std::cout << "Visualization working." << std::endl;
   if (!a.has("nogui")) {
-    std::cerr << "error: the interactive viewer is not part of this build; use --nogui" << std::endl;
+    std::cerr << "error: the interactive viewer is not part of this implementation; use --nogui" << std::endl;
     return 1;
   }
   if (device < 0) {
-    std::cerr << "error: --cuda -1 (CPU evaluation) is not provided by this build" << std::endl;
+    std::cerr << "error: --cuda -1 (CPU evaluation) is not provided by this implementation" << std::endl;
     return 2;
   }
   if (axis < 0 || axis > 2) {
