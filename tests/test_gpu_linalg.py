@@ -21,7 +21,8 @@ def test_zgemm(ctx, m, n, k):
     assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * np.sqrt(k)
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 3), (31, 1), (32, 4), (33, 2), (100, 1), (300, 7), (1100, 3)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 3), (31, 1), (32, 4), (33, 2), (100, 1), (300, 7), (1100, 3), (1100, 1),
+                                    (2000, 1)])
 def test_zgesv(ctx, n, nrhs):
     rng = np.random.default_rng(n * 7 + nrhs)
     A = crand(rng, n, n) + 2 * np.eye(n)
