@@ -40,6 +40,21 @@ def test_batch_equals_single_solves(ctx, eig_mode):
         sol.free()
 
 
+def test_large_batch_equals_single_solves(ctx):
+    # about 1 750 layers at 9x9 orders (162 x 162 eigenproblems): a batch this large takes the launch
+    # shapes chosen for many matrices (two QR workgroups per CU, deflation windows, Hessenberg panels),
+    # which must leave every instance's arithmetic as in its own solve
+    samples, params = instances(11, 700, M=(9, 9), grid=12)
+    for p in params:
+        p["eig_mode"] = 1
+    x, info, _ = sa.solve_batch(ctx, samples, params)
+    assert np.all(info == 0)
+    for i in (0, 351, 699):
+        sol = sa.scattervolume(ctx, samples[i], **params[i])
+        assert np.array_equal(sol.x, x[i]), i
+        sol.free()
+
+
 def test_batch_against_oracle_and_chunking(ctx):
     samples, params = instances(3, 6, M=(3, 3), grid=10)
     x_all, info, _ = sa.solve_batch(ctx, samples, params)
