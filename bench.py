@@ -125,7 +125,7 @@ def field_c3_leg(ctx, stream, dev):
     from scattersim_amd import workloads
     sample, opts, _ = workloads.plain_small()
     sol = sa.scattervolume(ctx, sample, **opts)
-    g = dict(x0=0.0, y0=0.0, z0=-1.0 + 1.0 / 128, dx=4.0 / 512, dy=4.0 / 512, dz=2.0 / 128, nx=512, ny=512, nz=128)
+    g = workloads.field_c3_grid()
     res = field_leg(ctx, sol, g, stream, dev)
     sol.free()
     res["workload"] = "plain_small solution on 512x512x128 (z in [-1, 1] um)"

@@ -84,6 +84,20 @@ struct Error : std::runtime_error {
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950), queried once per device.
+inline size_t lds_limit() {
+  static size_t cached[64] = {};
+  int dev = 0;
+  CWX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return 64 * 1024;
+  if (!cached[dev]) {
+    int v = 0;
+    CWX_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cached[dev] = v > 0 ? (size_t)v : 64 * 1024;
+  }
+  return cached[dev];
+}
+
 // Device buffer owned by the library (freed on destruction).
 template <typename T>
 struct DevBuf {

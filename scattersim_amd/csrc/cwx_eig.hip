@@ -1644,7 +1644,8 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   work.ensure(n, batch);
   const long nn = (long)n * n;
   const size_t sh2 = sizeof(zc) * (2 * (size_t)n + HH_THREADS);
-  CWX_CHECK(sh2 + sizeof(zc) * (HH_THREADS / 64 + 2 * HB) <= 64 * 1024, "zgeev: n too large for the LDS work vectors");
+  const size_t lds = lds_limit();
+  CWX_CHECK(sh2 + sizeof(zc) * (HH_THREADS / 64 + 2 * HB) <= lds, "zgeev: n too large for the LDS work vectors");
   // ---- Hessenberg reduction: blocked panels, then the last columns unblocked ----
   const long npan_max = n / HB + 1;
   const long sY = (long)n * HB, sT = npan_max * HB * HB, sWk = 2L * n * HB;
@@ -1703,9 +1704,9 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     hipLaunchKernelGGL(k_hqr<EIG_THREADS>, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n,
                        nn, W, sW, info, work.ring.p);
   const double smlnum = kSafmin * ((double)n / kUlp);
-  // one wavefront per eigenvector, as many per workgroup as fit 64 KiB of LDS
+  // one wavefront per eigenvector, as many per workgroup (up to 4) as fit 64 KiB of LDS
   const int waves = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / (sizeof(zc) * n)));
-  CWX_CHECK(sizeof(zc) * (size_t)n <= 64 * 1024, "zgeev: n too large for the LDS eigenvector buffer");
+  CWX_CHECK(sizeof(zc) * (size_t)n <= lds, "zgeev: n too large for the LDS eigenvector buffer");
   hipLaunchKernelGGL(k_trevc, dim3(cdiv(n, waves), batch), dim3(64 * waves),
                      sizeof(zc) * waves * (size_t)n, s, n, A, lda, sA, work.X.p, n, nn, smlnum);
   zgemm_batched(OpN, OpN, n, n, n, mk(1.0), work.Z.p, n, nn, work.X.p, n, nn, mk(0.0), V, ldv, sV,

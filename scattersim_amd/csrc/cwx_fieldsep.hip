@@ -414,7 +414,7 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
     return;
   }
   const size_t sh_coef = sizeof(zc) * 4 * (size_t)MF;
-  CWX_CHECK(sh_coef <= 64 * 1024, "too many orders for the separable evaluator");
+  CWX_CHECK(sh_coef <= lds_limit(), "too many orders for the separable evaluator");
   hipLaunchKernelGGL(k_sep_coeffs, dim3(cdiv(3 * MF, 256), g.nz), dim3(256), sh_coef, s, a, g, w.C.p);
   const int ntab = g.nx * st.M0 + g.ny * st.M1;
   hipLaunchKernelGGL(k_sep_tables, dim3(cdiv(ntab, 256)), dim3(256), 0, s, a, g, w.X.p, w.Y.p);
@@ -423,7 +423,7 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
   auto lds = [&](int tx) { return sizeof(zc) * (3 * (size_t)MF + tx * st.M0 + TY * st.M1 + 3 * tx * st.M1); };
   while (TX > 16 && lds(TX) > 64 * 1024) TX /= 2;
   const size_t sh = lds(TX);
-  CWX_CHECK(sh <= 64 * 1024, "too many orders for the separable evaluator tile");
+  CWX_CHECK(sh <= lds_limit(), "too many orders for the separable evaluator tile");
   dim3 grid(cdiv(g.nx, TX), cdiv(g.ny, TY), g.nz);
   if (out_f64)
     hipLaunchKernelGGL((k_sep_synth<double2>), grid, dim3(256), sh, s, a, g, w.C.p, w.X.p, w.Y.p, (double2*)out, TX);

@@ -900,9 +900,12 @@ void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int 
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
   {
-    // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map
-    const size_t avail = 64 * 1024 - sizeof(int) * (size_t)n;
-    CWX_CHECK(sizeof(int) * (size_t)n + sizeof(zc) * (size_t)n <= 64 * 1024, "zgetrs: n too large for the LDS row map");
+    // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map (the device's
+    // whole LDS allowance when one column does not fit in 64 KiB)
+    const size_t need = (sizeof(int) + sizeof(zc)) * (size_t)n;
+    const size_t budget = need <= 64 * 1024 ? 64 * 1024 : lds_limit();
+    CWX_CHECK(need <= budget, "zgetrs: n too large for the LDS row map");
+    const size_t avail = budget - sizeof(int) * (size_t)n;
     const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
     const size_t lds = sizeof(zc) * (size_t)ch * n + sizeof(int) * (size_t)n;
     // enough workgroups to spread the column chunks over the device (about 2048 in all)

@@ -253,14 +253,15 @@ __device__ inline zc phi_Q(const PhiLocal& a, const zc* NF, int rr, int cc) {
   return diag ? mk(upr * wqr * a.kinv) : mk(0.0);
 }
 
-// full Phi (4MF x 4MF, column-major) per layer
-__global__ void k_phi_full(InstArgs ia, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* Phi) {
-  const int l = blockIdx.y;
+// full Phi (4MF x 4MF, column-major) per layer; with a list, slot b holds layer list[b]
+__global__ void k_phi_full(InstArgs ia, const zc* __restrict__ NF, const zc* __restrict__ NI, zc* Phi,
+                           const int* __restrict__ list) {
+  const int l = list ? list[blockIdx.y] : blockIdx.y;
   const PhiLocal a = phi_local(ia, l);
   const int n = 4 * a.MF, n2 = 2 * a.MF;
   NF += (long)l * a.MF;
   NI += (long)l * a.MF;
-  Phi += (long)l * n * n;
+  Phi += (long)blockIdx.y * n * n;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
     const int r = (int)(e % n), c = (int)(e / n);
     zc v = mk(0.0);
@@ -319,23 +320,60 @@ __device__ void bitonic_sort(zc* val, int* idx, int npow2) {
   __syncthreads();
 }
 
-constexpr int SORT_MAX = 2048;
+constexpr int SORT_MAX = 4096;      // largest 4 M0 M1 (two sort capacities: 2048 and 4096 entries)
+// The reduced (P Q) form takes gamma = +-sqrt(mu).  A pair with |gamma| below this fraction of the
+// layer's largest |gamma| is nearly defective (gamma -> 0 merges the pair), and the layer is solved
+// through the full 4MF Phi instead, the reference's own formulation.
+constexpr double kReducedTau = 1e-4;
 
 // Full mode: eigenvalues W (n4) of Phi -> source column of each sorted, interleaved position
 // (ScatterVolume.cpp:190-230).  Reduced mode: mu (n2) of P Q -> down members gamma = +-sqrt(mu),
-// sorted descending; src[2i] = pair, src[2i+1] = pair | sign bit.
+// sorted descending; src[2i] = pair, src[2i+1] = pair | sign bit; unsafe[l] = 1 when the layer's
+// smallest |gamma| is below kReducedTau times its largest, or any gamma is not finite.  Workgroup b
+// sorts the eigenvalues at slot b of W into layer list[b] (layer b without a list).
+template <int CAP>
 __global__ __launch_bounds__(256) void k_sort_modes(int n4, int reduced, const zc* __restrict__ W,
-                                                    zc* __restrict__ gam, int* __restrict__ src) {
-  __shared__ zc val[SORT_MAX];
-  __shared__ int idx[SORT_MAX];
-  const int l = blockIdx.x;
+                                                    zc* __restrict__ gam, int* __restrict__ src,
+                                                    const int* __restrict__ list, int* __restrict__ unsafe) {
+  __shared__ zc val[CAP];
+  __shared__ int idx[CAP];
+  __shared__ double s_ext[2][4];
+  const int l = list ? list[blockIdx.x] : blockIdx.x;
   const int n2 = n4 / 2;
   const int cnt = reduced ? n2 : n4;
-  W += (long)l * (reduced ? n2 : n4);
+  W += (long)blockIdx.x * (reduced ? n2 : n4);
   gam += (long)l * n4;
   src += (long)l * n4;
   int np = 1;
   while (np < cnt) np <<= 1;
+  if (reduced && unsafe) {
+    double gmin = INFINITY, gmax = 0.0;
+    bool bad = false;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const double a = absz(zsqrt(W[i]));
+      bad |= !isfinite(a);
+      gmin = fmin(gmin, a);
+      gmax = fmax(gmax, a);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      gmin = fmin(gmin, __shfl_xor(gmin, off));
+      gmax = fmax(gmax, __shfl_xor(gmax, off));
+    }
+    const int any_bad = __syncthreads_or(bad ? 1 : 0);
+    if ((threadIdx.x & 63) == 0) {
+      s_ext[0][threadIdx.x >> 6] = gmin;
+      s_ext[1][threadIdx.x >> 6] = gmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = s_ext[0][0], b = s_ext[1][0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+        a = fmin(a, s_ext[0][w]);
+        b = fmax(b, s_ext[1][w]);
+      }
+      unsafe[l] = (any_bad || !(a > kReducedTau * b)) ? 1 : 0;
+    }
+  }
   for (int i = threadIdx.x; i < np; i += blockDim.x) {
     if (i < cnt) {
       zc v = W[i];
@@ -371,11 +409,51 @@ __global__ __launch_bounds__(256) void k_sort_modes(int n4, int reduced, const z
   }
 }
 
+// zgeev's normalisation of the reduced-form modes v+- = [a; +-(Q a)/gamma] (unit 2-norm, largest
+// component real; the lowest index wins ties, as k_normalize).  Both members of a pair have the same
+// norm and the same largest component k; for pair p of layer l, ps[l n2 + p] = {1/||v||, v_k of the +
+// member, k}.  One wavefront per pair.
+struct PairScale {
+  double inv_norm;
+  zc vk;
+  int k;
+};
+
+__global__ __launch_bounds__(64) void k_pair_scale(int n2, const zc* __restrict__ V, const zc* __restrict__ QA,
+                                                  const zc* __restrict__ W, PairScale* __restrict__ ps) {
+  const int p = blockIdx.x, l = blockIdx.y, lane = threadIdx.x;
+  const long mm = (long)n2 * n2;
+  const zc* a = V + l * mm + (long)p * n2;
+  const zc* qa = QA + l * mm + (long)p * n2;
+  zc g = zsqrt(W[(long)l * n2 + p]);
+  if (sorter_less(g, -g)) g = -g;  // the down member, as k_sort_modes takes it
+  const double g2 = norm2(g);
+  double sum = 0.0, best = -1.0;
+  int bi = 0;
+  for (int r = lane; r < 2 * n2; r += 64) {
+    const double m2 = r < n2 ? norm2(a[r]) : norm2(qa[r - n2]) / g2;
+    sum += m2;
+    if (m2 > best) { best = m2; bi = r; }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off);
+    const double ob = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bi, off);
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  }
+  if (lane == 0) {
+    const zc vk = bi < n2 ? a[bi] : zdiv(qa[bi - n2], g);
+    ps[(long)l * n2 + p] = PairScale{1.0 / sqrt(sum), vk, bi};
+  }
+}
+
 // Gd/Gc per layer (ScatterVolume.cpp:297-321) from the sorted modes.
 // Full mode: column j of V = Vsrc(:, src[j]).  Reduced: column = [a; +-(Q a)/gamma] with a the
-// eigenvector of P Q for the pair and gamma the down member.
+// eigenvector of P Q for the pair and gamma the down member, scaled by ps as zgeev normalises; a
+// layer with fslot[l] >= 0 was solved through the full Phi instead, its vectors at slot fslot[l] of Vf.
 __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V, const zc* __restrict__ QA,
-                             const zc* __restrict__ gam, const int* __restrict__ src,
+                             const PairScale* __restrict__ ps, const zc* __restrict__ Vf,
+                             const int* __restrict__ fslot, const zc* __restrict__ gam, const int* __restrict__ src,
                              const double* __restrict__ dz, zc* __restrict__ Gd, zc* __restrict__ Gc) {
   const int l = blockIdx.y;
   const int n4 = 4 * ia.MF, n2 = n4 / 2;
@@ -385,15 +463,18 @@ __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V,
   src += (long)l * n4;
   Gd += l * nn;
   Gc += l * nn;
-  const zc* Vl = V + (long)l * (reduced ? (long)n2 * n2 : nn);
-  const zc* QAl = reduced ? QA + (long)l * n2 * n2 : nullptr;
+  const int slot = (reduced && fslot) ? fslot[l] : -1;
+  const bool red = reduced && slot < 0;
+  const zc* Vl = red ? V + (long)l * n2 * n2 : (reduced ? Vf + (long)slot * nn : V + (long)l * nn);
+  const zc* QAl = red ? QA + (long)l * n2 * n2 : nullptr;
+  const PairScale* psl = red ? ps + (long)l * n2 : nullptr;
   const double dzl = dz[l];
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn; e += (long)gridDim.x * blockDim.x) {
     const int r = (int)(e % n4), j = (int)(e / n4);
     const zc g = gam[j];
     const int s = src[j];
     zc v;
-    if (!reduced) {
+    if (!red) {
       v = Vl[(long)s * n4 + r];
     } else {
       const int pair = s & ((1 << 30) - 1);
@@ -405,6 +486,12 @@ __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V,
         v = zdiv(QAl[(long)pair * n2 + (r - n2)], gd);
         if (neg) v = -v;
       }
+      const PairScale sc = psl[pair];
+      zc vk = sc.vk;
+      if (neg && sc.k >= n2) vk = -vk;
+      const zc rot = conj(vk) * (sc.inv_norm / absz(vk));
+      v = v * rot;
+      if (r == sc.k) v.im = 0.0;
     }
     const zc ik = mk(-g.im * k * dzl, g.re * k * dzl);  // i k gamma dz
     if (j & 1) {
@@ -415,6 +502,19 @@ __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V,
       Gc[e] = v * zexp(ik);
     }
   }
+}
+
+// count[b] = number of non-finite entries of x[b] (n entries per instance); one workgroup per instance
+__global__ __launch_bounds__(256) void k_count_nonfinite(int n, const zc* __restrict__ x, int* __restrict__ count) {
+  __shared__ int s_c;
+  if (threadIdx.x == 0) s_c = 0;
+  __syncthreads();
+  const zc* xb = x + (long)blockIdx.x * n;
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) c += (isfinite(xb[i].re) && isfinite(xb[i].im)) ? 0 : 1;
+  if (c) atomicAdd(&s_c, c);
+  __syncthreads();
+  if (threadIdx.x == 0) count[blockIdx.x] = s_c;
 }
 
 __global__ void k_fill(long n, zc* __restrict__ a) {
@@ -818,7 +918,7 @@ Setup build_phi_dev(const VolumeOptions& o, const zc* d_sample, int L, int s1, i
   const long n4 = 4L * MF, n2 = 2L * MF;
   if (blocks == 0)
     hipLaunchKernelGGL(k_phi_full, dim3(grid_for(n4 * n4), L), dim3(256), 0, s, is.args(), F.p, F.p + (long)L * MF,
-                       out);
+                       out, nullptr);
   else
     hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(n2 * n2), L), dim3(256), 0, s, is.args(), F.p,
                        F.p + (long)L * MF, out, out + L * n2 * n2);
@@ -929,7 +1029,7 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
   const int n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF, n6 = 6 * MF;
   const long nn = (long)n4 * n4;
   const int eig_mode = opts[0].eig_mode;
-  CWX_CHECK(n4 <= SORT_MAX, "4*M0*M1 exceeds the supported maximum of 2048");
+  CWX_CHECK(n4 <= SORT_MAX, "4*M0*M1 exceeds the supported maximum of 4096");
   CWX_CHECK(res.full == nullptr || count == 1, "full solutions are kept for single instances only");
   const InstArgs ia = is.args();
   Solution* full = res.full;
@@ -946,15 +1046,26 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
   DevBuf<zc>& gam = full ? full->gam : gam_local;
   gam.alloc((size_t)Ltot * n4);
   DevBuf<int> d_info(Ltot);
-  DevBuf<zc> W, V, QA;
+  DevBuf<zc> W, V, QA, Vf;
+  DevBuf<PairScale> pscale;
+  DevBuf<int> d_fslot, d_flist, d_finfo;
+  std::vector<int> flist;  // reduced mode: layers solved through the full Phi instead
   ctx.w0.ensure(std::max<size_t>((size_t)Ltot * nn, 1));
+  auto sort_modes = [&](int reduced, const zc* Wv, int nl, const int* list, int* unsafe) {
+    if (n4 <= 2048)
+      hipLaunchKernelGGL(k_sort_modes<2048>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, d_src.p, list,
+                         unsafe);
+    else
+      hipLaunchKernelGGL(k_sort_modes<SORT_MAX>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, d_src.p, list,
+                         unsafe);
+  };
   if (eig_mode == 0) {
     zc* Phi = ctx.w0.p;
-    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, NF, NI, Phi);
+    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, NF, NI, Phi, nullptr);
     W.alloc((size_t)Ltot * n4);
     V.alloc((size_t)Ltot * nn);
     zgeev_batched(n4, Phi, n4, nn, W.p, n4, V.p, n4, nn, Ltot, ctx.eig, d_info.p, s);
-    hipLaunchKernelGGL(k_sort_modes, dim3(Ltot), dim3(256), 0, s, n4, 0, W.p, gam.p, d_src.p);
+    sort_modes(0, W.p, Ltot, nullptr, nullptr);
   } else {
     const long mm = (long)n2 * n2;
     zc* P = ctx.w0.p;
@@ -968,18 +1079,46 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
     zgeev_batched(n2, PQ, n2, mm, W.p, n2, V.p, n2, mm, Ltot, ctx.eig, d_info.p, s);
     QA.alloc((size_t)Ltot * mm);
     zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), Q, n2, mm, V.p, n2, mm, mk(0.0), QA.p, n2, mm, Ltot, s);
-    hipLaunchKernelGGL(k_sort_modes, dim3(Ltot), dim3(256), 0, s, n4, 1, W.p, gam.p, d_src.p);
+    DevBuf<int> d_unsafe(Ltot);
+    sort_modes(1, W.p, Ltot, nullptr, d_unsafe.p);
+    pscale.alloc((size_t)Ltot * n2);
+    hipLaunchKernelGGL(k_pair_scale, dim3(n2, Ltot), dim3(64), 0, s, n2, V.p, QA.p, W.p, pscale.p);
+    std::vector<int> hunsafe(Ltot);
+    CWX_HIP(hipMemcpyAsync(hunsafe.data(), d_unsafe.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+    CWX_HIP(hipStreamSynchronize(s));
+    std::vector<int> fslot(Ltot, -1);
+    for (int l = 0; l < Ltot; ++l)
+      if (hunsafe[l]) {
+        fslot[l] = (int)flist.size();
+        flist.push_back(l);
+      }
+    if (!flist.empty()) {
+      // nearly defective pairs: those layers' modes come from zgeev of the full Phi (P, Q are no longer needed)
+      const int nf = (int)flist.size();
+      upload(d_flist, flist, s);
+      upload(d_fslot, fslot, s);
+      zc* Phi = ctx.w0.p;
+      hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), nf), dim3(256), 0, s, ia, NF, NI, Phi, d_flist.p);
+      DevBuf<zc> Wf((size_t)nf * n4);
+      Vf.alloc((size_t)nf * nn);
+      d_finfo.alloc(nf);
+      zgeev_batched(n4, Phi, n4, nn, Wf.p, n4, Vf.p, n4, nn, nf, ctx.eig, d_finfo.p, s);
+      sort_modes(0, Wf.p, nf, d_flist.p, nullptr);
+      CWX_HIP(hipStreamSynchronize(s));  // Wf is released on return
+    }
   }
   DevBuf<zc> Gd_local, Gc_local;
   DevBuf<zc>& Gd = full ? full->Gd : Gd_local;
   DevBuf<zc>& Gc = full ? full->Gc : Gc_local;
   Gd.alloc((size_t)Ltot * nn);
   Gc.alloc((size_t)Ltot * nn);
-  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, V.p, QA.p, gam.p,
-                     d_src.p, is.dz.p, Gd.p, Gc.p);
+  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, V.p, QA.p, pscale.p,
+                     Vf.p, d_fslot.p, gam.p, d_src.p, is.dz.p, Gd.p, Gc.p);
   V.release();
   QA.release();
   W.release();
+  Vf.release();
+  pscale.release();
 
   // ---- 4. LU of every Gd, T_g = Gd_g^-1 Gc_{g-1} (g not first in its instance), and per instance
   //         S = Gc_last T_last ... T_first+1 by a pairwise product tree (ScatterVolume.cpp:327-336) ----
@@ -1136,7 +1275,16 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
   CWX_HIP(hipMemcpyAsync(hlu.data(), d_info_lu.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipMemcpyAsync(hS.data(), infoS.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipMemcpyAsync(hA.data(), infoA.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  std::vector<int> hfinfo(flist.size());
+  if (!flist.empty())
+    CWX_HIP(hipMemcpyAsync(hfinfo.data(), d_finfo.p, sizeof(int) * flist.size(), hipMemcpyDeviceToHost, s));
+  // a solution with a value that is not finite is a failure, whatever the pivots said
+  DevBuf<int> d_nonfinite(count);
+  hipLaunchKernelGGL(k_count_nonfinite, dim3(count), dim3(256), 0, s, n6, x.p, d_nonfinite.p);
+  std::vector<int> hnf(count);
+  CWX_HIP(hipMemcpyAsync(hnf.data(), d_nonfinite.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipStreamSynchronize(s));
+  for (size_t f = 0; f < flist.size(); ++f) hinfo[flist[f]] = hfinfo[f];
   res.info.assign(count, 0);
   res.msg.assign(count, std::string());
   for (int i = 0; i < count; ++i) {
@@ -1158,6 +1306,10 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
     }
     if (!inf && hS[i]) { inf = hS[i]; m = "singular transfer product"; }
     if (!inf && hA[i]) { inf = hA[i]; m = "singular boundary system"; }
+    if (!inf && hnf[i]) {
+      inf = n6 + 1;
+      m = std::to_string(hnf[i]) + " entries of the boundary solution are not finite";
+    }
   }
 }
 
@@ -1460,7 +1612,7 @@ void cw2_read_file(Context& ctx, const std::string& path, Solution& sol) {
   get(size, 24);
   uint32_t L;
   get(&L, 4);
-  CWX_CHECK(m[0] >= 1 && m[1] >= 1 && (long)m[0] * m[1] * 4 <= 2048, path + ": unsupported Fourier orders");
+  CWX_CHECK(m[0] >= 1 && m[1] >= 1 && (long)m[0] * m[1] * 4 <= SORT_MAX, path + ": unsupported Fourier orders");
   CWX_CHECK(L >= 1 && L + 1 == ZL, path + ": layer count does not match the boundaries");
   const int M0 = m[0], M1 = m[1], MF = M0 * M1, n4 = 4 * MF, n6 = 6 * MF;
   const size_t nn = (size_t)n4 * n4;
@@ -1523,7 +1675,7 @@ void cw2_read_file(Context& ctx, const std::string& path, Solution& sol) {
   CWX_HIP(hipMemcpyAsync(NI.p, nif.data(), sizeof(zc) * nif.size(), hipMemcpyHostToDevice, s));
   // Gd / Gc: the mode matrix with the layer phases, by the solve's own kernel (the modes are sorted)
   hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for((long)nn), L), dim3(256), 0, s, is.args(), 0, dV.p, nullptr,
-                     sol.gam.p, src.p, is.dz.p, sol.Gd.p, sol.Gc.p);
+                     nullptr, nullptr, nullptr, sol.gam.p, src.p, is.dz.p, sol.Gd.p, sol.Gc.p);
   CWX_HIP(hipGetLastError());
   finish_solution(sol, o, is, NI.p, s);
   CWX_HIP(hipStreamSynchronize(s));

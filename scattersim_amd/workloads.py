@@ -78,6 +78,32 @@ def _stack_masks(L: int, N: int, cell: float, rng):
     return masks
 
 
+SWEEP_CELL = 12.0
+
+
+def sweep_stacks(seed: int = 0, nstacks: int = 16, N: int = 64):
+    """The seeded stacks (material masks, (L, N, N) bool) of the batched sweep."""
+    rng = np.random.default_rng(seed)
+    return [_stack_masks(int(rng.integers(4, 25)), N, SWEEP_CELL, rng) for _ in range(nstacks)]
+
+
+def sweep_instance(stack: int, lam: float, theta_deg: float, seed: int = 0, nstacks: int = 16, N: int = 64):
+    """One instance of the sweep's family: stack `stack` of sweep_stacks() at wavelength `lam` (um) and
+    incidence `theta_deg` in the xz plane.  Returns (sample (L, N, N) complex128, option dict)."""
+    m = sweep_stacks(seed, nstacks, N)[stack]
+    th = np.deg2rad(theta_deg)
+    sample = np.where(m, 1.5 + 0.004 / lam ** 2, 1.0).astype(np.complex128)
+    return sample, dict(lam=float(lam), direction=(float(np.sin(th)), 0.0, float(np.cos(th))), n=(1.0, 1.0),
+                        size=(SWEEP_CELL, SWEEP_CELL, 0.1 * m.shape[0]), coef=(11, 11))
+
+
+def field_c3_grid():
+    """The mixed-precision field grid: 512 x 512 x 128 points over the 4 x 4 um cell of plain_small, z in
+    [-1, 1] um (plane centres), so that the planes cross the layer and both half-spaces."""
+    return dict(x0=0.0, y0=0.0, z0=-1.0 + 1.0 / 128, dx=4.0 / 512, dy=4.0 / 512, dz=2.0 / 128, nx=512, ny=512,
+                nz=128)
+
+
 def sweep(count: int = 2048, seed: int = 0, nstacks: int = 16, N: int = 64):
     """Batched sweep: 256 wavelengths in [0.4, 0.8] um x 8 incidence angles in [0, 35] deg (instance i is
     wavelength i // 8 at angle i % 8; the first `count` instances are returned).  Instance i uses stack
@@ -87,9 +113,8 @@ def sweep(count: int = 2048, seed: int = 0, nstacks: int = 16, N: int = 64):
 
     Returns (packed, Ls, params): the layers of all instances back to back, (sum L, N, N) complex128;
     the layer count and the option dict of every instance."""
-    rng = np.random.default_rng(seed)
-    cell = 12.0
-    stacks = [_stack_masks(int(rng.integers(4, 25)), N, cell, rng) for _ in range(nstacks)]
+    cell = SWEEP_CELL
+    stacks = sweep_stacks(seed, nstacks, N)
     lams = np.linspace(0.4, 0.8, 256)
     thetas = np.deg2rad(np.linspace(0.0, 35.0, 8))
     Ls = np.array([stacks[i % nstacks].shape[0] for i in range(count)], np.int32)

@@ -63,3 +63,15 @@ def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def multiset_err(a, b) -> float:
+    """Relative distance of two eigenvalue multisets: each value of ``a`` is matched to one of ``b`` by a
+    minimum-cost assignment, so that near-ties in any sort order cannot pair the wrong values."""
+    from scipy.optimize import linear_sum_assignment
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    assert a.shape == b.shape
+    cost = np.abs(a[:, None] - b[None, :])
+    r, c = linear_sum_assignment(cost)
+    return float(np.max(cost[r, c]) / max(np.max(np.abs(b)), 1e-300))
