@@ -304,6 +304,13 @@ int cwx_field_init_solution(cwx_context* ctx, const cwx_solution* sol);
 int cwx_field_evaluate(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz,
                        float x_start, float y_start, float z_start, float x, float y, float z, float d,
                        int N);
+/* Replaces cpu_cw_evaluate_xy / _yz / _xz (cpuEvaluator.cpp:294-407), which scatterview runs for
+ * --cuda -1 (ScatterView.cpp:363-370): the same three slices with the region test selected by tol_mode,
+ * 1 being the CPU evaluator's z >= z_l without the 0.001 tolerance (cpuEvaluator.cpp:303,349,386).  The
+ * sum itself still runs on the GPU; tol_mode 0 is cwx_field_evaluate. */
+int cwx_field_evaluate_rule(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz,
+                            float x_start, float y_start, float z_start, float x, float y, float z, float d, int N,
+                            int tol_mode);
 /* Evaluate a regular grid into device memory: out[iz][iy][ix][3] complex<float> (out_f64 = 0) or
  * complex<double> (out_f64 = 1, FP64 tables only) at x0 + dx*ix etc.  tol_mode 0 uses the GPU
  * region test z >= z_l - 0.001 (gpuEvaluator.cu:34), 1 the CPU one z >= z_l (cpuEvaluator.cpp:303). */

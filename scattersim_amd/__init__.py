@@ -83,11 +83,12 @@ class Context:
     def field_init_solution(self, sol: "Solution"):
         check(lib().cwx_field_init_solution(self.h, sol.h))
 
-    def field_evaluate(self, x_start, y_start, z_start, x, y, z, d, N):
-        """gpu_cw_evaluate: returns (E_xy, E_xz, E_yz), each (N, N, 3) complex64."""
+    def field_evaluate(self, x_start, y_start, z_start, x, y, z, d, N, tol_mode: int = 0):
+        """gpu_cw_evaluate: returns (E_xy, E_xz, E_yz), each (N, N, 3) complex64.  tol_mode 1 takes the
+        region test of cpu_cw_evaluate_* (z >= z_l, cpuEvaluator.cpp:294-407) instead of z >= z_l - 0.001."""
         out = [np.zeros((N, N, 3), np.complex64) for _ in range(3)]
-        check(lib().cwx_field_evaluate(self.h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), x_start, y_start,
-                                       z_start, x, y, z, d, N))
+        check(lib().cwx_field_evaluate_rule(self.h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), x_start, y_start,
+                                            z_start, x, y, z, d, N, int(tol_mode)))
         return tuple(out)
 
     def field_solution_grid_dev(self, sol: "Solution", d_out: int, out_f64: bool, tol_mode: int, x0, y0, z0, dx,

@@ -107,6 +107,7 @@ SIGNATURES = {
     "cwx_field_init_f64": (I, [P, P, I64, P, I, P, P]),
     "cwx_field_init_solution": (I, [P, P]),
     "cwx_field_evaluate": (I, [P, P, P, P, F, F, F, F, F, F, F, I]),
+    "cwx_field_evaluate_rule": (I, [P, P, P, P, F, F, F, F, F, F, F, I, I]),
     "cwx_field_grid_dev": (I, [P, P, I, I, F, F, F, F, F, F, I, I, I]),
     "cwx_field_solution_grid_dev": (I, [P, P, P, I, I, F, F, F, F, F, F, I, I, I]),
 }

@@ -1,7 +1,9 @@
 // scatterview --nogui: load a .cw, evaluate the xy / xz / yz slices on the GPU and save one of them
 // as an (N, N, 3) complex64 .npy (ScatterView.cpp:678-795).  Options and defaults follow
 // ScatterView.cpp:681-697; like the reference, --nogui exits with status 1 after saving
-// (ScatterView.cpp:794).  The GUI and the CPU evaluator (--cuda -1) are not part of this implementation.
+// (ScatterView.cpp:794).  --cuda -1 selects the reference's CPU evaluator (ScatterView.cpp:363-370,
+// cpuEvaluator.cpp:294-407); here that means its region rule (z >= z_l, no tolerance) on device 0, the
+// sum still running on the GPU.  The GUI is not part of this implementation.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,7 +23,7 @@ int main(int argc, char** argv) {
     std::cout << "Allowed options:\n"
                  "  --input arg (=c.cw)     coupled wave structure to evaluate (also positional)\n"
                  "  --help                  produce help message\n"
-                 "  -c [ --cuda ] arg (=0)  device number (-1 is CPU-only, not provided by this implementation)\n"
+                 "  -c [ --cuda ] arg (=0)  device number (-1: the CPU evaluator's region rule z >= z_l, on device 0)\n"
                  "  --nogui                 save an output file without loading the GUI\n"
                  "  -v [ --verbose ]        produce verbose output\n"
                  "  --size arg (=100)       size of the sample being visualized\n"
@@ -53,10 +55,9 @@ int main(int argc, char** argv) {
     std::cerr << "error: the interactive viewer is not part of this implementation; use --nogui" << std::endl;
     return 1;
   }
-  if (device < 0) {
-    std::cerr << "error: --cuda -1 (CPU evaluation) is not provided by this implementation" << std::endl;
-    return 2;
-  }
+  // the reference's CPU path differs from its GPU path only in the region test (cpuEvaluator.cpp:303)
+  const int tol_mode = device < 0 ? 1 : 0;
+  if (device < 0) device = 0;
   float plane[3] = {0, 0, 0};
   if (axis >= 0 && axis <= 2) plane[axis] = slice;
   const auto t0 = std::chrono::system_clock::now();
@@ -123,8 +124,8 @@ int main(int argc, char** argv) {
   const float d = extent / (N - 1);
   const float xs = center[0] - extent / 2, ys = center[1] - extent / 2, zs = center[2] - extent / 2;
   std::vector<cwx_complexf> Exy((size_t)N * N * 3), Exz((size_t)N * N * 3), Eyz((size_t)N * N * 3);
-  if (cwx_field_evaluate(ctx, Exy.data(), Exz.data(), Eyz.data(), xs, ys, zs, plane[0], plane[1], plane[2], d,
-                         (int)N) != CWX_OK) {
+  if (cwx_field_evaluate_rule(ctx, Exy.data(), Exz.data(), Eyz.data(), xs, ys, zs, plane[0], plane[1], plane[2], d,
+                              (int)N, tol_mode) != CWX_OK) {
     std::cerr << "error: " << cwx_last_error() << std::endl;
     return finish(2);
   }

@@ -1059,24 +1059,34 @@ int cwx_field_solution_grid_dev(cwx_context* ctx, const cwx_solution* sol, void*
 
 int cwx_field_evaluate(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz, float x_start,
                        float y_start, float z_start, float x, float y, float z, float d, int N) {
+  return cwx_field_evaluate_rule(ctx, E_xy, E_xz, E_yz, x_start, y_start, z_start, x, y, z, d, N, 0);
+}
+
+int cwx_field_evaluate_rule(cwx_context* ctx, cwx_complexf* E_xy, cwx_complexf* E_xz, cwx_complexf* E_yz,
+                            float x_start, float y_start, float z_start, float x, float y, float z, float d, int N,
+                            int tol_mode) {
   return guard([&]() -> int {
     CWX_CHECK(ctx && E_xy && E_xz && E_yz, "NULL argument");
     CWX_CHECK(N >= 1 && N <= 65535, "bad slice resolution");
+    CWX_CHECK(tol_mode == 0 || tol_mode == 1, "tol_mode must be 0 (z >= z_l - 0.001) or 1 (z >= z_l)");
     if (ctx->c.table.nwaves <= 0) return fail(CWX_ERR_STATE, "no wave table: call cwx_field_init_* first");
     set_device(ctx);
     hipStream_t s = ctx->c.stream;
     const size_t bytes = sizeof(cwx_complexf) * 3 * (size_t)N * N;
     cwx::DevBuf<unsigned char> dE(bytes);
     // kernel_xy: plane z, rows along y (gpuEvaluator.cu:70-89)
-    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y_start, z, d, d, 0.f, N, N, 1}, 0, 0, dE.p, s);
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y_start, z, d, d, 0.f, N, N, 1}, tol_mode, 0, dE.p,
+                             s);
     CWX_HIP(hipMemcpyAsync(E_xy, dE.p, bytes, hipMemcpyDeviceToHost, s));
     CWX_HIP(hipStreamSynchronize(s));
     // kernel_xz: plane y, rows along z
-    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y, z_start, d, 0.f, d, N, 1, N}, 0, 0, dE.p, s);
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x_start, y, z_start, d, 0.f, d, N, 1, N}, tol_mode, 0, dE.p,
+                             s);
     CWX_HIP(hipMemcpyAsync(E_xz, dE.p, bytes, hipMemcpyDeviceToHost, s));
     CWX_HIP(hipStreamSynchronize(s));
     // kernel_yz: plane x, columns along y, rows along z
-    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x, y_start, z_start, 0.f, d, d, 1, N, N}, 0, 0, dE.p, s);
+    cwx::launch_field_direct(ctx->c.table, cwx::GridSpec{x, y_start, z_start, 0.f, d, d, 1, N, N}, tol_mode, 0, dE.p,
+                             s);
     CWX_HIP(hipMemcpyAsync(E_yz, dE.p, bytes, hipMemcpyDeviceToHost, s));
     CWX_HIP(hipStreamSynchronize(s));
     return CWX_OK;

@@ -13,7 +13,9 @@
                  (1, 0, 1), lambda = 1, one interface at z = 0 into 1.4+0.05i): all 4096 incident waves as
                  real (E0.x, E0.y, k.x, k.y, k.z) rows, and for the waves with k.y = 0 and E0.y != 0 (TE)
                  their reflected and transmitted records.
-* samples.npz -- the input grids data/layers.npy, data/circle.npy and 3_npy.npy.
+* samples.npz -- the input grids data/layers.npy, data/circle.npy, 3_npy.npy, data/cylinder.npy
+                 (100 layers of 1 x 100: the M1 = 1 path with a 100-deep chain), data/triangle.npy
+                 (225 x 225, odd) and data/RecBar.npy (stored in Fortran order, kept so).
 The reference's records hold only (E0.x, E0.y, k); for ky = 0 TE waves E0.y is the global Ey.
 """
 import os
@@ -71,7 +73,10 @@ def main(ref):
     np.savez_compressed(os.path.join(here, "samples.npz"),
                         layers=np.load(os.path.join(ref, "data", "layers.npy")),
                         circle=np.load(os.path.join(ref, "data", "circle.npy")),
-                        npy3=np.load(os.path.join(ref, "3_npy.npy")))
+                        npy3=np.load(os.path.join(ref, "3_npy.npy")),
+                        cylinder=np.load(os.path.join(ref, "data", "cylinder.npy")),
+                        triangle=np.load(os.path.join(ref, "data", "triangle.npy")),
+                        recbar=np.load(os.path.join(ref, "data", "RecBar.npy")))
 
 
 if __name__ == "__main__":

@@ -116,12 +116,21 @@ def test_cli_help_and_errors():
 
 
 def test_cli_negative_values_parse(tmp_path):
-    """scattervolume accepts negative numbers as option values (ScatterVolume.cpp:531-535)."""
+    """scattervolume accepts negative numbers as option values (ScatterVolume.cpp:531-535).  Parsing comes
+    first: a value list that loses its negative entry fails there (status 1, "take 3 values"), while a
+    parsed command reaches device creation, which fails without a GPU (status 2, "no HIP device") and
+    solves with one (status 0)."""
     np.save(tmp_path / "s.npy", np.ones((4, 4), np.complex128))
+    r = run("scattervolume", "--sample", str(tmp_path / "s.npy"), "--direction", "0", "1", "--output",
+            str(tmp_path / "bad.cw"))
+    assert r.returncode == 1 and "take 3 values" in r.stderr
     r = run("scattervolume", "--sample", str(tmp_path / "s.npy"), "--direction", "-0.2", "0", "1", "--z", "-1.5",
-            "--output", str(tmp_path / "o.cw"))
-    # without a GPU the tool fails at device creation, after option parsing succeeded
-    assert "error" not in r.stderr or "device" in r.stderr
+            "--ex", "-1", "0", "--ey", "0", "-0.5", "--output", str(tmp_path / "o.cw"))
+    if r.returncode == 0:
+        cw = o.cw_read(str(tmp_path / "o.cw"))
+        assert np.isclose(cw.z[0], -1.5 - 1.0) and cw.Pi[0, 3].real < 0
+    else:
+        assert r.returncode == 2 and "no HIP device" in r.stderr, r.stderr
 
 
 def test_oracle_c_field_restatement():
@@ -146,3 +155,43 @@ def test_oracle_c_field_restatement():
     lib.cw_eval_grid_f32(W32.ctypes.data, zl.ctypes.data, b.ctypes.data, e.ctypes.data, 2, 0, *g, out32.ctypes.data)
     ref32 = o.field_points(W32, zl, b, e, pts, tol=0.001)
     assert rel_err(out32.reshape(-1, 3), ref32) < 1e-4
+
+
+def test_library_order_selection_matches_oracle():
+    """The library's Fourier-order selection and silent cut-off (host code, no device work) against the
+    oracle's select_orders (ScatterVolume.cpp:599-651) over a seeded sweep of --coef / --size / --lambda /
+    --direction / --n and sample shapes, plus the known answers: 21 x 21 at 4 um and lambda 0.5 -> 10 x 10,
+    the in_n[1] quirk of the M1 branch, and the clamp to the pixel counts."""
+    rng = np.random.default_rng(8)
+    cases = [dict(lam=0.5, size=(4.0, 4.0, 1.0), coef=(21, 21)),
+             dict(lam=0.5, size=(4.0, 4.0, 1.0), coef=(21, 21), no_cutoff=True),
+             dict(lam=1.0, size=(10.0, 10.0, 4.0), coef=(), n=(1.0, 1.5)),
+             dict(lam=0.6, size=(3.0, 3.0, 1.0), coef=(49,))]
+    shapes = [(1, 128, 128), (1, 128, 128), (100, 1, 100), (1, 5, 9)]
+    expect = [(10, 10), (21, 21), None, (7, 5)]
+    for _ in range(400):
+        th, ph = rng.uniform(0, 1.2), rng.uniform(0, 2 * np.pi)
+        c = dict(lam=float(rng.uniform(0.3, 1.5)), size=(float(rng.uniform(1, 12)), float(rng.uniform(1, 12)), 1.0),
+                 direction=(float(np.sin(th) * np.cos(ph)), float(np.sin(th) * np.sin(ph)), float(np.cos(th))),
+                 n=(float(rng.uniform(1, 1.6)), float(rng.uniform(1, 1.6))))
+        k = int(rng.integers(0, 3))
+        c["coef"] = () if k == 0 else ((int(rng.integers(1, 40)),) if k == 1 else
+                                       (int(rng.integers(1, 25)), int(rng.integers(1, 25))))
+        cases.append(c)
+        shapes.append((1, int(rng.integers(1, 40)), int(rng.integers(1, 40))))
+        expect.append(None)
+    for c, shp, ex in zip(cases, shapes, expect):
+        p = o.VolumeParams(**c)
+        d = np.asarray(p.direction, float)
+        M0, M1, _ = o.select_orders(p, shp, d / np.linalg.norm(d) * p.n[0])
+        if M0 < 1 or M1 < 1:
+            # the heuristic can cut an axis to no orders at all; the reference then runs on MF = 0, the
+            # library refuses
+            with pytest.raises(_capi.CwxError) as ei:
+                sa.batch_orders([c], [shp[0]], shp[1], shp[2])
+            assert ei.value.code == _capi.CWX_ERR_ARG and "no orders" in str(ei.value)
+            continue
+        got = sa.batch_orders([c], [shp[0]], shp[1], shp[2])
+        assert got == (M0, M1), (c, shp, got, (M0, M1))
+        if ex is not None:
+            assert got == ex

@@ -289,3 +289,35 @@ def test_cli_compact_file_renders_like_full_file(tmp_path):
         assert sc.shape == (32, 32, 3) and sc.dtype == np.complex64
         assert np.array_equal(sf, v)
         assert rel_err(sc, v) < 1e-4, (axis, rel_err(sc, v))
+
+
+def test_cli_scatterviewsample_slices_match_oracle(tmp_path):
+    """scatterviewsample --nogui on a compact file: the slices of all three axes, inside and outside the
+    sample, against the oracle's FP64 sum over its own .cw records at the same points (ScatterViewSample.cpp
+    :354-376 geometry, gpuEvaluator.cu:34 region rule).  complex64 output: 2e-6 of the slice maximum."""
+    import subprocess
+    bin_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "scattersim_amd", "bin")
+    sample = disks_sample(16, 3, 3.0, 3, 1.0, lambda r: 1.4 + 0.02j * r.random(), seed=33)
+    np.save(tmp_path / "s.npy", sample)
+    p = o.VolumeParams(lam=0.7, size=(3.0, 3.0, 1.2), coef=(5, 5), direction=(0.1, 0.0, 1.0))
+    r = subprocess.run([os.path.join(bin_dir, "scattervolume"), "--sample", str(tmp_path / "s.npy"), "--lambda", "0.7",
+                        "--size", "3", "3", "1.2", "--coef", "5", "5", "--direction", "0.1", "0", "1", "--compact",
+                        "--output", str(tmp_path / "c.cw")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = o.build_cw(o.solve(sample, p))
+    W, zl, b, e = o.unpack(ref, np.complex128)
+    N, extent, center, sl = 32, 4.0, (0.3, 0.2, 0.017), 0.21
+    for axis in (0, 1, 2):
+        r = subprocess.run([os.path.join(bin_dir, "scatterviewsample"), "--nogui", "--resolution", "5", "--axis",
+                            str(axis), "--slice", str(sl), "--center", *map(str, center), "--extent", str(extent),
+                            "--input", str(tmp_path / "c.cw"), "--output", str(tmp_path / "sc.npy")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 1, r.stderr
+        got = np.load(tmp_path / "sc.npy").reshape(-1, 3)
+        plane = [0.0, 0.0, 0.0]
+        plane[axis] = sl
+        pts = o.slice_points(axis, N, extent, center, plane)
+        E = o.field_points(W, zl, b, e, pts, tol=0.001)
+        err = rel_err(got, E)
+        print(f"axis {axis}: {err:.2e}", flush=True)
+        assert err < 2e-6

@@ -111,3 +111,30 @@ def test_cw_file_roundtrip(ctx):
     assert np.array_equal(flat, recs)
     assert np.array_equal(W["E0"], recs[:, 0:3]) and np.array_equal(W["k"], recs[:, 3:6])
     assert np.allclose(zl, sol.z)
+
+
+def test_reduced_mode_falls_back_to_full_phi_at_zero_gamma(ctx):
+    """A uniform n = 1 layer between n = 1.2 half-spaces at lambda = 0.5 um in a 4 um cell with 17 x 1
+    orders: orders +-8 have Sx = 1 = n inside the layer, so gamma = 0 there and the reduced form's pair
+    gamma = +-sqrt(mu) is defective.  The layer must go through the full Phi (the reference's zgeev of
+    it), so eigen mode 1 gives exactly mode 0's solution, which matches the oracle.  With a few pixels of
+    n = 1.3 at lambda = 0.49 um nothing grazes and mode 1 keeps the reduced path: its x then differs from
+    mode 0 in the last bits."""
+    sample = np.ones((1, 32, 32), np.complex128)
+    base = dict(size=(4.0, 4.0, 1.0), coef=(17, 1), ex=(0, 0), ey=(1, 0), n=(1.2, 1.2), no_cutoff=True)
+    p = o.VolumeParams(lam=0.5, **base)
+    ref = o.solve(sample, p)
+    assert np.min(np.abs(ref.gammas[0])) == 0.0
+    x0 = gpu_solve(ctx, sample, p, 0).x
+    x1 = gpu_solve(ctx, sample, p, 1).x
+    assert np.all(np.isfinite(x1))
+    assert np.array_equal(x0, x1)
+    assert rel_err(x0, ref.x) < TOL
+    q = o.VolumeParams(lam=0.49, **base)
+    het = sample.copy()
+    het[0, 10:20, 5:9] = 1.3
+    refq = o.solve(het, q)
+    y0 = gpu_solve(ctx, het, q, 0).x
+    y1 = gpu_solve(ctx, het, q, 1).x
+    assert not np.array_equal(y0, y1)
+    assert rel_err(y1, refq.x) < TOL and rel_err(y0, refq.x) < TOL
