@@ -90,6 +90,18 @@ int cwx_zgetrf_batched_dev(cwx_context* ctx, int n, int batch, void* dA, int64_t
                            int64_t strideP, int* h_info);
 int cwx_zgetrs_batched_dev(cwx_context* ctx, int n, int nrhs, int batch, const void* dLU,
                            int64_t strideA, const void* dIpiv, int64_t strideP, void* dB, int64_t strideB);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  zgesv (MKL_linearsolve,
+ * third_Lapack.h:47-54) of `batch` device matrices with the kernels of cwx_zgetrf_batched_dev +
+ * cwx_zgetrs_batched_dev: dA is overwritten by its LU factors, dB (nrhs columns) by the solution.  The
+ * pivots and per-matrix status live in a static device array of CWX_NOALLOC_SCRATCH_BYTES per GPU,
+ * shared by every call on that GPU: n * batch may be at most CWX_NOALLOC_MAX_PIVOTS, else CWX_ERR_ARG.
+ * Calls that overlap on one GPU (two streams, two threads) race on the scratch.  No hipMalloc, hipFree
+ * or allocating library call is made.  h_info (host, may be NULL) as cwx_zgetrf_batched_dev. */
+#define CWX_NOALLOC_SCRATCH_BYTES (4 << 20)
+#define CWX_NOALLOC_MAX_PIVOTS (((4 << 20) - 65536 * 4) / 4)
+int cwx_zgesv_batched_dev_noalloc(cwx_context* ctx, int n, int nrhs, int batch, void* dA, int64_t strideA,
+                                  void* dB, int64_t strideB, int* h_info);
 int cwx_zgemm_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n, int k,
                           cwx_complex alpha, const void* dA, int lda, int64_t strideA, const void* dB,
                           int ldb, int64_t strideB, cwx_complex beta, void* dC, int ldc, int64_t strideC,

@@ -69,6 +69,7 @@ SIGNATURES = {
     "cwx_zgeev_batched_dev": (I, [P, I, I, P, I64, P, I64, P, I64, P]),
     "cwx_zgetrf_batched_dev": (I, [P, I, I, P, I64, P, I64, P]),
     "cwx_zgetrs_batched_dev": (I, [P, I, I, I, P, I64, P, I64, P, I64]),
+    "cwx_zgesv_batched_dev_noalloc": (I, [P, I, I, I, P, I64, P, I64, P]),
     "cwx_zgemm_batched_dev": (I, [P, I, I, I, I, I, cwx_complex, P, I, I64, P, I, I64, cwx_complex, P, I,
                                   I64, I]),
     "cwx_volume_params_default": (None, [C.POINTER(cwx_volume_params)]),

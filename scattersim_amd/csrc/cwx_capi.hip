@@ -5,6 +5,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -47,6 +48,32 @@ int guard(F&& f) {
 using cwx::zc;
 
 void set_device(cwx_context* ctx) { CWX_HIP(hipSetDevice(ctx->c.device)); }
+
+// Scratch of cwx_zgesv_batched_dev_noalloc: per-matrix status, then the pivots.  Every GPU holds its own
+// copy of the array for the life of the process; host code reaches it only through the device address
+// hipGetSymbolAddress returns for the current device (scratch_ptr), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[CWX_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Called by cwx_create with `device` current: looks up this device's copy of the scratch array once.
+void scratch_init(int device) {
+  CWX_CHECK(device >= 0 && device < kMaxDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  CWX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CWX_CHECK(p != nullptr, "no device address for the scratch array");
+  g_scratch_ptr[device] = p;
+}
+
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  void* p = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+  CWX_CHECK(p != nullptr, "the scratch array of this device was not set up (cwx_create)");
+  return p;
+}
 
 // host -> device copy of a column-major (rows x cols) block with leading dimension ld, packed
 void put_matrix(const cwx_complex* h, int rows, int cols, int ld, zc* d, hipStream_t s) {
@@ -109,6 +136,7 @@ int cwx_create(int device, cwx_context** ctx) {
     auto c = std::make_unique<cwx_context>();
     c->c.device = device;
     CWX_HIP(hipSetDevice(device));
+    scratch_init(device);
     CWX_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
     c->c.own_stream = true;
     *ctx = c.release();
@@ -322,6 +350,38 @@ int cwx_zgetrs_batched_dev(cwx_context* ctx, int n, int nrhs, int batch, const v
     set_device(ctx);
     cwx::zgetrs_batched(n, nrhs, (const zc*)dLU, n, strideA, (const int*)dIpiv, strideP, (zc*)dB, n, strideB,
                         batch, ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
+int cwx_zgesv_batched_dev_noalloc(cwx_context* ctx, int n, int nrhs, int batch, void* dA, int64_t strideA, void* dB,
+                                  int64_t strideB, int* h_info) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dA && dB, "NULL argument");
+    CWX_CHECK(n >= 1 && nrhs >= 1 && batch >= 1 && batch <= 65535, "bad dimensions");
+    CWX_CHECK(strideA >= (int64_t)n * n && strideB >= (int64_t)n * nrhs, "strides smaller than the matrices");
+    if ((long long)n * batch > (long long)CWX_NOALLOC_MAX_PIVOTS)
+      return fail(CWX_ERR_ARG, "zgesv_noalloc: n * batch = " + std::to_string((long long)n * batch) +
+                                   " pivots exceed the scratch limit of " + std::to_string(CWX_NOALLOC_MAX_PIVOTS));
+    // status words first (room for the largest batch), then the pivots
+    const size_t info_bytes = (size_t)65536 * sizeof(int);
+    static_assert((size_t)65536 * sizeof(int) + (size_t)CWX_NOALLOC_MAX_PIVOTS * sizeof(int) <=
+                      (size_t)CWX_NOALLOC_SCRATCH_BYTES, "scratch layout");
+    set_device(ctx);
+    unsigned char* base = static_cast<unsigned char*>(scratch_ptr(ctx->c.device));
+    int* info = reinterpret_cast<int*>(base);
+    int* piv = reinterpret_cast<int*>(base + info_bytes);
+    hipStream_t s = ctx->c.stream;
+    // zgetrf_batched clears info itself (a fill kernel), so the last call's contents do not leak in
+    cwx::zgetrf_batched(n, (zc*)dA, n, strideA, piv, n, info, batch, s);
+    cwx::zgetrs_batched(n, nrhs, (const zc*)dA, n, strideA, piv, n, (zc*)dB, n, strideB, batch, s);
+    CWX_HIP(hipGetLastError());
+    if (h_info) {
+      CWX_HIP(hipMemcpyAsync(h_info, info, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+      CWX_HIP(hipStreamSynchronize(s));
+      for (int b = 0; b < batch; ++b)
+        if (h_info[b]) return fail(h_info[b], "zgesv_noalloc: exactly singular matrix");
+    }
     return CWX_OK;
   });
 }
