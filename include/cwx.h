@@ -46,6 +46,10 @@ enum {
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* cwx_last_error(void);
+/* Process-wide tuning parameters of the kernels (no counterpart in the reference).  "qr_team": helper
+ * workgroups per matrix in the QR iteration when the matrices do not fill the device, -1 (default)
+ * chosen from the shapes, 0 none (one workgroup per matrix), k at most k.  Results do not depend on it. */
+int cwx_set_param(const char* name, int64_t value);
 /* Library version string. */
 const char* cwx_version(void);
 

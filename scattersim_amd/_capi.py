@@ -55,6 +55,7 @@ D = C.c_double
 SIGNATURES = {
     "cwx_last_error": (C.c_char_p, []),
     "cwx_version": (C.c_char_p, []),
+    "cwx_set_param": (I, [C.c_char_p, C.c_int64]),
     "cwx_create": (I, [I, C.POINTER(P)]),
     "cwx_destroy": (I, [P]),
     "cwx_set_stream": (I, [P, P]),

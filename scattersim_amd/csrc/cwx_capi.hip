@@ -124,6 +124,19 @@ cwx::VolumeOptions to_options(const cwx_volume_params* p) {
 extern "C" {
 
 const char* cwx_last_error(void) { return g_err.c_str(); }
+
+int cwx_set_param(const char* name, int64_t value) {
+  return guard([&]() -> int {
+    CWX_CHECK(name != nullptr, "name must not be NULL");
+    const std::string k(name);
+    if (k == "qr_team") {
+      CWX_CHECK(value >= -1 && value <= 64, "qr_team takes -1 (chosen from the shapes), 0 (off) or a helper count");
+      cwx::set_qr_team((int)value);
+      return CWX_OK;
+    }
+    return fail(CWX_ERR_ARG, "unknown parameter: " + k);
+  });
+}
 const char* cwx_version(void) { return "cwx 0.1 (gfx950)"; }
 
 int cwx_create(int device, cwx_context** ctx) {

@@ -258,6 +258,10 @@ __global__ __launch_bounds__(HH_THREADS) void k_unghr(int n, int jlo, const zc* 
 }
 
 // The Householder vectors below the subdiagonal are cleared so that A holds H.
+__global__ void k_fill_i32(long n, int* a) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = 0;
+}
+
 __global__ void k_clear_below_sub(int n, zc* A, int lda, long sA) {
   A += sA * blockIdx.y;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
@@ -1122,9 +1126,162 @@ __device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen)
   p[(long)wlen * stride] = a;
 }
 
+// ------------------------------------------------------------------------------------------------
+// QR teams.  When the matrices do not fill the device, each matrix gets a leader workgroup and G - 1
+// helpers (one cooperative launch, so that all of them are resident).  The leader runs the iteration:
+// deflation search, deflation windows, the bulge chase of every multishift window, and the window's
+// left products on the few columns the next window's block takes in (the window's critical columns).
+// Every other deferred product goes to the helpers: the left products on the columns further right, and
+// the right products on the rows of H above the window and on every row of Z.  The leader publishes each
+// window's rotations and extents in a slot of a per-matrix ring (NSLOT slots) and counts it in ctl[0];
+// helper h counts its finished windows in ctl[CTL_DONE + h - 1].
+//
+// Order.  A line (a row or column of H, a row of Z) belongs to one wavefront of one helper for the whole
+// launch and receives the windows in their order.  The helpers start window w only when all of them
+// have finished window w - 1, because a row line and a column line of different owners cross in one
+// entry, whose left products (earlier windows) precede its right products (later windows).  The
+// leader applies window w to its critical columns only after the helpers have finished window w - 1,
+// whose share held those columns.  What the helpers write lies above the published windows' rows or
+// right of their critical columns, where the leader writes no more in that sweep; what they read from
+// the leader was written before the publication (release / acquire at agent scope; one thread of a
+// workgroup performs them, next to a workgroup barrier, as the cache operations they stand for act on
+// the caches the whole workgroup shares).  Before anything
+// else touches H outside the blocks or Z (the end of a sweep: diagonal scaling, deflation window,
+// single-shift iterations, the end), the leader waits until the helpers have caught up.  A slot is
+// rewritten only after every helper has finished the window it held.  Every wait ends: on the counters,
+// on ctl[1] (stop) or ctl[2] (abort), or after `limit` ticks of the wall clock, after which the launch
+// reports failure.
+// ------------------------------------------------------------------------------------------------
+constexpr int NSLOT = 8;
+constexpr int CTL_INTS = 16, CTL_DONE = 8, TEAM_MAX = CTL_INTS - CTL_DONE + 1;
+constexpr int DESC_INTS = 8;  // kbase, rlo, rhi, whole window inside the block, last critical column
+
+struct HqrTeam {
+  int G = 1;                   // workgroups per matrix (1: the leader does every product itself)
+  int batch = 0;
+  int* ctl = nullptr;          // per matrix CTL_INTS counters, zero at launch
+  double* slots = nullptr;     // per matrix NSLOT x MS_RING doubles of rotations
+  int* desc = nullptr;         // per matrix NSLOT x DESC_INTS
+  unsigned long long limit = 0;  // wall-clock ticks any wait may take
+};
+
+// Polls read the counters relaxed (a coherent load, no cache invalidation); the acquire fence follows
+// once the awaited value has been seen.
+__device__ inline int ld_relaxed(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void st_release(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Thread 0 waits until every helper has finished `target` windows; false on an abort or a timeout.
+__device__ inline bool helpers_reached(const HqrTeam& t, int* ctl, int target) {
+  const unsigned long long t0 = wall_clock64();
+  for (int h = 0; h < t.G - 1; ++h) {
+    while (ld_relaxed(ctl + CTL_DONE + h) < target) {
+      if (ld_relaxed(ctl + 2) || wall_clock64() - t0 > t.limit) {
+        st_release(ctl + 2, 1);
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  return true;
+}
+
+// Leader: the whole workgroup waits until every helper has finished `target` windows, then sees their
+// writes.  Returns false on a timeout.
+__device__ inline bool team_wait(const HqrTeam& t, int* ctl, int target, int* s_flag) {
+  if (threadIdx.x == 0) {
+    *s_flag = helpers_reached(t, ctl, target) ? 1 : 0;
+    // one acquire per workgroup: its invalidation covers the caches the workgroup's waves share
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*s_flag) != 0;
+}
+
+// Every wave's stores have reached the L2 (vmcnt(0)); after the barrier one release store by thread 0
+// (which writes the L2 back) publishes them all.
+__device__ inline void wg_release_store(int* p, int v) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0) st_release(p, v);
+}
+
+// Leader: window `w`'s rotations and descriptor are in its slot; make them (and every earlier write)
+// visible, then count the window.
+__device__ inline void team_publish(int* ctl, int w) { wg_release_store(ctl, w + 1); }
+
+// Helper h (1 .. G-1) of matrix b: applies every published window to its lines, window after window,
+// until the leader stops.  Line l < n: row l of H (right rotations, rows above the window); n <= l < 2n:
+// row l - n of Z; 2n <= l < 3n: column l - 2n of H (left rotations, right of the critical columns).
+template <int NT>
+__device__ void hqr_help(int n, zc* H, int ldh, zc* Z, int ldz, const HqrTeam& t, int b, int h) {
+  __shared__ int s_go, s_desc[DESC_INTS];
+  int* ctl = t.ctl + (long)b * CTL_INTS;
+  const double* slots = t.slots + (long)b * NSLOT * MS_RING;
+  const int* desc = t.desc + (long)b * NSLOT * DESC_INTS;
+  const int nh = t.G - 1, nw = NT / 64, lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nchunks = (3 * n + 63) / 64;
+  constexpr unsigned long long kSign = 1ull << 63;
+  for (int w = 0;; ++w) {
+    if (threadIdx.x == 0) {
+      int go = 0;
+      const unsigned long long t0 = wall_clock64();
+      for (;;) {
+        if (ld_relaxed(ctl) > w) { go = 1; break; }
+        if (ld_relaxed(ctl + 1) || ld_relaxed(ctl + 2)) break;
+        if (wall_clock64() - t0 > t.limit) { st_release(ctl + 2, 1); break; }
+        __builtin_amdgcn_s_sleep(4);
+      }
+      // every helper has finished the window before (the group step)
+      if (go && w > 0 && !helpers_reached(t, ctl, w)) go = 0;
+      if (go) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        for (int i = 0; i < DESC_INTS; ++i) s_desc[i] = desc[(w % NSLOT) * DESC_INTS + i];
+      }
+      s_go = go;
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_go)) return;
+    // the slot's rotations are read through the scalar cache: drop what it holds of older windows
+    __builtin_amdgcn_s_dcache_inv();
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    const int kbase = __builtin_amdgcn_readfirstlane(s_desc[0]), rlo = __builtin_amdgcn_readfirstlane(s_desc[1]);
+    const int rhi = __builtin_amdgcn_readfirstlane(s_desc[2]);
+    const bool full = __builtin_amdgcn_readfirstlane(s_desc[3]) != 0;
+    const int chi = __builtin_amdgcn_readfirstlane(s_desc[4]);
+    const KDouble* R = scalar_view(slots + (long)(w % NSLOT) * MS_RING);
+    for (int c = (h - 1) + nh * wid; c < nchunks; c += nh * nw) {
+      const int l0 = c * 64;
+      // chunks with no line of this window: rows of H at or below it, columns of H up to its last
+      // critical column
+      if ((l0 >= rlo && l0 + 63 < n) || (l0 >= 2 * n && l0 + 63 <= 2 * n + chi)) continue;
+      const int l = l0 + lane;
+      zc* p = nullptr;
+      long st = ldh;
+      unsigned long long sg = kSign;
+      if (l < rlo) {
+        p = H + l + (long)kbase * ldh;
+      } else if (l >= n && l < 2 * n) {
+        p = Z + (l - n) + (long)kbase * ldz;
+        st = ldz;
+      } else if (l > 2 * n + chi && l < 3 * n) {
+        p = H + (long)(l - 2 * n) * ldh + kbase;
+        st = 1;
+        sg = 0;
+      }
+      if (p) {
+        if (full) ms_apply_line<true>(p, st, sg, 0, MS_LEN - 1, R);
+        else ms_apply_line<false>(p, st, sg, rlo - kbase, rhi - kbase, R);
+      }
+    }
+    wg_release_store(ctl + CTL_DONE + h - 1, w + 1);
+  }
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, zc* Z, int ldz,
-                                                     long sZ, zc* W, long sW, int* info, double* ring) {
+                                                     long sZ, zc* W, long sW, int* info, double* ring,
+                                                     HqrTeam team) {
   constexpr int BR = QR_LD, BC = QR_W + 3;  // single-shift block: QR_W + 2 rows (leading dimension QR_LD)
   static_assert(BR * BC <= MS_LD * (MS_LEN + 1), "window buffers share one LDS array");
   __shared__ zc MB[MS_LD * (MS_LEN + 1)];  // window block (single-shift: BR x BC, multishift: LEN x LEN+1)
@@ -1138,18 +1295,37 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
   __shared__ int s_k, s_m;
   __shared__ zc s_t, s_v1;
   __shared__ double s_v2;
+  __shared__ int s_team;
   zc* B = MB;
-  const int b = blockIdx.x;
+  // teams: the workgroups of one matrix share blockIdx.x mod 8 (one XCD under round-robin dispatch)
+  int b = blockIdx.x, role = 0;
+  if (team.G > 1) {
+    const int k = blockIdx.x / 8;
+    b = (int)(blockIdx.x % 8) + 8 * (k / team.G);
+    role = k % team.G;
+    if (b >= team.batch) return;
+  }
+  if (role > 0) {
+    hqr_help<NT>(n, H + sH * b, ldh, Z + sZ * b, ldz, team, b, role);
+    return;
+  }
+  const bool teamed = team.G > 1;
+  int* const ctl = teamed ? team.ctl + (long)b * CTL_INTS : nullptr;
+  int wseq = 0;  // windows published to the helpers
+  auto stop_team = [&]() {
+    if (teamed && threadIdx.x == 0) st_release(ctl + 1, 1);
+  };
   H += sH * b;
   Z += sZ * b;
   W += sW * b;
-  Rot2* const rg = reinterpret_cast<Rot2*>(ring + (long)MS_RING * b);
+  Rot2* const rg0 = reinterpret_cast<Rot2*>(ring + (long)MS_RING * b);
   const int tid = threadIdx.x, nt = blockDim.x;
   auto Hp = [&](int r, int c) -> zc& { return H[(long)c * ldh + r]; };
   auto Zp = [&](int r, int c) -> zc& { return Z[(long)c * ldz + r]; };
   if (tid == 0) info[b] = 0;
   if (n == 1) {
     if (tid == 0) W[0] = Hp(0, 0);
+    stop_team();
     return;
   }
   const int ilo = 0, ihi = n - 1;
@@ -1207,6 +1383,7 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
     }
     if (++its > itmax) {
       if (tid == 0) info[b] = I + 1;
+      stop_team();
       return;
     }
     const int m = I - L + 1;
@@ -1335,6 +1512,31 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
           }
           __syncthreads();
         }
+        const bool full = kbase >= L && kbase + MS_LEN - 1 <= I;
+        // last critical column: the next window's block reaches 16 columns further (none after the last)
+        const int chi = (t0 + MS_S < T_total) ? min(rhi + MS_S, I) : rhi;
+        // the window's rotations: the leader's own ring, or the window's slot in a team (wseq is carried
+        // by the iteration loop, whose trip count the compiler cannot prove uniform; the slot is)
+        Rot2* const rg = teamed ? reinterpret_cast<Rot2*>(
+                                      team.slots +
+                                      (long)__builtin_amdgcn_readfirstlane(b * NSLOT + wseq % NSLOT) * MS_RING)
+                                : rg0;
+        if (teamed) {
+          // the slot of this window is free once every helper has finished the window it held
+          if (wseq >= NSLOT && !team_wait(team, ctl, wseq - NSLOT + 1, &s_team)) {
+            if (tid == 0) info[b] = n + 1;
+            stop_team();
+            return;
+          }
+          if (tid == 0) {
+            int* d = team.desc + ((long)b * NSLOT + wseq % NSLOT) * DESC_INTS;
+            d[0] = kbase;
+            d[1] = rlo;
+            d[2] = rhi;
+            d[3] = full ? 1 : 0;
+            d[4] = chi;
+          }
+        }
         for (int e = tid; e < MS_S * MS_NB; e += nt) rg[(e % MS_NB) * MS_S + e / MS_NB] = rots2[e];
         for (int e = tid; e < nr * nc; e += nt) {
           const int r = rlo + e % nr, c = clo + e / nr;
@@ -1344,6 +1546,15 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
         // the previous window's rotations (s_waitcnt encodings of gfx9: 0x0F70 vmcnt(0), 0xC07F lgkmcnt(0))
         __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
+        if (teamed) {
+          team_publish(ctl, wseq++);
+          // the critical columns hold the helpers' share of the window before: wait for it
+          if (wseq >= 2 && !team_wait(team, ctl, wseq - 1, &s_team)) {
+            if (tid == 0) info[b] = n + 1;
+            stop_team();
+            return;
+          }
+        }
         __builtin_amdgcn_s_dcache_inv();
         __builtin_amdgcn_s_waitcnt(0xC07F);
         {
@@ -1362,10 +1573,10 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
           asm volatile("" ::"s"(acc));
         }
         // deferred products: columns right of the window (left rotations on rows kbase + q),
-        // rows above it and all rows of Z (right rotations on columns kbase + q)
-        const int nfr = n - 1 - rhi, nfa = rlo, nz = n;
-        const int lines = nfr + nfa + nz;
-        const bool full = kbase >= L && kbase + MS_LEN - 1 <= I;
+        // rows above it and all rows of Z (right rotations on columns kbase + q; the helpers' share
+        // in a team)
+        const int nfr = teamed ? chi - rhi : n - 1 - rhi, nfa = rlo, nz = n;
+        const int lines = teamed ? nfr : nfr + nfa + nz;
         constexpr unsigned long long kSign = 1ull << 63;
         for (int ln = tid; ln < lines; ln += nt) {
           const KDouble* R = scalar_view(reinterpret_cast<const double*>(rg));
@@ -1389,6 +1600,11 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
           else ms_apply_line<false>(p, st, sg, rlo - kbase, rhi - kbase, R);
         }
         __syncthreads();
+      }
+      if (teamed && !team_wait(team, ctl, wseq, &s_team)) {
+        if (tid == 0) info[b] = n + 1;
+        stop_team();
+        return;
       }
       real_subdiag(H, ldh, Z, ldz, n, L, I, MB);
       continue;
@@ -1542,6 +1758,7 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
     }
     __syncthreads();
   }
+  stop_team();
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1622,6 +1839,26 @@ __global__ __launch_bounds__(256) void k_normalize(int n, zc* V, int ldv, long s
   }
 }
 
+int g_qr_team = -1;  // helpers per QR team: -1 chosen from the shapes, 0 none, k at most k
+
+}  // namespace
+
+void set_qr_team(int helpers) { g_qr_team = helpers; }
+
+namespace {
+
+// Workgroups per matrix of the QR launch: a leader plus enough helpers that each helper wavefront takes
+// one 64-row chunk of the rows of H and Z per window, while the teams still fit the device once.
+int qr_team_size(int n, int batch, int cus) {
+  if (g_qr_team == 0 || n < 2 * MS_NMIN || n > MS_LD * (MS_LEN + 1)) return 1;
+  const int chunks = (3 * n + 63) / 64;
+  int helpers = (chunks + EIG_THREADS / 64 - 1) / (EIG_THREADS / 64);
+  helpers = std::min(helpers, TEAM_MAX - 1);
+  if (g_qr_team > 0) helpers = std::min(helpers, g_qr_team);
+  helpers = std::min(helpers, cus / std::max(1, batch) - 1);
+  return helpers >= 1 ? helpers + 1 : 1;
+}
+
 }  // namespace
 
 void EigWork::ensure(int n, int batch) {
@@ -1635,6 +1872,12 @@ void EigWork::ensure(int n, int batch) {
   T.ensure(npan * HB * HB * batch);
   W.ensure((size_t)2 * n * HB * batch);
   ring.ensure((size_t)MS_RING * batch);
+}
+
+void EigWork::ensure_team(int batch) {
+  team_ctl.ensure((size_t)CTL_INTS * batch);
+  team_slots.ensure((size_t)NSLOT * MS_RING * batch);
+  team_desc.ensure((size_t)NSLOT * DESC_INTS * batch);
 }
 
 void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
@@ -1697,12 +1940,52 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   int dev = 0, cus = 0;
   CWX_HIP(hipGetDevice(&dev));
   CWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  if (batch >= 2 * cus)
+  HqrTeam team;
+  team.G = (batch < cus) ? qr_team_size(n, batch, cus) : 1;
+  if (team.G > 1) {
+    // a cooperative launch keeps every workgroup of a team resident; without room for all of them the
+    // leaders work alone
+    int per_cu = 0;
+    CWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_hqr<EIG_THREADS>, EIG_THREADS, 0));
+    const long grid = 8L * ((batch + 7) / 8) * team.G;
+    if ((long)per_cu * cus < grid) team.G = 1;
+  }
+  if (team.G > 1) {
+    work.ensure_team(batch);
+    int rate_khz = 0;
+    CWX_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    team.batch = batch;
+    team.ctl = work.team_ctl.p;
+    team.slots = work.team_slots.p;
+    team.desc = work.team_desc.p;
+    team.limit = (unsigned long long)std::max(rate_khz, 1) * 1000ull * 20ull;  // 20 s
+    hipLaunchKernelGGL(k_fill_i32, dim3(cdiv((size_t)CTL_INTS * batch, 256)), dim3(256), 0, s,
+                       (long)CTL_INTS * batch, team.ctl);
+    int n_ = n, lda_ = lda, ldz = n;
+    long sA_ = sA, sZ = nn, sW_ = sW;
+    zc* Ap = A;
+    zc* Zp = work.Z.p;
+    zc* Wp = W;
+    int* infop = info;
+    double* ringp = work.ring.p;
+    void* args[] = {&n_, &Ap, &lda_, &sA_, &Zp, &ldz, &sZ, &Wp, &sW_, &infop, &ringp, &team};
+    const unsigned grid = (unsigned)(8 * ((batch + 7) / 8) * team.G);
+    if (hipLaunchCooperativeKernel((const void*)k_hqr<EIG_THREADS>, dim3(grid), dim3(EIG_THREADS), args, 0, s) !=
+        hipSuccess) {
+      // the runtime may refuse cooperative launches (for instance while another process holds the
+      // device); the leaders then work alone
+      (void)hipGetLastError();
+      team = HqrTeam();
+    }
+  }
+  if (team.G > 1) {
+  } else if (batch >= 2 * cus) {
     hipLaunchKernelGGL(k_hqr<EIG_THREADS_SMALL>, dim3(batch), dim3(EIG_THREADS_SMALL), 0, s, n, A, lda, sA,
-                       work.Z.p, n, nn, W, sW, info, work.ring.p);
-  else
+                       work.Z.p, n, nn, W, sW, info, work.ring.p, team);
+  } else {
     hipLaunchKernelGGL(k_hqr<EIG_THREADS>, dim3(batch), dim3(EIG_THREADS), 0, s, n, A, lda, sA, work.Z.p, n,
-                       nn, W, sW, info, work.ring.p);
+                       nn, W, sW, info, work.ring.p, team);
+  }
   const double smlnum = kSafmin * ((double)n / kUlp);
   // one wavefront per eigenvector, as many per workgroup (up to 4) as fit 64 KiB of LDS
   const int waves = (int)std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / (sizeof(zc) * n)));

@@ -105,8 +105,16 @@ struct EigWork {
   DevBuf<int> info;  // batch
   DevBuf<zc> Y, T, W; // blocked Hessenberg reduction: panel Y, T of every panel, GEMM scratch
   DevBuf<double> ring; // QR: the current window's rotations of every matrix, read back as scalars
+  DevBuf<int> team_ctl;       // QR teams (leader + helpers per matrix): counters, rotation slots, slot
+  DevBuf<double> team_slots;  // descriptors
+  DevBuf<int> team_desc;
   void ensure(int n, int batch);
+  void ensure_team(int batch);
 };
+
+// Helpers per matrix in the QR kernel's teams: -1 chooses from the shapes (the default), 0 turns teams
+// off, k > 0 allows at most k.
+void set_qr_team(int helpers);
 
 // A is destroyed (overwritten by the Schur form).  W receives eigenvalues, V the unit 2-norm right
 // eigenvectors with their largest component real.  info (device, batch) is 0 or the LAPACK-style

@@ -205,3 +205,31 @@ def test_zgeev_large_batch_matches_small_batch(ctx):
     for i, p in enumerate(picks):
         assert np.array_equal(W[p], Ws[i]) and np.array_equal(V[p], Vs[i])
         check_eig(A[p], W[p], V[p])
+
+
+@pytest.mark.parametrize("n,batch", [(450, 4), (196, 1), (900, 2)])
+def test_zgeev_teams_match_one_workgroup(ctx, n, batch):
+    """The QR iteration with helper workgroups per matrix (the default when the matrices do not fill the
+    device) gives exactly the eigenvalues and eigenvectors of one workgroup per matrix: every row receives
+    the same rotations in the same order."""
+    import ctypes as C
+    from scattersim_amd import _capi
+    rng = np.random.default_rng(n + batch)
+    A = np.stack([crand(rng, n, n) for _ in range(batch)])
+    out = []
+    try:
+        for team in (0, -1):
+            _capi.check(_capi.lib().cwx_set_param(b"qr_team", team))
+            dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(A.transpose(0, 2, 1)))
+            dW = sa.DeviceArray(ctx, (batch, n), np.complex128)
+            dV = sa.DeviceArray(ctx, (batch, n, n), np.complex128)
+            info = np.zeros(batch, np.int32)
+            _capi.check(_capi.lib().cwx_zgeev_batched_dev(ctx.h, n, batch, C.c_void_p(dA.ptr), n * n,
+                                                          C.c_void_p(dW.ptr), n, C.c_void_p(dV.ptr), n * n,
+                                                          info.ctypes.data_as(C.c_void_p)))
+            out.append((dW.numpy(), dV.numpy()))
+    finally:
+        _capi.lib().cwx_set_param(b"qr_team", -1)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for b in range(batch):
+        check_eig(A[b], out[1][0][b], out[1][1][b].T)
