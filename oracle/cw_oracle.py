@@ -470,6 +470,32 @@ def eig_layers(s: Setup):
     return gammas, vectors, Nifs, phis
 
 
+def eig_layers_reduced(s: Setup):
+    """The reduced eigenproblem the HIP path's eigen mode 1 solves instead of zgeev of the 4MF Phi (no
+    counterpart in the reference; used to time the CPU side of that mode and to check it).  Phi =
+    [[0, P], [Q, 0]] squares to diag(PQ, QP), so each eigenpair (mu, a) of PQ gives gamma = +-sqrt(mu)
+    with vectors [a; +-Q a / gamma]; they are sorted and interleaved like zgeev's (ScatterVolume.cpp:190-230).
+    Invariant outputs (x, fields) equal the full form's wherever no gamma is near 0."""
+    gammas, vectors, Nifs = [], [], []
+    MF = s.MF
+    for layer in s.layers:
+        Nf, Nif = layer_fourier(layer, s.M0, s.M1)
+        Nifs.append(Nif)
+        Phi = phi_matrix(Nf, Nif, s.up, s.wq, s.k)
+        P, Q = Phi[0:2 * MF, 2 * MF:], Phi[2 * MF:, 0:2 * MF]
+        mu, a = zgeev(P @ Q)
+        g = np.sqrt(mu.astype(np.complex128))
+        Qa = Q @ a
+        w = np.concatenate([g, -g])
+        V = np.concatenate([np.concatenate([a, a], axis=1),
+                            np.concatenate([Qa / g[None, :], -Qa / g[None, :]], axis=1)], axis=0)
+        V = V / np.linalg.norm(V, axis=0)[None, :]
+        gs, Vs = eigen_sort(w, V)
+        gammas.append(gs)
+        vectors.append(Vs)
+    return gammas, vectors, Nifs
+
+
 def solve(sample: np.ndarray, p: VolumeParams, eig=None, EF=None) -> Solution:
     """The scattervolume solve, ScatterVolume.cpp:574-889.
 

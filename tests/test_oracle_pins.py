@@ -344,3 +344,16 @@ def test_cw_roundtrip_and_legacy_reader():
     nrec = len(cw.Pi) + sum(len(a) for a in cw.Pr + cw.Pt)
     MF = sol.setup.MF
     assert nrec == MF * (3 + 4 * MF * sol.setup.L)
+
+
+def test_reduced_eigen_form_matches_full_phi():
+    """The oracle's reduced-form eigen step (P Q instead of the 4MF Phi, used to time the CPU side of the
+    reduced mode) gives the full form's boundary solution and fields on a three-layer oblique stack."""
+    from tests.helpers import disks_sample, fields, probe_points, rel_err
+    s = disks_sample(16, 3, 4.0, 4, 1.0, lambda r: 1.4 + 0.02j * r.random(), seed=5)
+    p = o.VolumeParams(lam=0.6, size=(4.0, 4.0, 1.0), coef=(5, 5), direction=(0.2, 0.1, 1.0))
+    a = o.solve(s, p)
+    b = o.solve(s, p, eig=o.eig_layers_reduced)
+    assert rel_err(b.x, a.x) < 1e-12
+    pts = probe_points(a.setup.z, p.size)
+    assert rel_err(fields(o.build_cw(b), pts), fields(o.build_cw(a), pts)) < 1e-11
