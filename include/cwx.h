@@ -46,10 +46,20 @@ enum {
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* cwx_last_error(void);
-/* Process-wide tuning parameters of the kernels (no counterpart in the reference).  "qr_team": helper
- * workgroups per matrix in the QR iteration when the matrices do not fill the device, -1 (default)
- * chosen from the shapes, 0 none (one workgroup per matrix), k at most k.  Results do not depend on it. */
+/* Launch shapes (no counterpart in the reference).  The library reads the table that
+ * `python -m scattersim_amd.tune` writes (data/tuning.txt beside the library's lib/ directory, or the
+ * file named by CWX_TUNING_FILE); cwx_set_param overrides one key for the process, -1 selecting the
+ * built-in choice and CWX_PARAM_TABLE returning to the table.  Keys:
+ *   "qr_team"          helper workgroups per matrix of the QR iteration when the matrices do not fill
+ *                      the device (0: none); results do not depend on it
+ *   "qr_threads"       256 or 512 threads per QR workgroup when they do
+ *   "batch_chunk_mib"  device memory of one chunk of cwx_solve_batch_dev when max_bytes is 0
+ * cwx_tuning_table returns the path of the table in use ("" if none).  Host-only calls: no device work. */
+#define CWX_PARAM_TABLE INT64_MIN
 int cwx_set_param(const char* name, int64_t value);
+/* The value in force for key `name` at shape (a, b): the override, else the table's rule, else -1. */
+int64_t cwx_get_param(const char* name, int64_t a, int64_t b);
+const char* cwx_tuning_table(void);
 /* Library version string. */
 const char* cwx_version(void);
 

@@ -42,6 +42,7 @@ class cwx_solution_info(C.Structure):
 
 
 CWX_OK, CWX_ERR_ARG, CWX_ERR_DEVICE, CWX_ERR_IO, CWX_ERR_STATE = 0, -1, -2, -3, -4
+CWX_PARAM_TABLE = -(1 << 63)
 (SOL_X, SOL_GAMMA, SOL_BETA, SOL_QCH, SOL_Z, SOL_RECORDS, SOL_SX, SOL_SY, SOL_SZ0, SOL_SZ1, SOL_EF,
  SOL_GD, SOL_GC, SOL_NIF) = range(14)
 
@@ -56,6 +57,8 @@ SIGNATURES = {
     "cwx_last_error": (C.c_char_p, []),
     "cwx_version": (C.c_char_p, []),
     "cwx_set_param": (I, [C.c_char_p, C.c_int64]),
+    "cwx_tuning_table": (C.c_char_p, []),
+    "cwx_get_param": (C.c_int64, [C.c_char_p, C.c_int64, C.c_int64]),
     "cwx_create": (I, [I, C.POINTER(P)]),
     "cwx_destroy": (I, [P]),
     "cwx_set_stream": (I, [P, P]),

@@ -23,6 +23,7 @@ LIB = os.path.join(LIBDIR, "libcwx.so")
 
 HIP_SOURCES = ["cwx_linalg.hip", "cwx_eig.hip", "cwx_field.hip", "cwx_fieldsep.hip", "cwx_solver.hip",
                "cwx_layer.hip", "cwx_mlp.hip", "cwx_capi.hip"]
+HOST_SOURCES = ["cwx_tune.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -52,10 +53,14 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        cmds.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"], verbose)
     os.replace(tmp, LIB)
     host = ["g++", "-O2", "-std=c++17", f"-I{CSRC}", f"-L{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib"]
     tools = [("cli_scattervolume.cpp", "scattervolume"), ("cli_scatterview.cpp", "scatterview"),

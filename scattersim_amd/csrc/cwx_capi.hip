@@ -14,6 +14,7 @@
 #include "cwx_layer.h"
 #include "cwx_mlp.h"
 #include "cwx_solver.h"
+#include "cwx_tune.h"
 
 struct cwx_context {
   cwx::Context c;
@@ -129,13 +130,30 @@ int cwx_set_param(const char* name, int64_t value) {
   return guard([&]() -> int {
     CWX_CHECK(name != nullptr, "name must not be NULL");
     const std::string k(name);
+    const bool unset = value == CWX_PARAM_TABLE;
     if (k == "qr_team") {
-      CWX_CHECK(value >= -1 && value <= 64, "qr_team takes -1 (chosen from the shapes), 0 (off) or a helper count");
-      cwx::set_qr_team((int)value);
-      return CWX_OK;
+      CWX_CHECK(unset || (value >= -1 && value <= 64), "qr_team takes -1 (built-in choice), 0 (off) or a helper count");
+    } else if (k == "qr_threads") {
+      CWX_CHECK(unset || value == -1 || value == 256 || value == 512, "qr_threads takes 256 or 512");
+    } else if (k == "batch_chunk_mib") {
+      CWX_CHECK(unset || value == -1 || value >= 64, "batch_chunk_mib takes at least 64");
+    } else {
+      return fail(CWX_ERR_ARG, "unknown parameter: " + k);
     }
-    return fail(CWX_ERR_ARG, "unknown parameter: " + k);
+    cwx::tune_override(k, (long)value, !unset && value != -1);
+    return CWX_OK;
   });
+}
+
+int64_t cwx_get_param(const char* name, int64_t a, int64_t b) {
+  if (!name) return -1;
+  return cwx::tune_lookup(name, (long)a, (long)b, -1);
+}
+
+const char* cwx_tuning_table(void) {
+  static thread_local std::string p;
+  p = cwx::tune_table_path();
+  return p.c_str();
 }
 const char* cwx_version(void) { return "cwx 0.1 (gfx950)"; }
 

@@ -15,6 +15,7 @@
 //   5. V = Z X on the GEMM kernel, then zgeev's normalisation (unit 2-norm, largest component real).
 #include "cwx_common.h"
 #include "cwx_kernels.h"
+#include "cwx_tune.h"
 
 namespace cwx {
 
@@ -1839,22 +1840,16 @@ __global__ __launch_bounds__(256) void k_normalize(int n, zc* V, int ldv, long s
   }
 }
 
-int g_qr_team = -1;  // helpers per QR team: -1 chosen from the shapes, 0 none, k at most k
-
-}  // namespace
-
-void set_qr_team(int helpers) { g_qr_team = helpers; }
-
-namespace {
-
-// Workgroups per matrix of the QR launch: a leader plus enough helpers that each helper wavefront takes
-// one 64-row chunk of the rows of H and Z per window, while the teams still fit the device once.
+// Workgroups per matrix of the QR launch: a leader plus helpers, as many as the tuning table gives for
+// (n, batch) ("qr_team"; -1 or no rule: enough that each helper wavefront takes one 64-line chunk of the
+// rows and columns of H and Z per window), while the teams still fit the device once.
 int qr_team_size(int n, int batch, int cus) {
-  if (g_qr_team == 0 || n < 2 * MS_NMIN || n > MS_LD * (MS_LEN + 1)) return 1;
+  const long want = tune_lookup("qr_team", n, batch, -1);
+  if (want == 0 || n < 2 * MS_NMIN || n > MS_LD * (MS_LEN + 1)) return 1;
   const int chunks = (3 * n + 63) / 64;
   int helpers = (chunks + EIG_THREADS / 64 - 1) / (EIG_THREADS / 64);
+  if (want > 0) helpers = (int)std::min<long>(want, TEAM_MAX - 1);
   helpers = std::min(helpers, TEAM_MAX - 1);
-  if (g_qr_team > 0) helpers = std::min(helpers, g_qr_team);
   helpers = std::min(helpers, cus / std::max(1, batch) - 1);
   return helpers >= 1 ? helpers + 1 : 1;
 }
@@ -1979,7 +1974,8 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     }
   }
   if (team.G > 1) {
-  } else if (batch >= 2 * cus) {
+  } else if (tune_lookup("qr_threads", n, batch, batch >= 2 * cus ? EIG_THREADS_SMALL : EIG_THREADS) ==
+             EIG_THREADS_SMALL) {
     hipLaunchKernelGGL(k_hqr<EIG_THREADS_SMALL>, dim3(batch), dim3(EIG_THREADS_SMALL), 0, s, n, A, lda, sA,
                        work.Z.p, n, nn, W, sW, info, work.ring.p, team);
   } else {

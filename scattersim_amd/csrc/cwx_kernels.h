@@ -112,9 +112,6 @@ struct EigWork {
   void ensure_team(int batch);
 };
 
-// Helpers per matrix in the QR kernel's teams: -1 chooses from the shapes (the default), 0 turns teams
-// off, k > 0 allows at most k.
-void set_qr_team(int helpers);
 
 // A is destroyed (overwritten by the Schur form).  W receives eigenvalues, V the unit 2-norm right
 // eigenvectors with their largest component real.  info (device, batch) is 0 or the LAPACK-style

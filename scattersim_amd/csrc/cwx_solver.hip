@@ -5,6 +5,7 @@
 // sort, Gd/Gc chain, boundary system), :739-889 (zgesv, beta, Q_check/Q_hat, Pz) and :907-987
 // (CoupledWaveStructure assembly).  All per-layer work is batched over layers.
 #include "cwx_solver.h"
+#include "cwx_tune.h"
 
 #include <algorithm>
 #include <array>
@@ -1338,7 +1339,8 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
   // bytes per layer: Phi or P,Q + LU (w0), PQ + T (w1), pool (w3), Gd, Gc, eigen work (Z, X), V, QA
   const size_t per_layer = sizeof(zc) * (size_t)(nn * 5 + n_eig * n_eig * 4 + (long)s1 * s2 * 2);
   const size_t per_inst = sizeof(zc) * (size_t)(nn + 2 * n4 * 3 * st0.MF + n6 * n6);
-  if (max_bytes == 0) max_bytes = (size_t)64 << 30;
+  if (max_bytes == 0) max_bytes = (size_t)tune_lookup("batch_chunk_mib", 4 * opts[0].coef[0] * opts[0].coef[1], count,
+                                                      64 << 10) << 20;
   info.assign(count, 0);
   first_msg.clear();
   int worst = 0;

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 import tempfile
 
 import numpy as np
@@ -195,3 +196,30 @@ def test_library_order_selection_matches_oracle():
         assert got == (M0, M1), (c, shp, got, (M0, M1))
         if ex is not None:
             assert got == ex
+
+
+def test_tuning_table_and_overrides(tmp_path):
+    """The launch-shape table (key a_lo a_hi b_lo b_hi value per line) is read from CWX_TUNING_FILE; the
+    first matching rule wins, cwx_set_param overrides a key and CWX_PARAM_TABLE returns to the table.
+    Runs in a child process, because the library reads its table once."""
+    table = tmp_path / "t.txt"
+    table.write_text("# comment\nqr_team 1 500 1 16 2  # two helpers\nqr_team 1 500 17 64 0\n"
+                     "qr_team 1 1000 1 100 5\nbatch_chunk_mib 1 1000 1 99999 8192\n")
+    code = f"""
+import ctypes as C, sys
+sys.path.insert(0, {ROOT!r})
+from scattersim_amd import _capi
+L = _capi.lib()
+assert L.cwx_tuning_table().decode() == {str(table)!r}
+g = lambda k, a, b: L.cwx_get_param(k.encode(), a, b)
+assert g("qr_team", 450, 1) == 2 and g("qr_team", 450, 32) == 0 and g("qr_team", 900, 32) == 5
+assert g("qr_team", 2000, 1) == -1 and g("qr_threads", 450, 1) == -1 and g("batch_chunk_mib", 484, 10) == 8192
+assert L.cwx_set_param(b"qr_team", 3) == 0 and g("qr_team", 450, 1) == 3
+assert L.cwx_set_param(b"qr_team", _capi.CWX_PARAM_TABLE) == 0 and g("qr_team", 450, 1) == 2
+assert L.cwx_set_param(b"qr_threads", 300) == _capi.CWX_ERR_ARG
+assert L.cwx_set_param(b"no_such_key", 1) == _capi.CWX_ERR_ARG
+print("ok")
+"""
+    env = dict(os.environ, CWX_TUNING_FILE=str(table))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=60)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
