@@ -164,6 +164,16 @@ int cwx_solve_volume_dev(cwx_context* ctx, const cwx_volume_params* p, const voi
 int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
                         const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int* h_info, int* M0,
                         int* M1);
+/* cwx_solve_batch_dev with every chunk of `chunk` instances one launch set captured in a HIP graph (no
+ * counterpart in the reference).  The first chunk of a given shape (layer counts in order, orders, eigen
+ * mode) is captured, and later chunks of the same shape replay that graph with their own samples and
+ * tables; the graphs stay with the context until cwx_destroy.  Results equal cwx_solve_batch_dev's.
+ * Teams of the QR iteration are not used inside a graph; an instance whose reduced eigenproblem has a
+ * nearly defective pair is solved again outside the graph.  *captured (may be NULL) receives the number
+ * of graphs this call captured. */
+int cwx_solve_batch_graph_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                              const void* d_samples, int s1, int s2, int chunk, void* d_x, int* h_info,
+                              int* captured);
 
 /* cwx_solve_batch_dev for instances whose Fourier orders (or eigen modes) differ, as wavelength and angle
  * sweeps with the order cut-off produce: the instances are grouped by orders and each group is one

@@ -477,6 +477,23 @@ def solve_batch(ctx: Context, samples, params, max_bytes: int = 0, shape=None, d
     return x, info, (M0.value, M1.value)
 
 
+def solve_batch_graph(ctx: Context, d_samples: int, params, Ls, s1: int, s2: int, chunk: int, d_x: int):
+    """cwx_solve_batch_graph_dev: the batched solve with each chunk of `chunk` instances replayed from a
+    captured HIP graph.  Device pointers in and out; returns (info, graphs captured by this call)."""
+    count = len(params)
+    pa = (_capi.cwx_volume_params * count)()
+    for i, o in enumerate(params):
+        pa[i] = _params(**o)
+    Ls = np.ascontiguousarray(Ls, np.int32)
+    info = np.zeros(count, np.int32)
+    cap = C.c_int()
+    st = lib().cwx_solve_batch_graph_dev(ctx.h, count, pa, _ptr(Ls), C.c_void_p(d_samples), int(s1), int(s2),
+                                         int(chunk), C.c_void_p(d_x), _ptr(info), C.byref(cap))
+    if st < 0:
+        raise _capi.CwxError(st, lib().cwx_last_error().decode())
+    return info, cap.value
+
+
 def zgeev(ctx: Context, A: np.ndarray):
     """MKL_eigensolve contract: returns (w, VR); A is not modified."""
     n = A.shape[0]

@@ -90,6 +90,7 @@ SIGNATURES = {
     "cwx_solution_write_cw": (I, [P, P, C.c_char_p]),
     "cwx_solution_write_cw2": (I, [P, P, C.c_char_p]),
     "cwx_solution_read_cw2": (I, [P, C.c_char_p, P]),
+    "cwx_solve_batch_graph_dev": (I, [P, I, C.POINTER(cwx_volume_params), P, P, I, I, I, P, P, P]),
     "cwx_solve_batch_mixed_dev": (I, [P, I, C.POINTER(cwx_volume_params), P, P, I, I, C.c_int64, P, P, P, P, P]),
     "cwx_solve_fields_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, I, P, P, C.POINTER(I),
                                  C.POINTER(I)]),

@@ -508,6 +508,29 @@ int cwx_solve_batch_dev(cwx_context* ctx, int count, const cwx_volume_params* pa
   });
 }
 
+int cwx_solve_batch_graph_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
+                              const void* d_samples, int s1, int s2, int chunk, void* d_x, int* h_info,
+                              int* captured) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && params && L && d_samples && d_x, "NULL argument");
+    CWX_CHECK(count >= 1 && s1 >= 1 && s2 >= 1 && chunk >= 1, "bad batch arguments");
+    std::vector<cwx::VolumeOptions> opts(count);
+    std::vector<int> Ls(L, L + count);
+    for (int i = 0; i < count; ++i) {
+      CWX_CHECK(Ls[i] >= 1, "instance " + std::to_string(i) + " has no layers");
+      opts[i] = to_options(params + i);
+    }
+    set_device(ctx);
+    std::vector<int> info;
+    std::string msg;
+    const int st_code = cwx::solve_batch_graph(ctx->c, opts, (const zc*)d_samples, Ls, s1, s2, chunk, (zc*)d_x,
+                                               info, msg, captured);
+    if (h_info) std::copy(info.begin(), info.end(), h_info);
+    if (st_code) return fail(st_code, msg);
+    return CWX_OK;
+  });
+}
+
 int cwx_solve_batch_mixed_dev(cwx_context* ctx, int count, const cwx_volume_params* params, const int* L,
                               const void* d_samples, int s1, int s2, int64_t max_bytes, void* d_x, int64_t* x_offsets,
                               int* h_info, int* M0s, int* M1s) {

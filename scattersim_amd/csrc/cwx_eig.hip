@@ -1876,7 +1876,7 @@ void EigWork::ensure_team(int batch) {
 }
 
 void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
-                   EigWork& work, int* info, hipStream_t s) {
+                   EigWork& work, int* info, hipStream_t s, bool teams) {
   if (n <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgeev: batch too large");
   work.ensure(n, batch);
@@ -1936,7 +1936,7 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   CWX_HIP(hipGetDevice(&dev));
   CWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HqrTeam team;
-  team.G = (batch < cus) ? qr_team_size(n, batch, cus) : 1;
+  team.G = (teams && batch < cus) ? qr_team_size(n, batch, cus) : 1;
   if (team.G > 1) {
     // a cooperative launch keeps every workgroup of a team resident; without room for all of them the
     // leaders work alone

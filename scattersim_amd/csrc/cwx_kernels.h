@@ -116,7 +116,9 @@ struct EigWork {
 // A is destroyed (overwritten by the Schur form).  W receives eigenvalues, V the unit 2-norm right
 // eigenvectors with their largest component real.  info (device, batch) is 0 or the LAPACK-style
 // failure index.
+// teams = false keeps one workgroup per matrix in the QR iteration (no cooperative launch, so that the
+// launch can be captured in a graph).
 void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ldv, long sV, int batch,
-                   EigWork& work, int* info, hipStream_t s);
+                   EigWork& work, int* info, hipStream_t s, bool teams = true);
 
 }  // namespace cwx

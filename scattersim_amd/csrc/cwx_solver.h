@@ -1,6 +1,7 @@
 // Host-side coupled-wave solver: option handling, device pipeline and solution storage.
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,14 @@ struct Solution {
 void solve_volume(Context& ctx, const VolumeOptions& o, const zc* d_sample, int L, int s1, int s2,
                   Solution& sol);
 
+// solve_batch with the launches of each chunk of `chunk` instances captured in a HIP graph the first time a
+// chunk of that shape (layer counts, orders, eigen mode) comes, and replayed for the others; the graphs
+// stay in the context.  Results equal solve_batch's.  captured (may be null) receives the number of
+// graphs this call captured.
+int solve_batch_graph(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples,
+                      const std::vector<int>& Ls, int s1, int s2, int chunk, zc* d_x, std::vector<int>& info,
+                      std::string& first_msg, int* captured);
+
 // Solve many independent samples that share the grid and the Fourier orders: instance i has opts[i]
 // and Ls[i] layers, stored back to back in d_samples.  Instances are processed in chunks whose
 // device workspace stays below max_bytes (0: 64 GiB); each chunk is one launch set over all of
@@ -108,14 +117,21 @@ struct SepWork {
 void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int out_f64, void* out,
                      SepWork& w, hipStream_t s);
 
+// Work areas of the set pipeline (eigen workspace, stacked matrices, pivots), grown on demand.
+struct SetWork {
+  EigWork eig;
+  DevBuf<zc> w0, w1, w2, w3;
+  DevBuf<int> ipiv;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  EigWork eig;
-  // scratch for the solver
-  DevBuf<zc> w0, w1, w2, w3, w4;
-  DevBuf<int> ipiv, info;
+  EigWork eig;      // the LAPACK-seam entry points
+  SetWork work;     // the solver
+  std::vector<std::shared_ptr<void>> graphs;  // captured batched solves (solve_batch_graph)
+  DevBuf<int> info;
   // uploaded field table
   DevBuf<unsigned char> ftab;
   DevBuf<float> fz;

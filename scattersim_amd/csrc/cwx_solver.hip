@@ -1019,121 +1019,144 @@ void finish_solution(Solution& sol, const VolumeOptions& o, const InstSet& is, c
   CWX_HIP(hipStreamSynchronize(s));  // d_segs is released on return
   }
 
-void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
-               int s1, int s2, SetResult& res) {
-  hipStream_t s = ctx.stream;
+// Buffers of one set solve.  An ordinary call keeps them until its status read-back; a captured graph
+// keeps them, with work areas of its own, for as long as it is replayed.
+struct Level {
+  size_t oA, oB, oC;
+  int batch;
+};
+
+struct SetBufs {
   InstSet is;
-  make_instances(opts, Ls, s1, s2, is, s);
-  const int count = is.count, Ltot = is.Ltot;
+  int count = 0, Ltot = 0, M0 = 0, M1 = 0, MF = 0, n2 = 0, n3 = 0, n4 = 0, n6 = 0, eig_mode = 0;
+  long nn = 0;
+  bool capture = false, keep_beta = false;
+  Solution* full = nullptr;
+  // Fourier stage
+  DevBuf<zc> dft1, dft2, sq, T1, F;
+  // eigen stage
+  DevBuf<int> d_src, d_info, d_unsafe;
+  DevBuf<zc> gam_local, W, V, QA;
+  DevBuf<PairScale> pscale;
+  DevBuf<zc> Vf;                       // reduced mode: layers redone through the full Phi
+  DevBuf<int> d_fslot, d_flist, d_finfo;
+  std::vector<int> flist;
+  // chain, boundary system, beta
+  DevBuf<zc> Gd_local, Gc_local, beta_local, Y, Abr, A;
+  DevBuf<int> d_info_lu, infoS, infoA, d_nonfinite;
+  DevBuf<zc>* gam = nullptr;
+  DevBuf<zc>* Gd = nullptr;
+  DevBuf<zc>* Gc = nullptr;
+  DevBuf<zc>* beta = nullptr;
+  DevBuf<zc>* x = nullptr;
+  SetWork own;
+  SetWork* wk = nullptr;
+  PtrLists pl;
+  std::vector<Level> levels, bsteps;
+  size_t oTLU = 0, oTP = 0, oTB = 0, oCS = 0, oCD = 0, oGd0 = 0, oLU0 = 0, oPiv0 = 0, oB0 = 0;
+  int nT = 0, ncS = 0;
+};
+
+// Host work of a set solve: instance tables, DFT matrices, every buffer and the pointer lists, all on the
+// device before the first kernel.  With `capture`, the launch sequence that follows may be recorded in a
+// graph: the buffers (and work areas) are the plan's own and the launches need no host decision.
+void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std::vector<int>& Ls, int s1, int s2,
+                 SetResult& res, SetBufs& b, bool capture) {
+  hipStream_t s = ctx.stream;
+  make_instances(opts, Ls, s1, s2, b.is, s);
+  InstSet& is = b.is;
+  b.capture = capture;
+  b.count = is.count;
+  b.Ltot = is.Ltot;
   const Setup& st0 = is.st[0];
-  const int M0 = st0.M0, M1 = st0.M1, MF = st0.MF;
-  const int n2 = 2 * MF, n3 = 3 * MF, n4 = 4 * MF, n6 = 6 * MF;
-  const long nn = (long)n4 * n4;
-  const int eig_mode = opts[0].eig_mode;
+  b.M0 = st0.M0;
+  b.M1 = st0.M1;
+  b.MF = st0.MF;
+  b.n2 = 2 * b.MF;
+  b.n3 = 3 * b.MF;
+  b.n4 = 4 * b.MF;
+  b.n6 = 6 * b.MF;
+  b.nn = (long)b.n4 * b.n4;
+  b.eig_mode = opts[0].eig_mode;
+  b.full = res.full;
+  const int count = b.count, Ltot = b.Ltot, MF = b.MF, n2 = b.n2, n3 = b.n3, n4 = b.n4, n6 = b.n6;
+  const long nn = b.nn, mm = (long)n2 * n2;
   CWX_CHECK(n4 <= SORT_MAX, "4*M0*M1 exceeds the supported maximum of 4096");
   CWX_CHECK(res.full == nullptr || count == 1, "full solutions are kept for single instances only");
-  const InstArgs ia = is.args();
-  Solution* full = res.full;
+  CWX_CHECK(!capture || (res.full == nullptr && res.K == 0), "a captured solve keeps only the boundary solution");
+  b.wk = capture ? &b.own : &ctx.work;
+  SetWork& wk = *b.wk;
 
-  // ---- 1. Fourier coefficients of n^2 and n^-2 of every layer ----
-  DevBuf<zc> F((size_t)2 * Ltot * MF);
-  fourier_stage(st0, Ltot, d_samples, F.p, s);
-  const zc* NF = F.p;
-  const zc* NI = F.p + (long)Ltot * MF;
-
-  // ---- 2-3. Phi and its eigen-decomposition ----
-  DevBuf<int> d_src((size_t)Ltot * n4);
-  DevBuf<zc> gam_local;
-  DevBuf<zc>& gam = full ? full->gam : gam_local;
-  gam.alloc((size_t)Ltot * n4);
-  DevBuf<int> d_info(Ltot);
-  DevBuf<zc> W, V, QA, Vf;
-  DevBuf<PairScale> pscale;
-  DevBuf<int> d_fslot, d_flist, d_finfo;
-  std::vector<int> flist;  // reduced mode: layers solved through the full Phi instead
-  ctx.w0.ensure(std::max<size_t>((size_t)Ltot * nn, 1));
-  auto sort_modes = [&](int reduced, const zc* Wv, int nl, const int* list, int* unsafe) {
-    if (n4 <= 2048)
-      hipLaunchKernelGGL(k_sort_modes<2048>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, d_src.p, list,
-                         unsafe);
-    else
-      hipLaunchKernelGGL(k_sort_modes<SORT_MAX>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, d_src.p, list,
-                         unsafe);
-  };
-  if (eig_mode == 0) {
-    zc* Phi = ctx.w0.p;
-    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, NF, NI, Phi, nullptr);
-    W.alloc((size_t)Ltot * n4);
-    V.alloc((size_t)Ltot * nn);
-    zgeev_batched(n4, Phi, n4, nn, W.p, n4, V.p, n4, nn, Ltot, ctx.eig, d_info.p, s);
-    sort_modes(0, W.p, Ltot, nullptr, nullptr);
-  } else {
-    const long mm = (long)n2 * n2;
-    zc* P = ctx.w0.p;
-    zc* Q = ctx.w0.p + Ltot * mm;
-    ctx.w1.ensure((size_t)Ltot * mm);
-    zc* PQ = ctx.w1.p;
-    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(mm), Ltot), dim3(256), 0, s, ia, NF, NI, P, Q);
-    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), P, n2, mm, Q, n2, mm, mk(0.0), PQ, n2, mm, Ltot, s);
-    W.alloc((size_t)Ltot * n2);
-    V.alloc((size_t)Ltot * mm);
-    zgeev_batched(n2, PQ, n2, mm, W.p, n2, V.p, n2, mm, Ltot, ctx.eig, d_info.p, s);
-    QA.alloc((size_t)Ltot * mm);
-    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), Q, n2, mm, V.p, n2, mm, mk(0.0), QA.p, n2, mm, Ltot, s);
-    DevBuf<int> d_unsafe(Ltot);
-    sort_modes(1, W.p, Ltot, nullptr, d_unsafe.p);
-    pscale.alloc((size_t)Ltot * n2);
-    hipLaunchKernelGGL(k_pair_scale, dim3(n2, Ltot), dim3(64), 0, s, n2, V.p, QA.p, W.p, pscale.p);
-    std::vector<int> hunsafe(Ltot);
-    CWX_HIP(hipMemcpyAsync(hunsafe.data(), d_unsafe.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
-    CWX_HIP(hipStreamSynchronize(s));
-    std::vector<int> fslot(Ltot, -1);
-    for (int l = 0; l < Ltot; ++l)
-      if (hunsafe[l]) {
-        fslot[l] = (int)flist.size();
-        flist.push_back(l);
+  // ---- Fourier stage: a pruned DFT F = w1 (M1 x s1) . A (s1 x s2) . w2 (s2 x M0) / (s1 s2) per layer,
+  //      with A the column-major layer matrix (FourierWave.h:152-195, 288, 341-343) ----
+  {
+    const int M0 = b.M0, M1 = b.M1;
+    std::vector<zc> w1((size_t)M1 * s1), w2((size_t)s2 * M0);
+    for (int i = 0; i < s1; ++i)
+      for (int r = 0; r < M1; ++r) {
+        const long t = ((long)st0.rows[r] * i) % s1;
+        const double ang = -2.0 * kPi * (double)t / (double)s1;
+        w1[(size_t)i * M1 + r] = mk(std::cos(ang) / ((double)s1 * s2), std::sin(ang) / ((double)s1 * s2));
       }
-    if (!flist.empty()) {
-      // nearly defective pairs: those layers' modes come from zgeev of the full Phi (P, Q are no longer needed)
-      const int nf = (int)flist.size();
-      upload(d_flist, flist, s);
-      upload(d_fslot, fslot, s);
-      zc* Phi = ctx.w0.p;
-      hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), nf), dim3(256), 0, s, ia, NF, NI, Phi, d_flist.p);
-      DevBuf<zc> Wf((size_t)nf * n4);
-      Vf.alloc((size_t)nf * nn);
-      d_finfo.alloc(nf);
-      zgeev_batched(n4, Phi, n4, nn, Wf.p, n4, Vf.p, n4, nn, nf, ctx.eig, d_finfo.p, s);
-      sort_modes(0, Wf.p, nf, d_flist.p, nullptr);
-      CWX_HIP(hipStreamSynchronize(s));  // Wf is released on return
-    }
+    for (int c = 0; c < M0; ++c)
+      for (int j = 0; j < s2; ++j) {
+        const long t = ((long)st0.cols[c] * j) % s2;
+        const double ang = -2.0 * kPi * (double)t / (double)s2;
+        w2[(size_t)c * s2 + j] = mk(std::cos(ang), std::sin(ang));
+      }
+    upload(b.dft1, w1, s);
+    upload(b.dft2, w2, s);
+    const long npix = (long)s1 * s2;
+    b.sq.ensure((size_t)2 * npix * Ltot);
+    b.T1.ensure((size_t)2 * Ltot * s1 * M0);
+    b.F.ensure((size_t)2 * Ltot * MF);
   }
-  DevBuf<zc> Gd_local, Gc_local;
-  DevBuf<zc>& Gd = full ? full->Gd : Gd_local;
-  DevBuf<zc>& Gc = full ? full->Gc : Gc_local;
-  Gd.alloc((size_t)Ltot * nn);
-  Gc.alloc((size_t)Ltot * nn);
-  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, V.p, QA.p, pscale.p,
-                     Vf.p, d_fslot.p, gam.p, d_src.p, is.dz.p, Gd.p, Gc.p);
-  V.release();
-  QA.release();
-  W.release();
-  Vf.release();
-  pscale.release();
-
-  // ---- 4. LU of every Gd, T_g = Gd_g^-1 Gc_{g-1} (g not first in its instance), and per instance
-  //         S = Gc_last T_last ... T_first+1 by a pairwise product tree (ScatterVolume.cpp:327-336) ----
-  zc* LUd = ctx.w0.p;  // Phi / P,Q no longer needed
-  hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * nn)), dim3(256), 0, s, (long)Ltot * nn, Gd.p, LUd);
-  ctx.ipiv.ensure((size_t)Ltot * n4 + (size_t)count * (n4 + n6));
-  int* piv = ctx.ipiv.p;
-  int* pivS = piv + (long)Ltot * n4;
-  int* pivA = pivS + (long)count * n4;
-  DevBuf<int> d_info_lu(Ltot), infoS(count), infoA(count);
-  zgetrf_batched(n4, LUd, n4, nn, piv, n4, d_info_lu.p, Ltot, s);
-  ctx.w1.ensure((size_t)Ltot * nn);
-  zc* Tm = ctx.w1.p;  // T_g at slot g
-  PtrLists pl;
+  // ---- eigen stage ----
+  b.d_src.ensure((size_t)Ltot * n4);
+  b.gam = res.full ? &res.full->gam : &b.gam_local;
+  b.gam->alloc((size_t)Ltot * n4);
+  b.d_info.ensure(Ltot);
+  b.d_unsafe.ensure(Ltot);
+  wk.w0.ensure(std::max<size_t>((size_t)Ltot * nn, 1));
+  wk.w1.ensure((size_t)Ltot * nn);
+  if (b.eig_mode == 0) {
+    b.W.ensure((size_t)Ltot * n4);
+    b.V.ensure((size_t)Ltot * nn);
+    wk.eig.ensure(n4, Ltot);
+  } else {
+    b.W.ensure((size_t)Ltot * n2);
+    b.V.ensure((size_t)Ltot * mm);
+    b.QA.ensure((size_t)Ltot * mm);
+    b.pscale.ensure((size_t)Ltot * n2);
+    wk.eig.ensure(n2, Ltot);
+  }
+  // ---- chain, boundary system, beta ----
+  b.Gd = res.full ? &res.full->Gd : &b.Gd_local;
+  b.Gc = res.full ? &res.full->Gc : &b.Gc_local;
+  b.Gd->alloc((size_t)Ltot * nn);
+  b.Gc->alloc((size_t)Ltot * nn);
+  wk.ipiv.ensure((size_t)Ltot * n4 + (size_t)count * (n4 + n6));
+  wk.w2.ensure((size_t)count * nn);
+  wk.w3.ensure((size_t)std::max(1, Ltot) * nn);
+  b.d_info_lu.ensure(Ltot);
+  b.infoS.ensure(count);
+  b.infoA.ensure(count);
+  b.d_nonfinite.ensure(count);
+  b.Y.ensure((size_t)count * n4 * n3);
+  b.Abr.ensure((size_t)count * n4 * n3);
+  b.A.ensure((size_t)count * n6 * n6);
+  b.x = res.x;
+  b.x->alloc((size_t)count * n6);
+  b.beta = res.full ? &res.full->beta : &b.beta_local;
+  const bool keep_beta = res.full && !opts[0].external;
+  b.keep_beta = keep_beta;
+  if (keep_beta) b.beta->alloc((size_t)Ltot * n4);
+  // pointer lists
+  zc* LUd = wk.w0.p;
+  zc* Tm = wk.w1.p;
+  int* piv = wk.ipiv.p;
+  zc* Sm = wk.w2.p;
+  zc* pool = wk.w3.p;
   std::vector<void*> tLU, tP, tB;
   for (int i = 0; i < count; ++i)
     for (int l = 1; l < is.L[i]; ++l) {
@@ -1142,107 +1165,203 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
       tP.push_back(piv + g * n4);
       tB.push_back(Tm + g * nn);
     }
-  const int nT = (int)tB.size();
-  const size_t oTLU = pl.add(tLU), oTP = pl.add(tP), oTB = pl.add(tB);
+  b.nT = (int)tB.size();
+  b.oTLU = b.pl.add(tLU);
+  b.oTP = b.pl.add(tP);
+  b.oTB = b.pl.add(tB);
   // product tree: level items are pointers; pairs (X_{2m+1}, X_{2m}) -> X_{2m+1} X_{2m}
-  ctx.w2.ensure((size_t)count * nn);
-  zc* Sm = ctx.w2.p;
-  ctx.w3.ensure((size_t)std::max(1, Ltot) * nn);
-  zc* pool = ctx.w3.p;
   long pool_used = 0;
   std::vector<std::vector<const zc*>> seq(count);
-  std::vector<int> single;  // instances with one layer: S = Gc of that layer
   for (int i = 0; i < count; ++i) {
     const int f = is.first[i], L = is.L[i];
     for (int l = 1; l < L; ++l) seq[i].push_back(Tm + (long)(f + l) * nn);
-    seq[i].push_back(Gc.p + (long)(f + L - 1) * nn);
+    seq[i].push_back(b.Gc->p + (long)(f + L - 1) * nn);
   }
-  struct Level {
-    size_t oA, oB, oC;
-    int batch;
-  };
-  std::vector<Level> levels;
   for (;;) {
-    std::vector<void*> A, B, Cp;
+    std::vector<void*> Av, Bv, Cv;
     for (int i = 0; i < count; ++i) {
       auto& q = seq[i];
       if (q.size() < 2) continue;
       std::vector<const zc*> next;
       for (size_t m = 0; m + 1 < q.size(); m += 2) {
         zc* out = (q.size() == 2) ? Sm + (long)i * nn : pool + (pool_used++) * nn;
-        A.push_back((void*)q[m + 1]);
-        B.push_back((void*)q[m]);
-        Cp.push_back(out);
+        Av.push_back((void*)q[m + 1]);
+        Bv.push_back((void*)q[m]);
+        Cv.push_back(out);
         next.push_back(out);
       }
       if (q.size() % 2) next.push_back(q.back());
       q = next;
     }
-    if (A.empty()) break;
-    levels.push_back(Level{pl.add(A), pl.add(B), pl.add(Cp), (int)A.size()});
+    if (Av.empty()) break;
+    b.levels.push_back(Level{b.pl.add(Av), b.pl.add(Bv), b.pl.add(Cv), (int)Av.size()});
   }
   CWX_CHECK(pool_used <= std::max(1, Ltot), "product tree exceeded its pool");
   std::vector<void*> cS, cD;
   for (int i = 0; i < count; ++i)
     if (is.L[i] == 1) {
-      cS.push_back(Gc.p + (long)is.first[i] * nn);
+      cS.push_back(b.Gc->p + (long)is.first[i] * nn);
       cD.push_back(Sm + (long)i * nn);
     }
-  const size_t oCS = pl.add(cS), oCD = pl.add(cD);
-  // boundary and beta pointer lists
+  b.ncS = (int)cS.size();
+  b.oCS = b.pl.add(cS);
+  b.oCD = b.pl.add(cD);
   std::vector<void*> gd0, lu0, piv0, b0;
-  DevBuf<zc> beta_local;
-  const bool modes = (full != nullptr);
-  DevBuf<zc>& beta = full ? full->beta : beta_local;
-  if (modes && !opts[0].external) beta.alloc((size_t)Ltot * n4);
   for (int i = 0; i < count; ++i) {
     const long f = is.first[i];
-    gd0.push_back(Gd.p + f * nn);
+    gd0.push_back(b.Gd->p + f * nn);
     lu0.push_back(LUd + f * nn);
     piv0.push_back(piv + f * n4);
-    b0.push_back(beta.p ? beta.p + f * n4 : nullptr);
+    b0.push_back(keep_beta ? b.beta->p + f * n4 : nullptr);
   }
-  const size_t oGd0 = pl.add(gd0), oLU0 = pl.add(lu0), oPiv0 = pl.add(piv0), oB0 = pl.add(b0);
-  std::vector<Level> bsteps;  // beta_j = T_j beta_{j-1} for every instance with more than j layers
-  if (beta.p) {
+  b.oGd0 = b.pl.add(gd0);
+  b.oLU0 = b.pl.add(lu0);
+  b.oPiv0 = b.pl.add(piv0);
+  b.oB0 = b.pl.add(b0);
+  if (keep_beta) {
+    // beta_j = T_j beta_{j-1} for every instance with more than j layers
     for (int j = 1; j < is.maxL; ++j) {
-      std::vector<void*> A, B, Cp;
+      std::vector<void*> Av, Bv, Cv;
       for (int i = 0; i < count; ++i)
         if (is.L[i] > j) {
           const long g = is.first[i] + j;
-          A.push_back(Tm + g * nn);
-          B.push_back(beta.p + (g - 1) * n4);
-          Cp.push_back(beta.p + g * n4);
+          Av.push_back(Tm + g * nn);
+          Bv.push_back(b.beta->p + (g - 1) * n4);
+          Cv.push_back(b.beta->p + g * n4);
         }
-      bsteps.push_back(Level{pl.add(A), pl.add(B), pl.add(Cp), (int)A.size()});
+      b.bsteps.push_back(Level{b.pl.add(Av), b.pl.add(Bv), b.pl.add(Cv), (int)Av.size()});
     }
   }
-  pl.commit(s);
+  b.pl.commit(s);
+  CWX_HIP(hipGetLastError());
+}
 
-  if (nT > 0) {
+// The kernels of a set solve, in stream order, on the buffers set_prepare made.  Without `capture` the
+// reduced mode may redo nearly defective layers through the full Phi (a host decision after the sort),
+// and a full solution gets its Q rows and .cw records; with `capture` nothing but launches is issued.
+void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, SetResult& res) {
+  hipStream_t s = ctx.stream;
+  const InstSet& is = b.is;
+  const InstArgs ia = is.args();
+  SetWork& wk = *b.wk;
+  const bool capture = b.capture;
+  const int count = b.count, Ltot = b.Ltot, MF = b.MF, n2 = b.n2, n3 = b.n3, n4 = b.n4, n6 = b.n6;
+  const int M0 = b.M0;
+  const long nn = b.nn, mm = (long)n2 * n2;
+  const int eig_mode = b.eig_mode;
+  Solution* full = b.full;
+
+  // ---- 1. Fourier coefficients of n^2 and n^-2 of every layer ----
+  {
+    const long npix = (long)s1 * s2;
+    hipLaunchKernelGGL(k_square_layers, dim3(grid_for(npix * Ltot)), dim3(256), 0, s, npix * Ltot, d_samples, b.sq.p);
+    for (int b0 = 0; b0 < 2 * Ltot; b0 += 65535) {
+      const int nb = std::min(65535, 2 * Ltot - b0);
+      zgemm_batched(OpN, OpN, s1, M0, s2, mk(1.0), b.sq.p + b0 * npix, s1, npix, b.dft2.p, s2, 0, mk(0.0),
+                    b.T1.p + (long)b0 * s1 * M0, s1, (long)s1 * M0, nb, s);
+      zgemm_batched(OpN, OpN, b.M1, M0, s1, mk(1.0), b.dft1.p, b.M1, 0, b.T1.p + (long)b0 * s1 * M0, s1,
+                    (long)s1 * M0, mk(0.0), b.F.p + (long)b0 * MF, b.M1, MF, nb, s);
+    }
+  }
+  const zc* NF = b.F.p;
+  const zc* NI = b.F.p + (long)Ltot * MF;
+
+  // ---- 2-3. Phi and its eigen-decomposition ----
+  DevBuf<zc>& gam = *b.gam;
+  auto sort_modes = [&](int reduced, const zc* Wv, int nl, const int* list, int* unsafe) {
+    if (n4 <= 2048)
+      hipLaunchKernelGGL(k_sort_modes<2048>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, b.d_src.p, list,
+                         unsafe);
+    else
+      hipLaunchKernelGGL(k_sort_modes<SORT_MAX>, dim3(nl), dim3(256), 0, s, n4, reduced, Wv, gam.p, b.d_src.p, list,
+                         unsafe);
+  };
+  if (eig_mode == 0) {
+    zc* Phi = wk.w0.p;
+    hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, NF, NI, Phi, nullptr);
+    zgeev_batched(n4, Phi, n4, nn, b.W.p, n4, b.V.p, n4, nn, Ltot, wk.eig, b.d_info.p, s, !capture);
+    sort_modes(0, b.W.p, Ltot, nullptr, nullptr);
+  } else {
+    zc* P = wk.w0.p;
+    zc* Q = wk.w0.p + Ltot * mm;
+    zc* PQ = wk.w1.p;
+    hipLaunchKernelGGL(k_phi_blocks, dim3(grid_for(mm), Ltot), dim3(256), 0, s, ia, NF, NI, P, Q);
+    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), P, n2, mm, Q, n2, mm, mk(0.0), PQ, n2, mm, Ltot, s);
+    zgeev_batched(n2, PQ, n2, mm, b.W.p, n2, b.V.p, n2, mm, Ltot, wk.eig, b.d_info.p, s, !capture);
+    zgemm_batched(OpN, OpN, n2, n2, n2, mk(1.0), Q, n2, mm, b.V.p, n2, mm, mk(0.0), b.QA.p, n2, mm, Ltot, s);
+    sort_modes(1, b.W.p, Ltot, nullptr, b.d_unsafe.p);
+    hipLaunchKernelGGL(k_pair_scale, dim3(n2, Ltot), dim3(64), 0, s, n2, b.V.p, b.QA.p, b.W.p, b.pscale.p);
+    if (!capture) {
+      std::vector<int> hunsafe(Ltot);
+      CWX_HIP(hipMemcpyAsync(hunsafe.data(), b.d_unsafe.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+      CWX_HIP(hipStreamSynchronize(s));
+      std::vector<int> fslot(Ltot, -1);
+      for (int l = 0; l < Ltot; ++l)
+        if (hunsafe[l]) {
+          fslot[l] = (int)b.flist.size();
+          b.flist.push_back(l);
+        }
+      if (!b.flist.empty()) {
+        // nearly defective pairs: those layers' modes come from zgeev of the full Phi (P, Q are no longer needed)
+        const int nf = (int)b.flist.size();
+        upload(b.d_flist, b.flist, s);
+        upload(b.d_fslot, fslot, s);
+        zc* Phi = wk.w0.p;
+        hipLaunchKernelGGL(k_phi_full, dim3(grid_for(nn), nf), dim3(256), 0, s, ia, NF, NI, Phi, b.d_flist.p);
+        DevBuf<zc> Wf((size_t)nf * n4);
+        b.Vf.alloc((size_t)nf * nn);
+        b.d_finfo.alloc(nf);
+        zgeev_batched(n4, Phi, n4, nn, Wf.p, n4, b.Vf.p, n4, nn, nf, wk.eig, b.d_finfo.p, s);
+        sort_modes(0, Wf.p, nf, b.d_flist.p, nullptr);
+        CWX_HIP(hipStreamSynchronize(s));  // Wf is released on return
+      }
+    }
+  }
+  DevBuf<zc>& Gd = *b.Gd;
+  DevBuf<zc>& Gc = *b.Gc;
+  hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, b.V.p, b.QA.p,
+                     b.pscale.p, b.Vf.p, b.d_fslot.p, gam.p, b.d_src.p, is.dz.p, Gd.p, Gc.p);
+  if (!capture) {
+    b.V.release();
+    b.QA.release();
+    b.W.release();
+    b.Vf.release();
+    b.pscale.release();
+  }
+
+  // ---- 4. LU of every Gd, T_g = Gd_g^-1 Gc_{g-1} (g not first in its instance), and per instance
+  //         S = Gc_last T_last ... T_first+1 by a pairwise product tree (ScatterVolume.cpp:327-336) ----
+  zc* LUd = wk.w0.p;  // Phi / P,Q no longer needed
+  hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * nn)), dim3(256), 0, s, (long)Ltot * nn, Gd.p, LUd);
+  int* piv = wk.ipiv.p;
+  int* pivS = piv + (long)Ltot * n4;
+  int* pivA = pivS + (long)count * n4;
+  zgetrf_batched(n4, LUd, n4, nn, piv, n4, b.d_info_lu.p, Ltot, s);
+  zc* Tm = wk.w1.p;  // T_g at slot g
+  zc* Sm = wk.w2.p;
+  const PtrLists& pl = b.pl;
+  if (b.nT > 0) {
     hipLaunchKernelGGL(k_copy, dim3(grid_for((long)(Ltot - 1) * nn)), dim3(256), 0, s, (long)(Ltot - 1) * nn, Gc.p,
                        Tm + nn);
-    zgetrs_set(n4, n4, pl.mats(oTLU), n4, pl.ints(oTP), pl.mats(oTB), n4, nT, s);
+    zgetrs_set(n4, n4, pl.mats(b.oTLU), n4, pl.ints(b.oTP), pl.mats(b.oTB), n4, b.nT, s);
   }
-  for (const Level& lv : levels)
+  for (const Level& lv : b.levels)
     zgemm_set(OpN, OpN, n4, n4, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
               lv.batch, s);
-  if (!cS.empty())
-    hipLaunchKernelGGL(k_copy_set, dim3(grid_for(nn), (unsigned)cS.size()), dim3(256), 0, s, nn, pl.mats(oCS),
-                       pl.mats(oCD));
+  if (b.ncS > 0)
+    hipLaunchKernelGGL(k_copy_set, dim3(grid_for(nn), (unsigned)b.ncS), dim3(256), 0, s, nn, pl.mats(b.oCS),
+                       pl.mats(b.oCD));
 
   // ---- 5. boundary system per instance: A = [Gauss; f2 | Gd0 S^-1 f3], b = [0; -f1 EF] ----
-  DevBuf<zc> Y((size_t)count * n4 * n3), Abr((size_t)count * n4 * n3), A((size_t)count * n6 * n6);
-  DevBuf<zc>& x = *res.x;
-  x.alloc((size_t)count * n6);
-  zgetrf_batched(n4, Sm, n4, nn, pivS, n4, infoS.p, count, s);
-  hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), count), dim3(256), 0, s, ia, 3, Y.p);
-  zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, Y.p, n4, (long)n4 * n3, count, s);
-  zgemm_set(OpN, OpN, n4, n3, n4, mk(1.0), pl.mats(oGd0), n4, strided<zc>(Y.p, (long)n4 * n3), n4, mk(0.0),
-            strided<zc>(Abr.p, (long)n4 * n3), n4, count, s);
-  hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6), count), dim3(256), 0, s, ia, Abr.p, A.p, x.p);
-  zgetrf_batched(n6, A.p, n6, (long)n6 * n6, pivA, n6, infoA.p, count, s);
-  zgetrs_batched(n6, 1, A.p, n6, (long)n6 * n6, pivA, n6, x.p, n6, n6, count, s);
+  zc* xp = b.x->p;
+  zgetrf_batched(n4, Sm, n4, nn, pivS, n4, b.infoS.p, count, s);
+  hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), count), dim3(256), 0, s, ia, 3, b.Y.p);
+  zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, b.Y.p, n4, (long)n4 * n3, count, s);
+  zgemm_set(OpN, OpN, n4, n3, n4, mk(1.0), pl.mats(b.oGd0), n4, strided<zc>(b.Y.p, (long)n4 * n3), n4, mk(0.0),
+            strided<zc>(b.Abr.p, (long)n4 * n3), n4, count, s);
+  hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6), count), dim3(256), 0, s, ia, b.Abr.p, b.A.p, xp);
+  zgetrf_batched(n6, b.A.p, n6, (long)n6 * n6, pivA, n6, b.infoA.p, count, s);
+  zgetrs_batched(n6, 1, b.A.p, n6, (long)n6 * n6, pivA, n6, xp, n6, n6, count, s);
   if (res.K > 0) {
     // b_k = [0; -f1 EF_k] for every further field, then the same factors (ScatterVolume.cpp:739-740 per field)
     CWX_CHECK(count == 1, "several incident fields are solved for one instance at a time");
@@ -1254,43 +1373,57 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
       const int kb = std::min(65535, K - k0);
       zgemm_batched(OpN, OpN, n4, kb, n3, mk(-1.0), F1.p, n4, 0, res.EFs + (long)k0 * n3, n3, 0, mk(0.0),
                     res.xs + (long)k0 * n6 + n2, n6, 0, 1, s);
-      zgetrs_batched(n6, kb, A.p, n6, (long)n6 * n6, pivA, n6, res.xs + (long)k0 * n6, n6, 0, 1, s);
+      zgetrs_batched(n6, kb, b.A.p, n6, (long)n6 * n6, pivA, n6, res.xs + (long)k0 * n6, n6, 0, 1, s);
     }
     CWX_HIP(hipStreamSynchronize(s));  // F1 is released on return
   }
 
   // ---- 6. beta recurrence, Q_check / Q_hat, Ez rows ----
-  if (beta.p) {
-    hipLaunchKernelGGL(k_beta0_rhs, dim3(grid_for(n4), count), dim3(256), 0, s, ia, x.p, pl.mats(oB0));
-    zgetrs_set(n4, 1, pl.mats(oLU0), n4, pl.ints(oPiv0), pl.mats(oB0), n4, count, s);
-    for (const Level& lv : bsteps)
+  if (b.keep_beta) {
+    hipLaunchKernelGGL(k_beta0_rhs, dim3(grid_for(n4), count), dim3(256), 0, s, ia, xp, pl.mats(b.oB0));
+    zgetrs_set(n4, 1, pl.mats(b.oLU0), n4, pl.ints(b.oPiv0), pl.mats(b.oB0), n4, count, s);
+    for (const Level& lv : b.bsteps)
       zgemm_set(OpN, OpN, n4, 1, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
                 lv.batch, s);
   }
-  if (full) finish_solution(*full, opts[0], is, NI, s);
-  CWX_HIP(hipGetLastError());
-
-  // ---- status per instance ----
-  std::vector<int> hinfo(Ltot), hlu(Ltot), hS(count), hA(count);
-  CWX_HIP(hipMemcpyAsync(hinfo.data(), d_info.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hlu.data(), d_info_lu.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hS.data(), infoS.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hA.data(), infoA.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
-  std::vector<int> hfinfo(flist.size());
-  if (!flist.empty())
-    CWX_HIP(hipMemcpyAsync(hfinfo.data(), d_finfo.p, sizeof(int) * flist.size(), hipMemcpyDeviceToHost, s));
   // a solution with a value that is not finite is a failure, whatever the pivots said
-  DevBuf<int> d_nonfinite(count);
-  hipLaunchKernelGGL(k_count_nonfinite, dim3(count), dim3(256), 0, s, n6, x.p, d_nonfinite.p);
-  std::vector<int> hnf(count);
-  CWX_HIP(hipMemcpyAsync(hnf.data(), d_nonfinite.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_count_nonfinite, dim3(count), dim3(256), 0, s, n6, xp, b.d_nonfinite.p);
+  CWX_HIP(hipGetLastError());
+}
+
+// Status of every instance after set_launch (one synchronisation).  Solutions kept in full get their Q rows
+// and records first.  In a captured solve an instance whose reduced eigenproblem had a nearly defective
+// pair reports kUnsafeReduced: it is to be solved again without the graph.
+constexpr int kUnsafeReduced = 1 << 29;
+
+void set_status(Context& ctx, SetBufs& b, const std::vector<VolumeOptions>& opts, SetResult& res) {
+  hipStream_t s = ctx.stream;
+  const InstSet& is = b.is;
+  const int count = b.count, Ltot = b.Ltot, n6 = b.n6;
+  if (b.full) finish_solution(*b.full, opts[0], is, b.F.p + (long)Ltot * b.MF, s);
+  std::vector<int> hinfo(Ltot), hlu(Ltot), hS(count), hA(count), hnf(count), hun(Ltot, 0);
+  CWX_HIP(hipMemcpyAsync(hinfo.data(), b.d_info.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hlu.data(), b.d_info_lu.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hS.data(), b.infoS.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hA.data(), b.infoA.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hnf.data(), b.d_nonfinite.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  if (b.capture && b.eig_mode == 1)
+    CWX_HIP(hipMemcpyAsync(hun.data(), b.d_unsafe.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  std::vector<int> hfinfo(b.flist.size());
+  if (!b.flist.empty())
+    CWX_HIP(hipMemcpyAsync(hfinfo.data(), b.d_finfo.p, sizeof(int) * b.flist.size(), hipMemcpyDeviceToHost, s));
   CWX_HIP(hipStreamSynchronize(s));
-  for (size_t f = 0; f < flist.size(); ++f) hinfo[flist[f]] = hfinfo[f];
+  for (size_t f = 0; f < b.flist.size(); ++f) hinfo[b.flist[f]] = hfinfo[f];
   res.info.assign(count, 0);
   res.msg.assign(count, std::string());
   for (int i = 0; i < count; ++i) {
     int& inf = res.info[i];
     std::string& m = res.msg[i];
+    for (int l = 0; l < is.L[i] && !inf; ++l)
+      if (hun[is.first[i] + l]) {
+        inf = kUnsafeReduced;
+        m = "layer " + std::to_string(l) + ": nearly defective pair in the reduced eigenproblem";
+      }
     for (int l = 0; l < is.L[i] && !inf; ++l) {
       const int g = is.first[i] + l;
       if (hinfo[g] != 0) {
@@ -1312,6 +1445,14 @@ void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d
       m = std::to_string(hnf[i]) + " entries of the boundary solution are not finite";
     }
   }
+}
+
+void solve_set(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples, const std::vector<int>& Ls,
+               int s1, int s2, SetResult& res) {
+  SetBufs b;
+  set_prepare(ctx, opts, Ls, s1, s2, res, b, false);
+  set_launch(ctx, b, d_samples, s1, s2, res);
+  set_status(ctx, b, opts, res);
 }
 
 }  // namespace
@@ -1377,6 +1518,120 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
     layer0 += layers;
     i0 = i1;
   }
+  return worst;
+}
+
+namespace {
+
+// A captured set solve: its plan (buffers, tables, pointer lists), its sample and result areas and the
+// instantiated graph of its launches, kept in the context and replayed for every chunk whose instances
+// have the same layer counts, orders and eigen mode.
+struct SetGraph {
+  std::vector<int> Ls;
+  int s1 = 0, s2 = 0, M0 = 0, M1 = 0, eig_mode = 0;
+  SetBufs b;
+  DevBuf<zc> samples, x;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~SetGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+}  // namespace
+
+int solve_batch_graph(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* d_samples,
+                      const std::vector<int>& Ls, int s1, int s2, int chunk, zc* d_x, std::vector<int>& info,
+                      std::string& first_msg, int* captured) {
+  const int count = (int)opts.size();
+  CWX_CHECK(count >= 1 && (int)Ls.size() == count, "batch is empty");
+  CWX_CHECK(chunk >= 1, "chunk must be at least one instance");
+  hipStream_t s = ctx.stream;
+  const Setup st0 = make_setup(opts[0], Ls[0], s1, s2);
+  const long n6 = 6L * st0.MF, npix = (long)s1 * s2;
+  info.assign(count, 0);
+  first_msg.clear();
+  int worst = 0, ncap = 0;
+  long layer0 = 0;
+  for (int i0 = 0; i0 < count; i0 += chunk) {
+    const int i1 = std::min(count, i0 + chunk);
+    std::vector<VolumeOptions> o(opts.begin() + i0, opts.begin() + i1);
+    std::vector<int> l(Ls.begin() + i0, Ls.begin() + i1);
+    long layers = 0;
+    for (int v : l) layers += v;
+    const Setup sc = make_setup(o[0], l[0], s1, s2);
+    SetGraph* g = nullptr;
+    for (auto& e : ctx.graphs) {
+      SetGraph* c = static_cast<SetGraph*>(e.get());
+      if (c->Ls == l && c->s1 == s1 && c->s2 == s2 && c->M0 == sc.M0 && c->M1 == sc.M1 &&
+          c->eig_mode == o[0].eig_mode) {
+        g = c;
+        break;
+      }
+    }
+    SetResult res;
+    if (!g) {
+      auto ng = std::make_shared<SetGraph>();
+      ng->Ls = l;
+      ng->s1 = s1;
+      ng->s2 = s2;
+      ng->M0 = sc.M0;
+      ng->M1 = sc.M1;
+      ng->eig_mode = o[0].eig_mode;
+      res.x = &ng->x;
+      set_prepare(ctx, o, l, s1, s2, res, ng->b, true);
+      ng->samples.alloc((size_t)layers * npix);
+      CWX_HIP(hipStreamSynchronize(s));
+      CWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        set_launch(ctx, ng->b, ng->samples.p, s1, s2, res);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(s, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      CWX_HIP(hipStreamEndCapture(s, &ng->graph));
+      CWX_HIP(hipGraphInstantiate(&ng->exec, ng->graph, nullptr, nullptr, 0));
+      ctx.graphs.push_back(ng);
+      g = ng.get();
+      ++ncap;
+    } else {
+      // the chunk's own wavelengths, incidences and tables into the plan's buffers (same sizes)
+      make_instances(o, l, s1, s2, g->b.is, s);
+    }
+    CWX_HIP(hipMemcpyAsync(g->samples.p, d_samples + layer0 * npix, sizeof(zc) * layers * npix,
+                           hipMemcpyDeviceToDevice, s));
+    CWX_HIP(hipGraphLaunch(g->exec, s));
+    CWX_HIP(hipMemcpyAsync(d_x + (long)i0 * n6, g->x.p, sizeof(zc) * n6 * (i1 - i0), hipMemcpyDeviceToDevice, s));
+    set_status(ctx, g->b, o, res);
+    long lay = layer0;
+    for (int i = i0; i < i1; ++i) {
+      int inf = res.info[i - i0];
+      std::string msg = res.msg[i - i0];
+      if (inf == kUnsafeReduced) {
+        // a nearly defective reduced pair: this instance again, outside the graph, where its layers can
+        // take the full Phi
+        DevBuf<zc> xi;
+        SetResult r1;
+        r1.x = &xi;
+        solve_set(ctx, {opts[i]}, d_samples + lay * npix, {Ls[i]}, s1, s2, r1);
+        CWX_HIP(hipMemcpyAsync(d_x + (long)i * n6, xi.p, sizeof(zc) * n6, hipMemcpyDeviceToDevice, s));
+        CWX_HIP(hipStreamSynchronize(s));
+        inf = r1.info[0];
+        msg = r1.msg[0];
+      }
+      info[i] = inf;
+      if (inf && !worst) {
+        worst = inf;
+        first_msg = "instance " + std::to_string(i) + ": " + msg;
+      }
+      lay += Ls[i];
+    }
+    layer0 += layers;
+  }
+  if (captured) *captured = ncap;
   return worst;
 }
 

@@ -99,3 +99,35 @@ def test_batch_with_mixed_orders(ctx):
         assert np.array_equal(sol.x, x)
         sol.free()
     assert orders[1] == (5, 5) and orders[5] == (3, 5)
+
+
+@pytest.mark.parametrize("eig_mode,count,chunk", [(1, 96, 32), (0, 32, 16)])
+def test_graph_replays_equal_stream_solve(ctx, eig_mode, count, chunk):
+    """The batched sweep through captured HIP graphs (cwx_solve_batch_graph_dev): chunks of `chunk`
+    instances (the sweep's 16 stacks in the same order in every chunk) replay the graph captured for the
+    first one.  Every replay equals the stream-ordered batched solve bit for bit, on two calls with other
+    work and synchronisations in between (the second call captures nothing)."""
+    from scattersim_amd import workloads
+    packed, Ls, params = workloads.sweep(count)
+    for p in params:
+        p["eig_mode"] = eig_mode
+    N = packed.shape[1]
+    x_ref, info, (M0, M1) = sa.solve_batch(ctx, [packed[a:a + l] for a, l in
+                                                 zip(np.concatenate([[0], np.cumsum(Ls)[:-1]]), Ls)], params)
+    assert not np.any(info)
+    n6 = 6 * M0 * M1
+    d_s = sa.DeviceArray.from_numpy(ctx, packed)
+    d_x = sa.DeviceArray(ctx, (count, n6), np.complex128)
+    for call in range(2):
+        ginfo, captured = sa.solve_batch_graph(ctx, d_s.ptr, params, Ls, N, N, chunk, d_x.ptr)
+        assert not np.any(ginfo)
+        assert captured == (1 if call == 0 else 0)
+        xg = d_x.numpy()
+        for c0 in range(0, count, chunk):
+            assert np.array_equal(xg[c0:c0 + chunk], x_ref[c0:c0 + chunk]), (call, c0)
+        # eager work between the calls
+        sol = sa.scattervolume(ctx, packed[:Ls[0]], **params[0])
+        assert np.array_equal(sol.x, x_ref[0])
+        sol.free()
+        ctx.synchronize()
+        d_x.numpy()[:] = 0
