@@ -52,6 +52,7 @@ const char* cwx_last_error(void);
  * built-in choice and CWX_PARAM_TABLE returning to the table.  Keys:
  *   "qr_team"          helper workgroups per matrix of the QR iteration when the matrices do not fill
  *                      the device (0: none); results do not depend on it
+ *   "lahr2_team"       helper workgroups per matrix of each Hessenberg panel in that case (0: none)
  *   "qr_threads"       256 or 512 threads per QR workgroup when they do
  *   "batch_chunk_mib"  device memory of one chunk of cwx_solve_batch_dev when max_bytes is 0
  * cwx_tuning_table returns the path of the table in use ("" if none).  Host-only calls: no device work. */

@@ -133,6 +133,8 @@ int cwx_set_param(const char* name, int64_t value) {
     const bool unset = value == CWX_PARAM_TABLE;
     if (k == "qr_team") {
       CWX_CHECK(unset || (value >= -1 && value <= 64), "qr_team takes -1 (built-in choice), 0 (off) or a helper count");
+    } else if (k == "lahr2_team") {
+      CWX_CHECK(unset || (value >= -1 && value <= 64), "lahr2_team takes -1 (built-in choice), 0 (off) or a helper count");
     } else if (k == "qr_threads") {
       CWX_CHECK(unset || value == -1 || value == 256 || value == 512, "qr_threads takes 256 or 512");
     } else if (k == "batch_chunk_mib") {

@@ -48,6 +48,70 @@ __device__ inline zc block_sum(zc v, zc* red) {
   return s;
 }
 
+// ---- hand-offs between the workgroups of one matrix (Hessenberg panels and QR iteration) ----
+// Counters are polled relaxed (a coherent load, no cache invalidation); the acquire fence follows once
+// the awaited value has been seen.  One thread of a workgroup performs each release and acquire, next
+// to a workgroup barrier: the cache operations they stand for act on the caches the whole workgroup
+// shares.  Every wait ends, on its counter, on an abort flag or after `limit` wall-clock ticks.
+constexpr int CTL_INTS = 16, CTL_DONE = 8, TEAM_MAX = CTL_INTS - CTL_DONE + 1;
+
+__device__ inline int ld_relaxed(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void st_release(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Every wave's stores have reached the L2 (vmcnt(0)); after the barrier one release store by thread 0
+// (which writes the L2 back) publishes them all.
+__device__ inline void wg_release_store(int* p, int v) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0) st_release(p, v);
+}
+
+// Thread 0: waits until every counter p[0..cnt-1] is at least `target`; false on an abort (*abort set,
+// by another workgroup or here on the time limit).
+__device__ inline bool counters_reach(const int* p, int cnt, int target, int* abort, unsigned long long limit) {
+  const unsigned long long t0 = wall_clock64();
+  for (int h = 0; h < cnt; ++h) {
+    while (ld_relaxed(p + h) < target) {
+      if (ld_relaxed(abort) || wall_clock64() - t0 > limit) {
+        st_release(abort, 1);
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  return true;
+}
+
+// The whole workgroup waits for counters_reach and then sees what was published; false on an abort.
+__device__ inline bool wg_wait(const int* p, int cnt, int target, int* abort, unsigned long long limit, int* s_flag) {
+  if (threadIdx.x == 0) {
+    *s_flag = counters_reach(p, cnt, target, abort, limit) ? 1 : 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*s_flag) != 0;
+}
+
+// A matrix's workgroups when several share it: G per matrix (G = 1: one workgroup), counters per matrix
+// (ctl[0] published steps, ctl[2] abort, ctl[CTL_DONE + h - 1] steps finished by helper h).  The
+// workgroups of one matrix share blockIdx.x mod 8 (one XCD under round-robin dispatch).
+struct Team {
+  int G = 1;
+  int batch = 0;
+  int* ctl = nullptr;
+  unsigned long long limit = 0;
+};
+
+__device__ inline void team_role(const Team& t, int& b, int& role) {
+  b = blockIdx.x;
+  role = 0;
+  if (t.G > 1) {
+    const int k = blockIdx.x / 8;
+    b = (int)(blockIdx.x % 8) + 8 * (k / t.G);
+    role = k % t.G;
+  }
+}
+
 __device__ inline double dlapy3(double x, double y, double z) {
   double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
   if (w == 0.0) return fabs(x) + fabs(y) + fabs(z);
@@ -118,6 +182,42 @@ __device__ inline void gemv_rows(const zc* __restrict__ A, int lda, int rows, in
       zc acc = part[r];
       for (int q = 1; q < G; ++q) acc += part[q * rows + r];
       w[r] = acc * scale;
+    }
+    __syncthreads();
+  }
+}
+
+// out[r] = sum_c A(r, c) v[c] for r0 <= r < r1, c < cols, in a summation order fixed per row whatever
+// r0 and r1 are: GV_GROUPS lanes take interleaved columns of a row (four accumulators each), and their
+// partial sums are added in group order.  So the rows of one product may be split over workgroups and
+// give the same bits as one workgroup.  part: LDS, blockDim.x entries.  Ends with a barrier.
+constexpr int GV_GROUPS = 8;
+__device__ inline void gemv_rows_det(const zc* __restrict__ A, int lda, int r0, int r1, int cols, const zc* v,
+                                     zc* out, zc* part) {
+  const int rpass = (int)blockDim.x / GV_GROUPS;
+  const int g = threadIdx.x / rpass, rr = threadIdx.x % rpass;
+  for (int base = r0; base < r1; base += rpass) {
+    const int r = base + rr;
+    zc a0 = mk(0.0), a1 = mk(0.0), a2 = mk(0.0), a3 = mk(0.0);
+    if (r < r1) {
+      const zc* p = A + r;
+      int c = g;
+      for (; c + 3 * GV_GROUPS < cols; c += 4 * GV_GROUPS) {
+        const zc x0 = p[(long)c * lda], x1 = p[(long)(c + GV_GROUPS) * lda];
+        const zc x2 = p[(long)(c + 2 * GV_GROUPS) * lda], x3 = p[(long)(c + 3 * GV_GROUPS) * lda];
+        fma_acc(a0, x0, v[c]);
+        fma_acc(a1, x1, v[c + GV_GROUPS]);
+        fma_acc(a2, x2, v[c + 2 * GV_GROUPS]);
+        fma_acc(a3, x3, v[c + 3 * GV_GROUPS]);
+      }
+      for (; c < cols; c += GV_GROUPS) fma_acc(a0, p[(long)c * lda], v[c]);
+    }
+    part[g * rpass + rr] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if ((int)threadIdx.x < rpass && base + (int)threadIdx.x < r1) {
+      zc acc = part[threadIdx.x];
+      for (int q = 1; q < GV_GROUPS; ++q) acc += part[q * rpass + threadIdx.x];
+      out[base + threadIdx.x] = acc;
     }
     __syncthreads();
   }
@@ -282,16 +382,52 @@ __global__ void k_clear_below_sub(int n, zc* A, int lda, long sA) {
 constexpr int HB = 32;     // panel width
 constexpr int HB_NX = 64;  // the last columns are reduced unblocked
 
+// Rows [r0, r1) of the trailing-matrix product of workgroup `role` in a team of G.
+__device__ inline void lahr2_rows(int m, int G, int role, int& r0, int& r1) {
+  r0 = (int)((long)m * role / G);
+  r1 = (int)((long)m * (role + 1) / G);
+}
+
+// Helper `role` of a panel team: for every column of the panel, once the leader has published the
+// column's Householder vector (V(:, j), counter i0 + j + 1), its rows of y = A(i0+1:n, c+1:n) v go to
+// Y(:, j), where the leader picks them up.
+__device__ void lahr2_help(int n, int i0, int nb, const zc* A, int lda, const zc* V, int ldv, zc* Y, int ldy,
+                           const Team& t, int b, int role, zc* smem) {
+  __shared__ int s_ok;
+  zc* vv = smem + n;
+  zc* part = smem + 2 * n;
+  int* ctl = t.ctl + (long)b * CTL_INTS;
+  const int m = n - i0 - 1;
+  int r0, r1;
+  lahr2_rows(m, t.G, role, r0, r1);
+  for (int j = 0; j < nb; ++j) {
+    const int c = i0 + j;
+    if (!wg_wait(ctl, 1, c + 1, ctl + 2, t.limit, &s_ok)) return;
+    for (int r = j + threadIdx.x; r < m; r += blockDim.x) vv[r - j] = V[(long)j * ldv + r];
+    __syncthreads();
+    gemv_rows_det(A + (long)(c + 1) * lda + i0 + 1, lda, r0, r1, m - j, vv, Y + (long)j * ldy, part);
+    wg_release_store(ctl + CTL_DONE + role - 1, c + 1);
+  }
+}
+
 __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc* A, int lda, long sA, zc* V,
                                                       int ldv, long sV, zc* Y, int ldy, long sY, zc* T, long sT,
-                                                      zc* tau) {
+                                                      zc* tau, Team team) {
   extern __shared__ zc smem[];
   zc* bcol = smem;           // m: the current column, rows i0+1..n-1; then the new column of Y
   zc* vv = smem + n;         // m: the new Householder vector over columns c+1..n-1
   zc* part = smem + 2 * n;   // blockDim.x
   __shared__ zc red[HH_THREADS / 64];
   __shared__ zc w[HB], t2[HB];
-  const int bi = blockIdx.x;
+  __shared__ int s_ok;
+  int bi, role;
+  team_role(team, bi, role);
+  if (bi >= (team.G > 1 ? team.batch : (int)gridDim.x)) return;
+  if (role > 0) {
+    lahr2_help(n, i0, nb, A + sA * bi, lda, V + sV * bi, ldv, Y + sY * bi, ldy, team, bi, role, smem);
+    return;
+  }
+  int* const ctl = team.G > 1 ? team.ctl + (long)bi * CTL_INTS : nullptr;
   A += sA * bi;
   V += sV * bi;
   Y += sY * bi;
@@ -370,8 +506,24 @@ __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc*
     }
     if (tid == 0) tau[c] = tj;
     __syncthreads();
-    // y = A(i0+1:n, c+1:n) v  (the trailing matrix, not yet touched by this panel's right updates)
-    gemv_rows(A + (long)(c + 1) * lda + i0 + 1, lda, m, m - j, vv, mk(1.0), bcol, part);
+    // y = A(i0+1:n, c+1:n) v  (the trailing matrix, not yet touched by this panel's right updates); in a
+    // team the helpers take their rows once V(:, j) is published, and leave them in Y(:, j)
+    int r0 = 0, r1 = m;
+    if (ctl) {
+      wg_release_store(ctl, c + 1);
+      lahr2_rows(m, team.G, 0, r0, r1);
+    }
+    gemv_rows_det(A + (long)(c + 1) * lda + i0 + 1, lda, r0, r1, m - j, vv, bcol, part);
+    if (ctl) {
+      if (!wg_wait(ctl + CTL_DONE, team.G - 1, c + 1, ctl + 2, team.limit, &s_ok)) {
+        // a helper did not answer: mark the panel failed (tau = NaN poisons the matrix, and the
+        // eigensolver's non-finite check reports it)
+        if (tid == 0) tau[c] = mk(__builtin_nan(""), 0.0);
+        return;
+      }
+      for (int r = r1 + tid; r < m; r += nt) bcol[r] = Yp(r, j);
+      __syncthreads();
+    }
     // t = V(:, 0:j)^H v
     for (int q = wid; q < j; q += nw) {
       zc acc = mk(0.0);
@@ -1154,7 +1306,6 @@ __device__ inline void ss_apply_line(zc* p, long stride, const Rot* R, int wlen)
 // reports failure.
 // ------------------------------------------------------------------------------------------------
 constexpr int NSLOT = 8;
-constexpr int CTL_INTS = 16, CTL_DONE = 8, TEAM_MAX = CTL_INTS - CTL_DONE + 1;
 constexpr int DESC_INTS = 8;  // kbase, rlo, rhi, whole window inside the block, last critical column
 
 struct HqrTeam {
@@ -1166,24 +1317,9 @@ struct HqrTeam {
   unsigned long long limit = 0;  // wall-clock ticks any wait may take
 };
 
-// Polls read the counters relaxed (a coherent load, no cache invalidation); the acquire fence follows
-// once the awaited value has been seen.
-__device__ inline int ld_relaxed(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ inline void st_release(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
-
 // Thread 0 waits until every helper has finished `target` windows; false on an abort or a timeout.
 __device__ inline bool helpers_reached(const HqrTeam& t, int* ctl, int target) {
-  const unsigned long long t0 = wall_clock64();
-  for (int h = 0; h < t.G - 1; ++h) {
-    while (ld_relaxed(ctl + CTL_DONE + h) < target) {
-      if (ld_relaxed(ctl + 2) || wall_clock64() - t0 > t.limit) {
-        st_release(ctl + 2, 1);
-        return false;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  return true;
+  return counters_reach(ctl + CTL_DONE, t.G - 1, target, ctl + 2, t.limit);
 }
 
 // Leader: the whole workgroup waits until every helper has finished `target` windows, then sees their
@@ -1196,14 +1332,6 @@ __device__ inline bool team_wait(const HqrTeam& t, int* ctl, int target, int* s_
   }
   __syncthreads();
   return __builtin_amdgcn_readfirstlane(*s_flag) != 0;
-}
-
-// Every wave's stores have reached the L2 (vmcnt(0)); after the barrier one release store by thread 0
-// (which writes the L2 back) publishes them all.
-__device__ inline void wg_release_store(int* p, int v) {
-  __builtin_amdgcn_s_waitcnt(0x0F70);
-  __syncthreads();
-  if (threadIdx.x == 0) st_release(p, v);
 }
 
 // Leader: window `w`'s rotations and descriptor are in its slot; make them (and every earlier write)
@@ -1891,6 +2019,30 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
   zc* Wk = work.W.p;
   zc* Wk2 = work.W.p + (long)n * HB;
   auto S = [](const zc* p, long st) { return strided<zc>(p, st); };
+  int dev = 0, cus = 0;
+  CWX_HIP(hipGetDevice(&dev));
+  CWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // panel teams: when the matrices leave most of the device idle, helper workgroups take rows of each
+  // panel column's trailing-matrix product ("lahr2_team" helpers; by default as many as fit)
+  Team pteam;
+  if (teams && batch < cus && n - 1 - HB_NX > 0) {
+    const long want = tune_lookup("lahr2_team", n, batch, -1);
+    int G = want >= 0 ? (int)want + 1 : 8;
+    G = std::min({G, TEAM_MAX, cus / (8 * ((batch + 7) / 8))});
+    int per_cu = 0;
+    CWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_lahr2, HH_THREADS, sh2));
+    if (G > 1 && (long)per_cu * cus >= 8L * ((batch + 7) / 8) * G) {
+      work.ensure_team(batch);
+      int rate_khz = 0;
+      CWX_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+      pteam.G = G;
+      pteam.batch = batch;
+      pteam.ctl = work.team_ctl.p;
+      pteam.limit = (unsigned long long)std::max(rate_khz, 1) * 1000ull * 20ull;  // 20 s
+      hipLaunchKernelGGL(k_fill_i32, dim3(cdiv((size_t)CTL_INTS * batch, 256)), dim3(256), 0, s,
+                         (long)CTL_INTS * batch, pteam.ctl);
+    }
+  }
   int i0 = 0, npan = 0;
   for (; i0 < n - 1 - HB_NX; i0 += HB, ++npan) {
     const int nb = std::min(HB, n - 1 - i0);
@@ -1898,8 +2050,23 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     const int nc = n - i0 - nb;
     zc* Vp = Vd + (long)npan * HB * n;
     zc* Tp = work.T.p + (long)npan * HB * HB;
-    hipLaunchKernelGGL(k_lahr2, dim3(batch), dim3(HH_THREADS), sh2, s, n, i0, nb, A, lda, sA, Vp, n, nn,
-                       work.Y.p + i0 + 1, n, sY, Tp, sT, work.tau.p);
+    bool launched = false;
+    if (pteam.G > 1) {
+      int n_ = n, i0_ = i0, nb_ = nb, lda_ = lda, ldv = n, ldy = n;
+      long sA_ = sA, sV = nn, sY_ = sY, sT_ = sT;
+      zc *Ap = A, *Vq = Vp, *Yq = work.Y.p + i0 + 1, *Tq = Tp, *tq = work.tau.p;
+      void* args[] = {&n_, &i0_, &nb_, &Ap, &lda_, &sA_, &Vq, &ldv, &sV, &Yq, &ldy, &sY_, &Tq, &sT_, &tq, &pteam};
+      const unsigned grid = (unsigned)(8 * ((batch + 7) / 8) * pteam.G);
+      if (hipLaunchCooperativeKernel((const void*)k_lahr2, dim3(grid), dim3(HH_THREADS), args, sh2, s) == hipSuccess)
+        launched = true;
+      else {
+        (void)hipGetLastError();
+        pteam = Team();  // refused (for instance while another process holds the device): one workgroup
+      }
+    }
+    if (!launched)
+      hipLaunchKernelGGL(k_lahr2, dim3(batch), dim3(HH_THREADS), sh2, s, n, i0, nb, A, lda, sA, Vp, n, nn,
+                         work.Y.p + i0 + 1, n, sY, Tp, sT, work.tau.p, Team());
     // Y(0:i0+1, :) = A(0:i0+1, i0+1:n) V T
     zgemm_set(OpN, OpN, i0 + 1, nb, m, mk(1.0), S(A + (long)(i0 + 1) * lda, sA), lda, S(Vp, nn), n, mk(0.0),
               S(Wk, sWk), n, batch, s);
@@ -1932,9 +2099,6 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     zgemm_set(OpN, OpN, m, m, nb, mk(-1.0), S(Vp, nn), n, S(Wk2, sWk), HB, mk(1.0), S(Zsub, nn), n, batch, s);
   }
   hipLaunchKernelGGL(k_clear_below_sub, dim3(cdiv(n * n, 256 * 4), batch), dim3(256), 0, s, n, A, lda, sA);
-  int dev = 0, cus = 0;
-  CWX_HIP(hipGetDevice(&dev));
-  CWX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HqrTeam team;
   team.G = (teams && batch < cus) ? qr_team_size(n, batch, cus) : 1;
   if (team.G > 1) {

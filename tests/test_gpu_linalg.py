@@ -209,9 +209,10 @@ def test_zgeev_large_batch_matches_small_batch(ctx):
 
 @pytest.mark.parametrize("n,batch", [(450, 4), (196, 1), (900, 2)])
 def test_zgeev_teams_match_one_workgroup(ctx, n, batch):
-    """The QR iteration with helper workgroups per matrix (the default when the matrices do not fill the
-    device) gives exactly the eigenvalues and eigenvectors of one workgroup per matrix: every row receives
-    the same rotations in the same order."""
+    """The Hessenberg panels and the QR iteration with helper workgroups per matrix (the default when the
+    matrices do not fill the device) give exactly the eigenvalues and eigenvectors of one workgroup per
+    matrix: every row of the panels' products is summed in the same order, and every row of the QR
+    receives the same rotations in the same order."""
     import ctypes as C
     from scattersim_amd import _capi
     rng = np.random.default_rng(n + batch)
@@ -220,6 +221,7 @@ def test_zgeev_teams_match_one_workgroup(ctx, n, batch):
     try:
         for team in (0, -1):
             _capi.check(_capi.lib().cwx_set_param(b"qr_team", team))
+            _capi.check(_capi.lib().cwx_set_param(b"lahr2_team", team))
             dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(A.transpose(0, 2, 1)))
             dW = sa.DeviceArray(ctx, (batch, n), np.complex128)
             dV = sa.DeviceArray(ctx, (batch, n, n), np.complex128)
@@ -230,6 +232,7 @@ def test_zgeev_teams_match_one_workgroup(ctx, n, batch):
             out.append((dW.numpy(), dV.numpy()))
     finally:
         _capi.lib().cwx_set_param(b"qr_team", -1)
+        _capi.lib().cwx_set_param(b"lahr2_team", -1)
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     for b in range(batch):
         check_eig(A[b], out[1][0][b], out[1][1][b].T)
