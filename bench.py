@@ -293,6 +293,56 @@ def layer_leg(ctx, stream, dev, no_cpu: bool):
     return out
 
 
+def chain_gemm_leg(ctx, stream, dev):
+    """The transfer chain's complex products (MKL_multiply at ScatterVolume.cpp:330-334, 768-781) at the
+    chain shapes of the sweep (n = 4 MF = 484) and of volume32 (n = 900), FP64 against the FP32 matrix
+    cores (cwx_zgemm_split32_batched_dev, split operands or operands rounded to FP32); HIP events on the
+    library's stream; error = max |C - C_fp64| / max |C_fp64| on random unit-variance operands."""
+    import torch
+    import scattersim_amd as sa
+    lib = sa.lib()
+    one, zero = sa._capi.cwx_complex(1, 0), sa._capi.cwx_complex(0, 0)
+    out = {}
+    for n, batch in ((484, 128), (900, 24)):
+        g = torch.Generator(device=dev).manual_seed(n)
+        A = torch.randn((batch, n, n), dtype=torch.complex128, device=dev, generator=g)
+        B = torch.randn((batch, n, n), dtype=torch.complex128, device=dev, generator=g)
+        C = torch.empty_like(A)
+        fl = 8.0 * n ** 3 * batch
+        row = {"batch": batch, "flops_per_launch": fl}
+        ref = None
+        for name, terms in (("fp64", 0), ("fp32_split3", 3), ("fp32_rounded", 1)):
+            def call():
+                if terms == 0:
+                    sa.check(lib.cwx_zgemm_batched_dev(ctx.h, 0, 0, n, n, n, one, A.data_ptr(), n, n * n,
+                                                       B.data_ptr(), n, n * n, zero, C.data_ptr(), n, n * n, batch))
+                else:
+                    sa.check(lib.cwx_zgemm_split32_batched_dev(ctx.h, 0, 0, n, n, n, one, A.data_ptr(), n, n * n,
+                                                               B.data_ptr(), n, n * n, zero, C.data_ptr(), n, n * n,
+                                                               batch, terms))
+            call()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                call()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1) * 1e-3)
+            t = min(ts)
+            if ref is None:
+                ref = C.clone()
+                err = 0.0
+            else:
+                err = float((C - ref).abs().max() / ref.abs().max())
+            row[name] = {"ms": t * 1e3, "tflops": fl / t / 1e12, "rel_err_vs_fp64": err}
+        out[f"n{n}"] = row
+        del A, B, C, ref
+    torch.cuda.synchronize()
+    return out
+
+
 def fields_leg(ctx, d_sample, shape, opts, eig_mode: int, plain_ms: float):
     """All 2MF unit transverse incident fields of the workload's cell in one cwx_solve_fields_dev call (the
     order basis of its reflection and transmission matrices), against one plain solve per field."""
@@ -533,6 +583,7 @@ def main():
         nxt = {"layer": layer_leg(ctx, stream, dev, args.no_cpu),
                "fields": fields_leg(ctx, d_sample, (L, s1, s2), opts, args.eig_mode, ms),
                "compact_cw": compact_leg(sol),
+               "chain_gemm": chain_gemm_leg(ctx, stream, dev),
                "learned": learned_leg(ctx, args.no_cpu)}
     sol.free()
 

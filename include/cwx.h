@@ -121,6 +121,17 @@ int cwx_zgemm_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n
                           cwx_complex alpha, const void* dA, int lda, int64_t strideA, const void* dB,
                           int ldb, int64_t strideB, cwx_complex beta, void* dC, int ldc, int64_t strideC,
                           int batch);
+/* The product of cwx_zgemm_batched_dev (MKL_multiply, third_Lapack.h:75-85, as called by the transfer
+ * chain at ScatterVolume.cpp:330-334 and 768-781) on the FP32 matrix cores: every FP64 operand is split
+ * into hi = fl32(x) and lo = fl32(x - hi), a real product taken as hi hi + hi lo + lo hi, 16-term partial
+ * sums accumulated in FP32 and added in FP64.  Relative error about 1e-7 of |A| |B|, not FP64: for
+ * callers that accept that accuracy.  terms = 3 takes that split form, terms = 1 only hi hi (the operands
+ * rounded to FP32, a third of the matrix-core work).  Otherwise the arguments and errors of
+ * cwx_zgemm_batched_dev. */
+int cwx_zgemm_split32_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n, int k,
+                                  cwx_complex alpha, const void* dA, int lda, int64_t strideA, const void* dB,
+                                  int ldb, int64_t strideB, cwx_complex beta, void* dC, int ldc,
+                                  int64_t strideC, int batch, int terms);
 
 /* ---- the scattervolume solve (ScatterVolume.cpp:506-1019) ------------------------------------ */
 /* Options of ScatterVolume.cpp:510-530; cwx_volume_params_default fills the reference defaults. */

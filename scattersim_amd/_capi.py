@@ -76,6 +76,8 @@ SIGNATURES = {
     "cwx_zgesv_batched_dev_noalloc": (I, [P, I, I, I, P, I64, P, I64, P]),
     "cwx_zgemm_batched_dev": (I, [P, I, I, I, I, I, cwx_complex, P, I, I64, P, I, I64, cwx_complex, P, I,
                                   I64, I]),
+    "cwx_zgemm_split32_batched_dev": (I, [P, I, I, I, I, I, cwx_complex, P, I, I64, P, I, I64, cwx_complex, P,
+                                          I, I64, I, I]),
     "cwx_volume_params_default": (None, [C.POINTER(cwx_volume_params)]),
     "cwx_solve_volume": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),
     "cwx_solve_volume_dev": (I, [P, C.POINTER(cwx_volume_params), P, I, I, I, C.POINTER(P)]),

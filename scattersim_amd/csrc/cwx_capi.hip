@@ -434,6 +434,23 @@ int cwx_zgemm_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n
   });
 }
 
+int cwx_zgemm_split32_batched_dev(cwx_context* ctx, int transa, int transb, int m, int n, int k, cwx_complex alpha,
+                                  const void* dA, int lda, int64_t strideA, const void* dB, int ldb, int64_t strideB,
+                                  cwx_complex beta, void* dC, int ldc, int64_t strideC, int batch, int terms) {
+  return guard([&]() -> int {
+    CWX_CHECK(ctx && dA && dB && dC, "NULL argument");
+    CWX_CHECK(m >= 1 && n >= 1 && k >= 1 && batch >= 1, "bad dimensions");
+    CWX_CHECK(transa >= 0 && transa <= 2 && transb >= 0 && transb <= 2, "trans must be 0 (N), 1 (T) or 2 (C)");
+    CWX_CHECK(terms == 1 || terms == 3, "terms must be 1 or 3");
+    set_device(ctx);
+    cwx::zgemm_split32_set((cwx::Op)transa, (cwx::Op)transb, m, n, k, zc{alpha.re, alpha.im},
+                           cwx::strided((const zc*)dA, strideA), lda, cwx::strided((const zc*)dB, strideB), ldb,
+                           zc{beta.re, beta.im}, cwx::strided((zc*)dC, strideC), ldc, batch, terms == 3,
+                           ctx->c.stream);
+    return CWX_OK;
+  });
+}
+
 // ---------------------------------------------------------------------------------------------
 void cwx_volume_params_default(cwx_volume_params* p) {
   if (!p) return;

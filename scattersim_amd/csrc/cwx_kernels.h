@@ -80,6 +80,11 @@ inline void zgemm_batched(Op opa, Op opb, int m, int n, int k, zc alpha, const z
             s);
 }
 
+// The same product on the FP32 matrix cores: split operands (hi * hi + hi * lo + lo * hi) or, with split
+// false, the operands rounded to FP32; FP32 partial sums of 16 terms added in FP64.  About 1e-7 relative.
+void zgemm_split32_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda, MatSet B, int ldb, zc beta,
+                       MatSet C, int ldc, int batch, bool split, hipStream_t s);
+
 // LU with partial pivoting, in place (LAPACK zgetrf contract, 0-based pivots).  info[b] = 0 or
 // the 1-based index of the first exactly-zero pivot.
 void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hipStream_t s);

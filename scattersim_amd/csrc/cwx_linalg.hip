@@ -245,6 +245,136 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_mfma(Op opa, Op opb, int m, in
 }
 
 // ------------------------------------------------------------------------------------------------
+// Mixed-precision GEMM on the FP32 matrix cores (v_mfma_f32_16x16x4f32): every FP64 operand is split
+// into hi = fl32(x) and lo = fl32(x - hi), and each real product X Y is taken as Xh Yh + Xh Yl + Xl Yh
+// (Xl Yl, about 2^-48 of it, is dropped); with SPLIT false only Xh Yh is taken, a plain FP32 product of
+// the rounded operands.  The matrix core accumulates 16 values of k in FP32; each
+// such partial sum is then added to FP64 accumulators.  A complex product is four real products, so a
+// complex tile costs 12 FP32 matrix instructions where the FP64 kernel issues 4 FP64 ones, against a
+// 2:1 FP32:FP64 matrix rate on MI355X; the FP32 rounding inside each 16-term partial sum bounds the
+// accuracy near 2^-24 of the partial sums.  Same tiling as k_zgemm_mfma: a 64x64 output tile per
+// 256-lane workgroup, each wavefront a 32x32 quarter as 2x2 tiles of 16x16.
+// Lane layout of v_mfma_f32_16x16x4f32: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// D[4 (l / 16) + q][l % 16] in register q of lane l.
+// ------------------------------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256, 2) void k_zgemm_split32(Op opa, Op opb, int m, int n, int k, zc alpha,
+                                                       MatSet Aset, int lda, MatSet Bset, int ldb, zc beta,
+                                                       MatSet Cset, int ldc) {
+  // planes: 0 re-hi, 1 re-lo, 2 im-hi, 3 im-lo
+  __shared__ float As[4][MK][GM + 1], Bs[4][MK][GN + 1];
+  const int b = blockIdx.z;
+  const zc* __restrict__ A = Aset.at(b);
+  const zc* __restrict__ B = Bset.at(b);
+  zc* __restrict__ C = Cset.at(b);
+  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+  double PD[2][2][4], QD[2][2][4], RD[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) PD[i][j][q] = QD[i][j][q] = RD[i][j][q] = 0.0;
+  auto split = [](double x, float& hi, float& lo) {
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+  };
+  for (int k0 = 0; k0 < k; k0 += MK) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      int r, kk;
+      if (opa == OpN) { r = e & (GM - 1); kk = e >> 6; }
+      else { kk = e & (MK - 1); r = e >> 4; }
+      const int gr = m0 + r, gk = k0 + kk;
+      const zc v = (gr < m && gk < k) ? load_op(A, lda, opa, gr, gk) : mk(0.0);
+      split(v.re, As[0][kk][r], As[1][kk][r]);
+      split(v.im, As[2][kk][r], As[3][kk][r]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      int kk, c;
+      if (opb == OpN) { kk = e & (MK - 1); c = e >> 4; }
+      else { c = e & (GN - 1); kk = e >> 6; }
+      const int gk = k0 + kk, gc = n0 + c;
+      const zc v = (gk < k && gc < n) ? load_op(B, ldb, opb, gk, gc) : mk(0.0);
+      split(v.re, Bs[0][kk][c], Bs[1][kk][c]);
+      split(v.im, Bs[2][kk][c], Bs[3][kk][c]);
+    }
+    __syncthreads();
+    f4v P[2][2], Q[2][2], R[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) P[i][j] = Q[i][j] = R[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k4 = 0; k4 < MK; k4 += 4) {
+      const int kk = k4 + (lane >> 4);
+      float a[4][2], bb[4][2];  // [plane][tile]
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          a[pl][i] = As[pl][kk][wm + 16 * i + (lane & 15)];
+          bb[pl][i] = Bs[pl][kk][wn + 16 * i + (lane & 15)];
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          // Re Re, Im Im, Re Im + Im Re: the hi hi terms, then (split form) hi lo and lo hi
+          P[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], bb[0][j], P[i][j], 0, 0, 0);
+          Q[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2][i], bb[2][j], Q[i][j], 0, 0, 0);
+          R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], bb[2][j], R[i][j], 0, 0, 0);
+          R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2][i], bb[0][j], R[i][j], 0, 0, 0);
+          if (SPLIT) {
+            P[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], bb[1][j], P[i][j], 0, 0, 0);
+            P[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][i], bb[0][j], P[i][j], 0, 0, 0);
+            Q[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2][i], bb[3][j], Q[i][j], 0, 0, 0);
+            Q[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3][i], bb[2][j], Q[i][j], 0, 0, 0);
+            R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i], bb[3][j], R[i][j], 0, 0, 0);
+            R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][i], bb[2][j], R[i][j], 0, 0, 0);
+            R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2][i], bb[1][j], R[i][j], 0, 0, 0);
+            R[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3][i], bb[0][j], R[i][j], 0, 0, 0);
+          }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          PD[i][j][q] += (double)P[i][j][q];
+          QD[i][j][q] += (double)Q[i][j][q];
+          RD[i][j][q] += (double)R[i][j][q];
+        }
+    __syncthreads();
+  }
+  const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gr = m0 + wm + 16 * i + 4 * (lane >> 4) + q;
+        const int gc = n0 + wn + 16 * j + (lane & 15);
+        if (gr < m && gc < n) {
+          zc* cp = C + (long)gc * ldc + gr;
+          zc v = alpha * mk(PD[i][j][q] - QD[i][j][q], RD[i][j][q]);
+          if (!beta0) v = v + beta * (*cp);
+          *cp = v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
 // LU panel: one workgroup per matrix factors columns [j0, j0+nb) over rows [j0, n) with partial
 // pivoting (pivot = first maximum of |re|+|im|, as izamax).  Swaps are applied to the panel columns
 // only; the other columns are swapped by k_lu_swap_trsm.
@@ -866,6 +996,21 @@ void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda,
   }
   dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
   hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  CWX_HIP(hipGetLastError());
+}
+
+void zgemm_split32_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda, MatSet B, int ldb, zc beta,
+                       MatSet C, int ldc, int batch, bool split, hipStream_t s) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "zgemm: batch too large");
+  CWX_CHECK(k > 0, "zgemm: k must be positive");
+  dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
+  if (split)
+    hipLaunchKernelGGL(k_zgemm_split32<true>, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                       ldc);
+  else
+    hipLaunchKernelGGL(k_zgemm_split32<false>, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta,
+                       C, ldc);
   CWX_HIP(hipGetLastError());
 }
 

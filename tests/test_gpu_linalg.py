@@ -21,6 +21,50 @@ def test_zgemm(ctx, m, n, k):
     assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * np.sqrt(k)
 
 
+def op(X, t):
+    return X if t == 0 else (X.T if t == 1 else X.conj().T)
+
+
+@pytest.mark.parametrize("m,n,k,ta,tb,batch", [(1, 1, 1, 0, 0, 1), (100, 37, 129, 0, 0, 2), (65, 130, 17, 1, 2, 1),
+                                               (484, 484, 484, 0, 0, 3), (200, 96, 300, 2, 1, 2)])
+def test_zgemm_fp32_matrix_cores(ctx, m, n, k, ta, tb, batch):
+    """cwx_zgemm_split32_batched_dev against the FP64 product: elementwise within 16 * 2^-24 of |A| |B|
+    (FP32 accumulation of at most 12 matrix-core steps per 16 values of k, FP64 across them), for the split
+    form (3 terms) and the rounded form (1 term).  Both errors are printed: the split form is no more accurate
+    once k passes a few dozen (1.6e-8 against 1.5e-8 at k = 484), since the FP32 rounding of the hi hi
+    partial sums, not the operands' rounding, sets the error."""
+    rng = np.random.default_rng(m + 3 * n + 7 * k + ta + tb)
+    shA = (k, m) if ta else (m, k)
+    shB = (n, k) if tb else (k, n)
+    A, B, C0 = crand(rng, batch, *shA), crand(rng, batch, *shB), crand(rng, batch, m, n)
+    alpha, beta = 0.75 + 0.5j, -0.5 + 0.25j
+    lda, ldb = shA[0], shB[0]
+    dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(A.transpose(0, 2, 1)))
+    dB = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(B.transpose(0, 2, 1)))
+    errs = {}
+    for terms in (3, 1):
+        dC = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(C0.transpose(0, 2, 1)))
+        sa.check(sa.lib().cwx_zgemm_split32_batched_dev(
+            ctx.h, ta, tb, m, n, k, sa._capi.cwx_complex(alpha.real, alpha.imag), dA.ptr, lda, A[0].size, dB.ptr, ldb,
+            B[0].size, sa._capi.cwx_complex(beta.real, beta.imag), dC.ptr, m, m * n, batch, terms))
+        C = dC.numpy().transpose(0, 2, 1)
+        dC.free()
+        worst = 0.0
+        for i in range(batch):
+            oa, ob = op(A[i], ta), op(B[i], tb)
+            ref = alpha * (oa @ ob) + beta * C0[i]
+            # componentwise scale: |alpha| (|Re A| + |Im A|)(|Re B| + |Im B|)
+            absA = np.abs(oa.real) + np.abs(oa.imag)
+            absB = np.abs(ob.real) + np.abs(ob.imag)
+            scale = abs(alpha) * (absA @ absB)
+            worst = max(worst, float(np.max(np.abs(C[i] - ref) / scale)))
+        errs[terms] = worst
+    print(f"{m}x{n}x{k} op({ta},{tb}) batch {batch}: split {errs[3]:.2e}, rounded {errs[1]:.2e} of |A||B|", flush=True)
+    assert errs[3] < 16 * 2.0 ** -24 and errs[1] < 16 * 2.0 ** -24
+    dA.free()
+    dB.free()
+
+
 @pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 3), (31, 1), (32, 4), (33, 2), (100, 1), (300, 7), (1100, 3), (1100, 1),
                                     (2000, 1)])
 def test_zgesv(ctx, n, nrhs):
