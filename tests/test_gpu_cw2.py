@@ -119,10 +119,10 @@ CASES = {
 }
 
 
-def solve(ctx, sample, p):
+def solve(ctx, sample, p, eig_mode=0):
     return sa.scattervolume(ctx, sample, lam=p.lam, direction=p.direction, ex=p.ex, ey=p.ey, ez=p.ez, n=p.n,
                             kappa=p.kappa, size=p.size, z=p.z, coef=p.coef, psf=p.psf, external=p.external,
-                            npz=p.npz, no_cutoff=p.no_cutoff)
+                            npz=p.npz, no_cutoff=p.no_cutoff, eig_mode=eig_mode)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
@@ -167,6 +167,41 @@ def test_compact_file_layout_and_mode_fields(ctx, case):
         assert rel_err(Em, Er[sel]) < 1e-9, (l, rel_err(Em, Er[sel]))
     # the trailer follows the layers; the reference reader never looks at it
     assert f["tail"][:8] == b"CWXTRL01"
+
+
+def test_compact_mode_columns_normalised_like_zgeev_in_both_modes(ctx):
+    """gg's columns are the solution's mode vectors.  zgeev (third_Lapack.h:38-45) returns each eigenvector
+    of Phi with unit 2-norm and its largest component real and positive; the reduced mode forms
+    v+- = [a; +-Q a / gamma] from P Q's eigenvectors and must rescale each to the same convention.  For a
+    simple eigenvalue that fixes the vector, so the reduced mode's columns equal the full mode's."""
+    make, p = CASES["three_layers_5x5"]
+    sample = make()
+    gg = {}
+    for mode in (0, 1):
+        sol = solve(ctx, sample, p, mode)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "c.cw")
+            sol.write_cw2(path)
+            f = read_compact(path)
+        gg[mode] = [(lay["gg"], lay["gamma"]) for lay in f["layers"]]
+        sol.free()
+    worst_norm = worst_imag = worst_diff = 0.0
+    compared = 0
+    for (g0, gam0), (g1, gam1) in zip(gg[0], gg[1]):
+        assert np.allclose(gam0, gam1, rtol=1e-10, atol=1e-12)
+        for g in (g0, g1):
+            worst_norm = max(worst_norm, float(np.max(np.abs(np.linalg.norm(g, axis=0) - 1.0))))
+            big = g[np.argmax(np.abs(g), axis=0), np.arange(g.shape[1])]
+            assert np.all(big.real > 0)
+            worst_imag = max(worst_imag, float(np.max(np.abs(big.imag))))
+        gap = np.abs(gam0[:, None] - gam0[None, :]) + np.eye(len(gam0)) * 1e9
+        simple = np.min(gap, axis=1) > 1e-6 * np.max(np.abs(gam0))
+        assert simple.sum() > len(gam0) // 2
+        worst_diff = max(worst_diff, float(np.max(np.abs(g1[:, simple] - g0[:, simple]))))
+        compared += int(simple.sum())
+    print(f"|norm - 1| {worst_norm:.1e}, imag of largest {worst_imag:.1e}, reduced vs full on {compared} simple "
+          f"modes {worst_diff:.1e}", flush=True)
+    assert worst_norm < 1e-12 and worst_imag < 1e-12 and worst_diff < 1e-8
 
 
 def test_compact_file_ft_inverts_to_sample(ctx):
