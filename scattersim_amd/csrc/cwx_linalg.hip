@@ -6,6 +6,7 @@
 // dimensions, as LAPACK; every entry point is batched over matrices placed at a fixed element stride.
 #include "cwx_common.h"
 #include "cwx_kernels.h"
+#include "cwx_tune.h"
 
 namespace cwx {
 
@@ -146,98 +147,142 @@ __global__ __launch_bounds__(256) void k_zgemv(int m, int n, int k, zc alpha, Ma
 }
 
 // ------------------------------------------------------------------------------------------------
-// GEMM on the FP64 matrix cores: 64x64 output tile per 256-lane workgroup, each wavefront a 32x32
-// quarter as 2x2 tiles of v_mfma_f64_16x16x4f64.  A complex product is four real products
-// (Re Re, Im Im, Re Im, Im Re) into three accumulators; K is staged through LDS in steps of 16 as
-// separate real and imaginary planes.  The operands enter the instruction swapped (B tile as its
-// first operand), so each lane's results are consecutive rows of the column-major C.
-// Lane layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
-// D[(l / 16) + 4q][l % 16] in register q of lane l.
+// GEMM on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  On MI355X one such instruction keeps its SIMD's
+// matrix unit busy for 64 cycles, so the kernel is built to issue them back to back: each wavefront owns
+// TM x TN tiles of 16x16 outputs (the workgroup 2 x 2 wavefronts, a (32 TM) x (32 TN) output tile), the
+// next 16-deep k panel is loaded from global memory into registers while the current one is multiplied out
+// of LDS, and the operands sit in LDS as (re, im) pairs, read as 16-byte words.  A complex product is four
+// real products into two accumulators, Re += Br Ar - Bi Ai and Im += Bi Ar + Br Ai.  The operands enter the
+// instruction swapped (the B fragment first), so each lane's results are consecutive rows of the
+// column-major C.  Lane layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, D[(l / 16) + 4q][l % 16] in register q of lane l.
 // ------------------------------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int MK = 16;
 
+// Operands are read through global-address-space pointers: a flat load would also count against the LDS
+// wait counter, so every LDS read's wait would wait for the prefetched panel too.
+typedef __attribute__((address_space(1))) const zc gzc;
+typedef double dv2 __attribute__((ext_vector_type(2)));
+__device__ inline dv2 gload2(gzc* p) { return *(const __attribute__((address_space(1))) dv2*)(p); }
+
+template <int TM, int TN>
 __global__ __launch_bounds__(256, 2) void k_zgemm_mfma(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet Aset,
                                                     int lda, MatSet Bset, int ldb, zc beta, MatSet Cset, int ldc) {
-  __shared__ double Ar[MK][GM + 1], Ai[MK][GM + 1], Br[MK][GN + 1], Bi[MK][GN + 1];
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  constexpr int NA = BM * MK / 256, NB = MK * BN / 256;
+  __shared__ double2 As[MK][BM + 1], Bs[MK][BN + 1];
   const int b = blockIdx.z;
-  const zc* __restrict__ A = Aset.at(b);
-  const zc* __restrict__ B = Bset.at(b);
+  gzc* A = (gzc*)Aset.at(b);
+  gzc* B = (gzc*)Bset.at(b);
   zc* __restrict__ C = Cset.at(b);
-  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
-  d4 P[2][2], Q[2][2], R[2][2];
+  const int wm = (wv & 1) * 16 * TM, wn = (wv >> 1) * 16 * TN;
+  // element e of a k panel: (row, k) of op(A), (k, column) of op(B); the fastest index follows memory
+  auto a_at = [&](int e, int& r, int& kk) {
+    if (opa == OpN) { r = e % BM; kk = e / BM; }
+    else { kk = e % MK; r = e / MK; }
+  };
+  auto b_at = [&](int e, int& kk, int& c) {
+    if (opb == OpN) { kk = e % MK; c = e / MK; }
+    else { c = e % BN; kk = e / BN; }
+  };
+  // The next panel's loads are issued unconditionally (out-of-range elements read element (0, 0), which
+  // exists) and masked when they are written to LDS: a load under a branch, or a select right after it,
+  // would make the wavefront wait for it before the multiplications.
+  dv2 pa[NA], pb[NB];
+  unsigned va = 0, vb = 0;
+  auto fetch = [&](int k0) {
+    va = vb = 0;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      P[i][j] = d4{0, 0, 0, 0};
-      Q[i][j] = d4{0, 0, 0, 0};
-      R[i][j] = d4{0, 0, 0, 0};
+    for (int u = 0; u < NA; ++u) {
+      int r, kk;
+      a_at(t + 256 * u, r, kk);
+      const int gr = m0 + r, gk = k0 + kk;
+      const bool ok = gr < m && gk < k;
+      va |= ok ? 1u << u : 0u;
+      const long off = ok ? (opa == OpN ? (long)gk * lda + gr : (long)gr * lda + gk) : 0;
+      pa[u] = gload2(A + off);
     }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      int kk, c;
+      b_at(t + 256 * u, kk, c);
+      const int gk = k0 + kk, gc = n0 + c;
+      const bool ok = gk < k && gc < n;
+      vb |= ok ? 1u << u : 0u;
+      const long off = ok ? (opb == OpN ? (long)gc * ldb + gk : (long)gk * ldb + gc) : 0;
+      pb[u] = gload2(B + off);
+    }
+  };
+  const double sa = opa == OpC ? -1.0 : 1.0, sb = opb == OpC ? -1.0 : 1.0;
+  d4 Re[TM][TN], Im[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) Re[i][j] = Im[i][j] = d4{0, 0, 0, 0};
+  fetch(0);
   for (int k0 = 0; k0 < k; k0 += MK) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = t + 256 * u;
+    for (int u = 0; u < NA; ++u) {
       int r, kk;
-      if (opa == OpN) { r = e & (GM - 1); kk = e >> 6; }
-      else { kk = e & (MK - 1); r = e >> 4; }
-      const int gr = m0 + r, gk = k0 + kk;
-      const zc v = (gr < m && gk < k) ? load_op(A, lda, opa, gr, gk) : mk(0.0);
-      Ar[kk][r] = v.re;
-      Ai[kk][r] = v.im;
+      a_at(t + 256 * u, r, kk);
+      const bool ok = (va >> u) & 1u;
+      As[kk][r] = double2{ok ? pa[u].x : 0.0, ok ? sa * pa[u].y : 0.0};
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = t + 256 * u;
+    for (int u = 0; u < NB; ++u) {
       int kk, c;
-      if (opb == OpN) { kk = e & (MK - 1); c = e >> 4; }
-      else { c = e & (GN - 1); kk = e >> 6; }
-      const int gk = k0 + kk, gc = n0 + c;
-      const zc v = (gk < k && gc < n) ? load_op(B, ldb, opb, gk, gc) : mk(0.0);
-      Br[kk][c] = v.re;
-      Bi[kk][c] = v.im;
+      b_at(t + 256 * u, kk, c);
+      const bool ok = (vb >> u) & 1u;
+      Bs[kk][c] = double2{ok ? pb[u].x : 0.0, ok ? sb * pb[u].y : 0.0};
     }
     __syncthreads();
+    if (k0 + MK < k) fetch(k0 + MK);  // in flight while this panel is multiplied
+    auto step = [&](const double2 (&a)[TM], const double2 (&bb)[TN]) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          Re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb[j].x, a[i].x, Re[i][j], 0, 0, 0);
+          Im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb[j].y, a[i].x, Im[i][j], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          Re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bb[j].y, a[i].y, Re[i][j], 0, 0, 0);
+          Im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bb[j].x, a[i].y, Im[i][j], 0, 0, 0);
+        }
+    };
+    auto frags = [&](int k4, double2 (&a)[TM], double2 (&bb)[TN]) {
+      const int kk = k4 + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = As[kk][wm + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bb[j] = Bs[kk][wn + 16 * j + (lane & 15)];
+    };
 #pragma unroll
     for (int k4 = 0; k4 < MK; k4 += 4) {
-      const int kk = k4 + (lane >> 4);
-      double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = Ar[kk][wm + 16 * i + (lane & 15)];
-        ai[i] = Ai[kk][wm + 16 * i + (lane & 15)];
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        br[j] = Br[kk][wn + 16 * j + (lane & 15)];
-        bi[j] = Bi[kk][wn + 16 * j + (lane & 15)];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          P[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ar[i], P[i][j], 0, 0, 0);
-          Q[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ai[i], Q[i][j], 0, 0, 0);
-          R[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[j], ar[i], R[i][j], 0, 0, 0);
-          R[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[j], ai[i], R[i][j], 0, 0, 0);
-        }
+      double2 a[TM], bb[TN];
+      frags(k4, a, bb);
+      step(a, bb);
     }
     __syncthreads();
   }
   const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gr = m0 + wm + 16 * i + (lane & 15);
         const int gc = n0 + wn + 16 * j + (lane >> 4) + 4 * q;
         if (gr < m && gc < n) {
           zc* cp = C + (long)gc * ldc + gr;
-          zc v = alpha * mk(P[i][j][q] - Q[i][j][q], R[i][j][q]);
+          zc v = alpha * mk(Re[i][j][q], Im[i][j][q]);
           if (!beta0) v = v + beta * (*cp);
           *cp = v;
         }
@@ -252,7 +297,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_mfma(Op opa, Op opb, int m, in
 // such partial sum is then added to FP64 accumulators.  A complex product is four real products, so a
 // complex tile costs 12 FP32 matrix instructions where the FP64 kernel issues 4 FP64 ones, against a
 // 2:1 FP32:FP64 matrix rate on MI355X; the FP32 rounding inside each 16-term partial sum bounds the
-// accuracy near 2^-24 of the partial sums.  Same tiling as k_zgemm_mfma: a 64x64 output tile per
+// accuracy near 2^-24 of the partial sums.  The tiling of k_zgemm_mfma<2, 2>: a 64x64 output tile per
 // 256-lane workgroup, each wavefront a 32x32 quarter as 2x2 tiles of 16x16.
 // Lane layout of v_mfma_f32_16x16x4f32: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
 // D[4 (l / 16) + q][l % 16] in register q of lane l.
@@ -994,8 +1039,21 @@ void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda,
     CWX_HIP(hipGetLastError());
     return;
   }
-  dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
-  hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  // the matrix-core kernel with 64x64 tiles; the others stay selectable (cwx_set_param "zgemm_kernel")
+  const int kind = (int)tune_lookup("zgemm_kernel", m, n, 1);
+  if (kind == 1) {
+    hipLaunchKernelGGL((k_zgemm_mfma<2, 2>), dim3(cdiv(m, 64), cdiv(n, 64), batch), dim3(256), 0, s, opa, opb, m, n,
+                       k, alpha, A, lda, B, ldb, beta, C, ldc);
+  } else if (kind == 2) {
+    hipLaunchKernelGGL((k_zgemm_mfma<4, 2>), dim3(cdiv(m, 128), cdiv(n, 64), batch), dim3(256), 0, s, opa, opb, m,
+                       n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  } else if (kind == 3) {
+    hipLaunchKernelGGL((k_zgemm_mfma<2, 4>), dim3(cdiv(m, 64), cdiv(n, 128), batch), dim3(256), 0, s, opa, opb, m,
+                       n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  } else {
+    dim3 grid(cdiv(m, GM), cdiv(n, GN), batch);
+    hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, s, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  }
   CWX_HIP(hipGetLastError());
 }
 
