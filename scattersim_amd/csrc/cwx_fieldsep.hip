@@ -244,40 +244,55 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
   }
 }
 
-// T[iz][c][b][ix] = sum_a exp(i kx_a x_ix) C[iz][c][a M1 + b] as complex64, b padded to K4 with zeros;
-// Yf[b][iy] = exp(i ky_b y_iy) as complex64 (same padding).
-// C[e] = sum of the S slices of the coefficient pass, added in slice order (into slice 0).
-__global__ void k_sep_reduce(long n, int S, long cstride, zc* __restrict__ C) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    zc v = C[e];
-    for (int sl = 1; sl < S; ++sl) v = v + C[sl * cstride + e];
-    C[e] = v;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, const zc* __restrict__ C,
-                                                  const zc* __restrict__ X, int K4, float2* __restrict__ T) {
+// T[iz][c][b][ix] = sum_a exp(i kx_a x_ix) C[iz][c][a M1 + b] as complex64, b padded to K4 with zeros:
+// one thread per column ix of a plane and group of 4 entries (c, b..b+3), so that its row of X is read
+// once for the four.  The plane's coefficients are the S slices of the coefficient pass, added in slice
+// order.
+__global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, int S, long cstride,
+                                                  const zc* __restrict__ C, const zc* __restrict__ X, int K4,
+                                                  float2* __restrict__ T) {
   extern __shared__ zc sCp[];  // 3 MF of this plane
   const int iz = blockIdx.y, MF = a.MF, M0 = a.M0, M1 = a.M1;
-  for (int e = threadIdx.x; e < 3 * MF; e += blockDim.x) sCp[e] = C[(long)iz * 3 * MF + e];
+  for (int e = threadIdx.x; e < 3 * MF; e += blockDim.x) {
+    const long o = (long)iz * 3 * MF + e;
+    zc v = C[o];
+    for (int sl = 1; sl < S; ++sl) v = v + C[sl * cstride + o];
+    sCp[e] = v;
+  }
   __syncthreads();
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ix >= g.nx) return;
+  const int G = K4 / 4, c = blockIdx.z / G, b0 = 4 * (blockIdx.z % G);
   const long per = (long)3 * K4 * g.nx;
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= per) return;
-  const int ix = (int)(e % g.nx), b = (int)((e / g.nx) % K4), c = (int)(e / ((long)g.nx * K4));
-  zc acc = mk(0.0);
-  if (b < M1)
-    for (int ai = 0; ai < M0; ++ai) fma_acc(acc, X[(long)ix * M0 + ai], sCp[c * MF + ai * M1 + b]);
-  T[(long)iz * per + e] = make_float2((float)acc.re, (float)acc.im);
+  float2* Tp = T + (long)iz * per + ix;
+  zc acc[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)};
+  for (int ai = 0; ai < M0; ++ai) {
+    const zc xa = X[(long)ix * M0 + ai];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b0 + u < M1) fma_acc(acc[u], xa, sCp[c * MF + ai * M1 + b0 + u]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) Tp[(long)(c * K4 + b0 + u) * g.nx] = make_float2((float)acc[u].re, (float)acc[u].im);
 }
 
-__global__ void k_sep_ytab(SepArgs a, GridSpec g, int K4, float2* __restrict__ Yf) {
+// X[ix][a] = exp(i kx_a x_ix) (FP64), then Yf[b][iy] = exp(i ky_b y_iy) as complex64, b padded to K4 with
+// zeros: one launch for both tables of the complex64 path.
+__global__ void k_sep_tables32(SepArgs a, GridSpec g, int K4, zc* __restrict__ X, float2* __restrict__ Yf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= K4 * g.ny) return;
-  const int iy = t % g.ny, b = t / g.ny;
+  const int nX = g.nx * a.M0;
+  if (t < nX) {
+    const int ix = t / a.M0, ai = t % a.M0;
+    const double xv = (double)(coord(g.x0, g.dx, ix));
+    X[t] = cexpi(a.Sx[ai * a.M1] * a.k, xv);
+    return;
+  }
+  const int u = t - nX;
+  if (u >= K4 * g.ny) return;
+  const int iy = u % g.ny, b = u / g.ny;
   zc v = mk(0.0);
   if (b < a.M1) v = cexpi(a.Sy[b] * a.k, (double)coord(g.y0, g.dy, iy));
-  Yf[t] = make_float2((float)v.re, (float)v.im);
+  Yf[u] = make_float2((float)v.re, (float)v.im);
 }
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -286,6 +301,7 @@ constexpr int SY_TILES = 4;  // y tiles of 16 rows per wavefront
 // E(x, y) = sum_b Y(y, b) T(b, x) per component with v_mfma_f32_16x16x4f32 (A[i][k] from lane i + 16k,
 // B[k][j] from lane j + 16k, D[4 (l / 16) + q][l % 16] in register q of lane l).  A wavefront owns 16
 // columns and SY_TILES x 16 rows of one plane; the T fragments of its columns stay in registers.
+template <int NK>
 __global__ __launch_bounds__(256) void k_sep_synth_mfma(GridSpec g, int K4, const float2* __restrict__ T,
                                                         const float2* __restrict__ Yf, float2* __restrict__ out) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -296,9 +312,11 @@ __global__ __launch_bounds__(256) void k_sep_synth_mfma(GridSpec g, int K4, cons
   const int lx = lane & 15, lk = lane >> 4;
   const long per = (long)K4 * g.nx;
   const float2* Tz = T + (long)iz * 3 * per;
-  constexpr int KMAX = 16;  // up to 64 orders along y
+  // NK = K4 / 4 blocks of 4 orders along y, up to 16 (a template parameter, so that only the fragments
+  // in use take registers and more wavefronts keep stores in flight)
+  constexpr int KMAX = NK;
   float2 tf[3][KMAX];
-  const int nk = K4 / 4;
+  const int nk = NK;
 #pragma unroll
   for (int kb = 0; kb < KMAX; ++kb)
     if (kb < nk)
@@ -401,15 +419,21 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
     const int msl = (n4 + S - 1) / S;
     hipLaunchKernelGGL(k_sep_coeffs_g, dim3(cdiv(3 * MF, 64), (unsigned)groups.size(), S), dim3(64 * SEP_W),
                        sizeof(zc) * gmax * msl, s, a, g, (const PlaneGroup*)w.groups.p, w.C.p, cstride);
-    hipLaunchKernelGGL(k_sep_tables, dim3(cdiv(g.nx * st.M0, 256)), dim3(256), 0, s, a, g, w.X.p, w.Y.p);
-    const long per = (long)3 * K4 * g.nx;
-    if (S > 1)
-      hipLaunchKernelGGL(k_sep_reduce, dim3(cdiv(cstride, 256)), dim3(256), 0, s, cstride, S, cstride, w.C.p);
-    hipLaunchKernelGGL(k_sep_rows, dim3((unsigned)cdiv((int)per, 256), g.nz), dim3(256), sizeof(zc) * 3 * MF, s, a,
-                       g, w.C.p, w.X.p, K4, w.Tf.p);
-    hipLaunchKernelGGL(k_sep_ytab, dim3(cdiv(K4 * g.ny, 256)), dim3(256), 0, s, a, g, K4, w.Yf.p);
+    hipLaunchKernelGGL(k_sep_tables32, dim3(cdiv(g.nx * st.M0 + K4 * g.ny, 256)), dim3(256), 0, s, a, g, K4, w.X.p,
+                       w.Yf.p);
+    CWX_CHECK(3 * (K4 / 4) <= 65535, "bad order count");
+    hipLaunchKernelGGL(k_sep_rows, dim3(cdiv(g.nx, 256), g.nz, 3 * (K4 / 4)), dim3(256), sizeof(zc) * 3 * MF, s, a, g,
+                       S, cstride, w.C.p, w.X.p, K4, w.Tf.p);
     dim3 grid(cdiv(g.nx, 16), cdiv(g.ny, 4 * 16 * SY_TILES), g.nz);
-    hipLaunchKernelGGL(k_sep_synth_mfma, grid, dim3(256), 0, s, g, K4, w.Tf.p, w.Yf.p, (float2*)out);
+    switch (K4 / 4) {
+#define CWX_SYNTH(NK) \
+  case NK: hipLaunchKernelGGL(k_sep_synth_mfma<NK>, grid, dim3(256), 0, s, g, K4, w.Tf.p, w.Yf.p, (float2*)out); break;
+      CWX_SYNTH(1) CWX_SYNTH(2) CWX_SYNTH(3) CWX_SYNTH(4) CWX_SYNTH(5) CWX_SYNTH(6) CWX_SYNTH(7) CWX_SYNTH(8)
+      CWX_SYNTH(9) CWX_SYNTH(10) CWX_SYNTH(11) CWX_SYNTH(12) CWX_SYNTH(13) CWX_SYNTH(14) CWX_SYNTH(15)
+      CWX_SYNTH(16)
+#undef CWX_SYNTH
+      default: CWX_CHECK(false, "more than 64 orders along y in the complex64 evaluator");
+    }
     CWX_HIP(hipGetLastError());
     return;
   }
