@@ -117,10 +117,28 @@ def field_leg(ctx, sol, g, stream, dev):
     del ref
     bytes_out = npts * 3 * 8
     gbs = bytes_out / tf / 1e9
+    # the coefficient pass reads each layer's 3MF x 4MF mode matrix once per group of up to 8 consecutive
+    # planes inside that layer (the library's grouping: region by z >= z_l - 0.001, float32 coordinates)
+    zl = np.asarray(sol.z, np.float64)
+    dz32, z032 = float(np.float32(g["dz"])), float(np.float32(g["z0"]))
+    zs = np.array([np.float32(dz32 * i + z032) for i in range(g["nz"])], np.float64)
+    region = np.array([int(np.sum(z >= zl.astype(np.float32).astype(np.float64) - 0.001)) for z in zs])
+    groups_in_layers = 0
+    i = 0
+    while i < len(region):
+        j = i + 1
+        while j < len(region) and j - i < 8 and region[j] == region[i]:
+            j += 1
+        if 1 <= region[i] <= len(zl) - 1:
+            groups_in_layers += 1
+        i = j
+    bytes_q = groups_in_layers * (3 * sol.MF) * (4 * sol.MF) * 16
     return {"form": "separable exp(i kx x).C(z).exp(i ky y), FP64 inside, complex64 out",
             "gpts_per_s": npts / tf / 1e9, "points": npts, "ms": tf * 1e3,
             "rel_err_vs_fp64": err, "err_reference": "complex128 separable form on every point, max |dE| / max |E|",
-            "hbm_write_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+            "hbm_write_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+            "mode_matrix_read_bytes": bytes_q,
+            "hbm_rw_frac": (bytes_out + bytes_q) / tf / 1e9 / PEAK_HBM_GBS}
 
 
 def field_c3_leg(ctx, stream, dev):

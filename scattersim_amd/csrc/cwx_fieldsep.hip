@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace cwx {
 
@@ -167,6 +168,25 @@ __global__ __launch_bounds__(256) void k_sep_synth(SepArgs a, GridSpec g, const 
 // Complex64 output path: coefficients of all planes of one layer in one pass over [Q_check | Q_hat],
 // then T = X C per plane (FP64, rounded to complex64), then E = T Y^T on the FP32 matrix cores.
 // ------------------------------------------------------------------------------------------------
+// Entry t of the complex64 path's tables: X[ix][a] = exp(i kx_a x_ix) (FP64) for t < nx M0, then
+// Yf[b][iy] = exp(i ky_b y_iy) as complex64, b padded to K4 with zeros.
+__device__ inline void sep_table32_entry(const SepArgs& a, const GridSpec& g, int K4, zc* __restrict__ X,
+                                         float2* __restrict__ Yf, int t) {
+  const int nX = g.nx * a.M0;
+  if (t < nX) {
+    const int ix = t / a.M0, ai = t % a.M0;
+    const double xv = (double)(coord(g.x0, g.dx, ix));
+    X[t] = cexpi(a.Sx[ai * a.M1] * a.k, xv);
+    return;
+  }
+  const int u = t - nX;
+  if (u >= K4 * g.ny) return;
+  const int iy = u % g.ny, b = u / g.ny;
+  zc v = mk(0.0);
+  if (b < a.M1) v = cexpi(a.Sy[b] * a.k, (double)coord(g.y0, g.dy, iy));
+  Yf[u] = make_float2((float)v.re, (float)v.im);
+}
+
 struct PlaneGroup {
   int iz0, count, region;  // planes iz0 .. iz0+count-1, all in region `region` (0: above, L+1: below)
 };
@@ -175,12 +195,19 @@ constexpr int SEP_G = 8;  // planes per group
 // 64 coefficients per workgroup over one slice of the 4MF modes (blockIdx.z of gridDim.z slices, so that a few
 // layers still give the chip enough workgroups to stream Q at full rate); the SEP_W wavefronts split the slice
 // and their partial sums are added in LDS in a fixed order.  Slice s writes C + s cstride; the rows pass adds
-// the slices in order.
+// the slices in order.  Workgroups with blockIdx.x >= xb0 (of plane group 0, slice 0) fill the x and y
+// phase tables instead, which saves a dependent launch.
 constexpr int SEP_W = 8;
 __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec g, const PlaneGroup* __restrict__ groups,
-                                                      zc* __restrict__ C, long cstride) {
+                                                      zc* __restrict__ C, long cstride, int xb0, int K4,
+                                                      zc* __restrict__ X, float2* __restrict__ Yf) {
   extern __shared__ zc ph[];  // count x slice mode phases
   __shared__ zc part[SEP_W][SEP_G][64];
+  if ((int)blockIdx.x >= xb0) {
+    if (blockIdx.y == 0 && blockIdx.z == 0)
+      sep_table32_entry(a, g, K4, X, Yf, ((int)blockIdx.x - xb0) * blockDim.x + threadIdx.x);
+    return;
+  }
   const PlaneGroup gr = groups[blockIdx.y];
   const int MF = a.MF, n3 = 3 * MF, n4 = 4 * MF;
   const int l = gr.region;
@@ -251,18 +278,22 @@ __global__ __launch_bounds__(64 * SEP_W) void k_sep_coeffs_g(SepArgs a, GridSpec
 __global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, int S, long cstride,
                                                   const zc* __restrict__ C, const zc* __restrict__ X, int K4,
                                                   float2* __restrict__ T) {
-  extern __shared__ zc sCp[];  // 3 MF of this plane
+  extern __shared__ zc sCp[];  // the M0 x 4 coefficients (c, a, b0..b0+3) this workgroup uses
   const int iz = blockIdx.y, MF = a.MF, M0 = a.M0, M1 = a.M1;
-  for (int e = threadIdx.x; e < 3 * MF; e += blockDim.x) {
-    const long o = (long)iz * 3 * MF + e;
-    zc v = C[o];
-    for (int sl = 1; sl < S; ++sl) v = v + C[sl * cstride + o];
+  const int G = K4 / 4, c = blockIdx.z / G, b0 = 4 * (blockIdx.z % G);
+  for (int e = threadIdx.x; e < 4 * M0; e += blockDim.x) {
+    const int ai = e >> 2, b = b0 + (e & 3);
+    zc v = mk(0.0);
+    if (b < M1) {
+      const long o = (long)iz * 3 * MF + c * MF + ai * M1 + b;
+      v = C[o];
+      for (int sl = 1; sl < S; ++sl) v = v + C[sl * cstride + o];
+    }
     sCp[e] = v;
   }
   __syncthreads();
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   if (ix >= g.nx) return;
-  const int G = K4 / 4, c = blockIdx.z / G, b0 = 4 * (blockIdx.z % G);
   const long per = (long)3 * K4 * g.nx;
   float2* Tp = T + (long)iz * per + ix;
   zc acc[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)};
@@ -270,33 +301,14 @@ __global__ __launch_bounds__(256) void k_sep_rows(SepArgs a, GridSpec g, int S, 
     const zc xa = X[(long)ix * M0 + ai];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (b0 + u < M1) fma_acc(acc[u], xa, sCp[c * MF + ai * M1 + b0 + u]);
+      if (b0 + u < M1) fma_acc(acc[u], xa, sCp[4 * ai + u]);
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) Tp[(long)(c * K4 + b0 + u) * g.nx] = make_float2((float)acc[u].re, (float)acc[u].im);
 }
 
-// X[ix][a] = exp(i kx_a x_ix) (FP64), then Yf[b][iy] = exp(i ky_b y_iy) as complex64, b padded to K4 with
-// zeros: one launch for both tables of the complex64 path.
-__global__ void k_sep_tables32(SepArgs a, GridSpec g, int K4, zc* __restrict__ X, float2* __restrict__ Yf) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nX = g.nx * a.M0;
-  if (t < nX) {
-    const int ix = t / a.M0, ai = t % a.M0;
-    const double xv = (double)(coord(g.x0, g.dx, ix));
-    X[t] = cexpi(a.Sx[ai * a.M1] * a.k, xv);
-    return;
-  }
-  const int u = t - nX;
-  if (u >= K4 * g.ny) return;
-  const int iy = u % g.ny, b = u / g.ny;
-  zc v = mk(0.0);
-  if (b < a.M1) v = cexpi(a.Sy[b] * a.k, (double)coord(g.y0, g.dy, iy));
-  Yf[u] = make_float2((float)v.re, (float)v.im);
-}
-
 typedef float f4 __attribute__((ext_vector_type(4)));
-constexpr int SY_TILES = 4;  // y tiles of 16 rows per wavefront
+constexpr int SY_TILES = 8;  // y tiles of 16 rows per wavefront
 
 // E(x, y) = sum_b Y(y, b) T(b, x) per component with v_mfma_f32_16x16x4f32 (A[i][k] from lane i + 16k,
 // B[k][j] from lane j + 16k, D[4 (l / 16) + q][l % 16] in register q of lane l).  A wavefront owns 16
@@ -369,22 +381,16 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
     zd[l] = st.z[l];
     zf[l] = (float)((sol.opt.external && l > 0 && l < L) ? 0.0 : st.z[l]);
   }
-  w.zf.ensure(L + 1);
-  w.zd.ensure(L + 1);
-  CWX_HIP(hipMemcpyAsync(w.zf.p, zf.data(), sizeof(float) * (L + 1), hipMemcpyHostToDevice, s));
-  CWX_HIP(hipMemcpyAsync(w.zd.p, zd.data(), sizeof(double) * (L + 1), hipMemcpyHostToDevice, s));
-  w.C.ensure((size_t)g.nz * 3 * MF);
-  w.X.ensure((size_t)g.nx * st.M0);
-  w.Y.ensure((size_t)g.ny * st.M1);
-  SepArgs a{st.M0, st.M1, MF, L, sol.opt.external ? 1 : 0, st.k, sol.opt.n[0], sol.opt.n[1],
-            sol.tabs.p, sol.tabs.p + MF, sol.tabs.p + 2 * MF, sol.tabs.p + 3 * MF, sol.tabs.p + 4 * MF,
-            sol.x.p, sol.gam.p, sol.Qch.p, w.zf.p, w.zd.p, tol_mode};
+  // the boundary tables (and the plane groups below) go to the device in one copy: each small copy
+  // from host memory costs the stream several microseconds
+  const size_t zd_off = ((sizeof(float) * (L + 1) + 15) / 16) * 16;
+  const size_t gr_off = zd_off + ((sizeof(double) * (L + 1) + 15) / 16) * 16;
+  std::vector<PlaneGroup> groups;
+  const int n4 = 4 * MF;
+  const int gmax = std::max(1, std::min(SEP_G, (int)((32 * 1024) / (sizeof(zc) * n4))));
   if (!out_f64) {
     // plane groups: consecutive planes of one region, at most SEP_G and at most what the phase
     // table in LDS holds; regions computed with the device's coordinate and test
-    const int n4 = 4 * MF;
-    const int gmax = std::max(1, std::min(SEP_G, (int)((32 * 1024) / (sizeof(zc) * n4))));
-    std::vector<PlaneGroup> groups;
     auto region = [&](int iz) {
       const float z = std::fmaf(g.dz, (float)iz, g.z0);
       int l = 0;
@@ -405,25 +411,44 @@ void field_separable(const Solution& sol, const GridSpec& g, int tol_mode, int o
       iz += cnt;
     }
     CWX_CHECK(groups.size() <= 65535, "too many plane groups");
-    w.groups.ensure(sizeof(PlaneGroup) * groups.size());
-    CWX_HIP(hipMemcpyAsync(w.groups.p, groups.data(), sizeof(PlaneGroup) * groups.size(), hipMemcpyHostToDevice, s));
+  }
+  std::vector<unsigned char> pack(gr_off + sizeof(PlaneGroup) * groups.size());
+  std::memcpy(pack.data(), zf.data(), sizeof(float) * (L + 1));
+  std::memcpy(pack.data() + zd_off, zd.data(), sizeof(double) * (L + 1));
+  if (!groups.empty()) std::memcpy(pack.data() + gr_off, groups.data(), sizeof(PlaneGroup) * groups.size());
+  // repeated evaluations of one solution on one grid reuse the tables already on the device
+  if (pack != w.pack_host || !w.pack.p) {
+    w.pack.ensure(pack.size());
+    CWX_HIP(hipMemcpyAsync(w.pack.p, pack.data(), pack.size(), hipMemcpyHostToDevice, s));
+    w.pack_host = pack;
+  }
+  const float* d_zf = reinterpret_cast<const float*>(w.pack.p);
+  const double* d_zd = reinterpret_cast<const double*>(w.pack.p + zd_off);
+  const PlaneGroup* d_groups = reinterpret_cast<const PlaneGroup*>(w.pack.p + gr_off);
+  w.C.ensure((size_t)g.nz * 3 * MF);
+  w.X.ensure((size_t)g.nx * st.M0);
+  w.Y.ensure((size_t)g.ny * st.M1);
+  SepArgs a{st.M0, st.M1, MF, L, sol.opt.external ? 1 : 0, st.k, sol.opt.n[0], sol.opt.n[1],
+            sol.tabs.p, sol.tabs.p + MF, sol.tabs.p + 2 * MF, sol.tabs.p + 3 * MF, sol.tabs.p + 4 * MF,
+            sol.x.p, sol.gam.p, sol.Qch.p, d_zf, d_zd, tol_mode};
+  if (!out_f64) {
     const int K4 = (st.M1 + 3) / 4 * 4;
     CWX_CHECK(K4 <= 64, "more than 64 orders along y in the complex64 evaluator");
     w.Tf.ensure((size_t)g.nz * 3 * K4 * g.nx);
     w.Yf.ensure((size_t)K4 * g.ny);
     // mode slices: enough workgroups to keep HBM busy when there are few plane groups
     const long base = (long)cdiv(3 * MF, 64) * (long)groups.size();
-    const int S = (int)std::max<long>(1, std::min<long>({16L, (4096 + base - 1) / base, (long)n4 / 64}));
+    const int S = (int)std::max<long>(1, std::min<long>({16L, (4096 + base - 1) / base, (long)n4 / 32}));
     const long cstride = (long)g.nz * 3 * MF;
     w.C.ensure((size_t)S * cstride);
     const int msl = (n4 + S - 1) / S;
-    hipLaunchKernelGGL(k_sep_coeffs_g, dim3(cdiv(3 * MF, 64), (unsigned)groups.size(), S), dim3(64 * SEP_W),
-                       sizeof(zc) * gmax * msl, s, a, g, (const PlaneGroup*)w.groups.p, w.C.p, cstride);
-    hipLaunchKernelGGL(k_sep_tables32, dim3(cdiv(g.nx * st.M0 + K4 * g.ny, 256)), dim3(256), 0, s, a, g, K4, w.X.p,
-                       w.Yf.p);
+    const int xb0 = cdiv(3 * MF, 64), xbt = cdiv(g.nx * st.M0 + K4 * g.ny, 64 * SEP_W);
+    hipLaunchKernelGGL(k_sep_coeffs_g, dim3(xb0 + xbt, (unsigned)groups.size(), S), dim3(64 * SEP_W),
+                       sizeof(zc) * gmax * msl, s, a, g, d_groups, w.C.p, cstride, xb0, K4,
+                       w.X.p, w.Yf.p);
     CWX_CHECK(3 * (K4 / 4) <= 65535, "bad order count");
-    hipLaunchKernelGGL(k_sep_rows, dim3(cdiv(g.nx, 256), g.nz, 3 * (K4 / 4)), dim3(256), sizeof(zc) * 3 * MF, s, a, g,
-                       S, cstride, w.C.p, w.X.p, K4, w.Tf.p);
+    hipLaunchKernelGGL(k_sep_rows, dim3(cdiv(g.nx, 256), g.nz, 3 * (K4 / 4)), dim3(256), sizeof(zc) * 4 * st.M0, s,
+                       a, g, S, cstride, w.C.p, w.X.p, K4, w.Tf.p);
     dim3 grid(cdiv(g.nx, 16), cdiv(g.ny, 4 * 16 * SY_TILES), g.nz);
     switch (K4 / 4) {
 #define CWX_SYNTH(NK) \

@@ -108,9 +108,8 @@ void cw2_read_file(Context& ctx, const std::string& path, Solution& sol);
 struct SepWork {
   DevBuf<zc> C, X, Y;
   DevBuf<float2> Tf, Yf;          // complex64 path: T = X C per plane, y phase table
-  DevBuf<unsigned char> groups;   // plane groups of the coefficient pass
-  DevBuf<float> zf;
-  DevBuf<double> zd;
+  DevBuf<unsigned char> pack;     // boundary z (float, double) and the plane groups of the coefficient pass
+  std::vector<unsigned char> pack_host;  // what pack holds
 };
 
 // Field of a solution on a regular grid by order-separable synthesis (see cwx_fieldsep.hip).
