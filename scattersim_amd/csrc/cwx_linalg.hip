@@ -864,25 +864,42 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
 // written back permuted, both coalesced, instead of n dependent swaps of scattered 16-byte elements
 // per column.  Workgroups (matrix, y) take the chunks y, y + gridDim.y, ... of their matrix, each forming
 // the map itself; dynamic LDS: ch x n complex, then n ints.
-constexpr int SWP_THREADS = 256;
+constexpr int SWP_THREADS = 512;
+// B <- P B for the interchanges (t, ipiv[t]), t = 0..n-1 in order (zlaswp).  The row of P B at position i
+// comes from the row found by undoing the interchanges from the last back to the first.  With zgetrf's
+// pivots (ipiv[t] >= t) only t <= i can move it, so each position is traced by one thread over t = i .. 0,
+// all positions at once, instead of one thread applying n dependent interchanges; other pivot lists are
+// traced over all t.  Then each chunk of columns is staged in LDS and written back permuted.  Pivots
+// outside [0, n) are taken as no interchange (they would address rows outside B).
 __global__ __launch_bounds__(SWP_THREADS) void k_laswp(int n, int nrhs, int ch, MatSet Bset, int ldb, IntSet pset) {
   extern __shared__ __align__(16) unsigned char swp_lds[];
   zc* buf = reinterpret_cast<zc*>(swp_lds);
   int* src = reinterpret_cast<int*>(swp_lds + sizeof(zc) * (size_t)ch * n);
+  int* pv = src + n;
   const int b = blockIdx.x;
   zc* B = Bset.at(b);
   const int* ipiv = pset.at(b);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) src[i] = i;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int j = 0; j < n; ++j) {
-      const int p = ipiv[j];
-      if (p != j) {
-        const int t = src[j];
-        src[j] = src[p];
-        src[p] = t;
-      }
+  int below = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int p = ipiv[i];
+    pv[i] = (p >= 0 && p < n) ? p : i;
+    below |= (p >= 0 && p < i) ? 1 : 0;
+  }
+  const bool general = __syncthreads_or(below);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int k = i, t = general ? n - 1 : i;
+    for (; t >= 7; t -= 8) {
+      int p[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) p[u] = pv[t - u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) k = (k == t - u) ? p[u] : ((k == p[u]) ? t - u : k);
     }
+    for (; t >= 0; --t) {
+      const int q = pv[t];
+      k = (k == t) ? q : ((k == q) ? t : k);
+    }
+    src[i] = k;
   }
   __syncthreads();
   for (int c0 = blockIdx.y * ch; c0 < nrhs; c0 += gridDim.y * ch) {
@@ -1105,12 +1122,12 @@ void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int 
   {
     // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map (the device's
     // whole LDS allowance when one column does not fit in 64 KiB)
-    const size_t need = (sizeof(int) + sizeof(zc)) * (size_t)n;
+    const size_t need = (2 * sizeof(int) + sizeof(zc)) * (size_t)n;
     const size_t budget = need <= 64 * 1024 ? 64 * 1024 : lds_limit();
     CWX_CHECK(need <= budget, "zgetrs: n too large for the LDS row map");
-    const size_t avail = budget - sizeof(int) * (size_t)n;
+    const size_t avail = budget - 2 * sizeof(int) * (size_t)n;
     const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
-    const size_t lds = sizeof(zc) * (size_t)ch * n + sizeof(int) * (size_t)n;
+    const size_t lds = sizeof(zc) * (size_t)ch * n + 2 * sizeof(int) * (size_t)n;
     // enough workgroups to spread the column chunks over the device (about 2048 in all)
     const int chunks = (nrhs + ch - 1) / ch;
     const int gy = std::max(1, std::min({chunks, 65535, 2048 / batch}));

@@ -21,6 +21,32 @@ def test_zgemm(ctx, m, n, k):
     assert np.max(np.abs(C - ref)) <= 1e-13 * np.max(np.abs(ref)) * np.sqrt(k)
 
 
+@pytest.mark.parametrize("n,nrhs,kind", [(1, 1, "getrf"), (37, 5, "getrf"), (1350, 1, "getrf"), (900, 7, "getrf"),
+                                         (37, 3, "general"), (1350, 1, "general")])
+def test_getrs_row_interchanges(ctx, n, nrhs, kind):
+    """zgetrs with identity factors is the row interchange alone (zlaswp): B <- P B for (t, ipiv[t]) in order,
+    with zgetrf's pivots (ipiv[t] >= t) and with arbitrary ones, against numpy applying the swaps."""
+    rng = np.random.default_rng(n + nrhs)
+    B = crand(rng, nrhs, n)  # column-major n x nrhs
+    if kind == "getrf":
+        ipiv = np.array([rng.integers(t, n) for t in range(n)], np.int32)
+    else:
+        ipiv = rng.integers(0, n, n).astype(np.int32)
+    ref = B.T.copy()
+    for t in range(n):
+        p = ipiv[t]
+        ref[[t, p]] = ref[[p, t]]
+    LU = np.eye(n, dtype=np.complex128)
+    dLU = sa.DeviceArray.from_numpy(ctx, LU)
+    dP = sa.DeviceArray.from_numpy(ctx, ipiv)
+    dB = sa.DeviceArray.from_numpy(ctx, B)
+    sa.check(sa.lib().cwx_zgetrs_batched_dev(ctx.h, n, nrhs, 1, dLU.ptr, n * n, dP.ptr, n, dB.ptr, n * nrhs))
+    X = dB.numpy().T
+    assert np.array_equal(X, ref)
+    for d in (dLU, dP, dB):
+        d.free()
+
+
 def op(X, t):
     return X if t == 0 else (X.T if t == 1 else X.conj().T)
 
