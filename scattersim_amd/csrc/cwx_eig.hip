@@ -112,6 +112,25 @@ __device__ inline void team_role(const Team& t, int& b, int& role) {
   }
 }
 
+// 1/x for normal x: v_rcp_f64 refined by two Newton steps (within an ulp of the rounded quotient).
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+// sqrt(x) for normal positive x: v_rsq_f64 and Goldschmidt's iteration for the root and its half
+// reciprocal, then one correction of the root (within an ulp of the rounded root).
+__device__ inline double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+}
+
 __device__ inline double dlapy3(double x, double y, double z) {
   double w = fmax(fabs(x), fmax(fabs(y), fabs(z)));
   if (w == 0.0) return fabs(x) + fabs(y) + fabs(z);
@@ -121,17 +140,18 @@ __device__ inline double dlapy3(double x, double y, double z) {
 // zlarfg for n = 2: beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha) / beta,
 // v2 = x / (alpha - beta).  When |(alpha, x)|^2 is comfortably inside the double range it is formed
 // directly, so the chain is one square root and two independent reciprocals (|alpha - beta| >= |beta|,
-// no cancellation); otherwise the scaled forms of zlarfg (dlapy3 and Smith's division) are used.
+// no cancellation), each from the hardware estimate and Newton steps; otherwise the scaled forms of
+// zlarfg (dlapy3 and Smith's division) are used.
 // Returns false when x = 0 and alpha is real (tau = 0, the identity).
 __device__ inline bool reflector2(zc& alpha, zc x, zc& tau, zc& v2) {
   if (x.re == 0.0 && x.im == 0.0 && alpha.im == 0.0) return false;
   const double s2 = fma(alpha.re, alpha.re, fma(alpha.im, alpha.im, norm2(x)));
   double beta;
   if (s2 > 1e-280 && s2 < 1e280) {
-    beta = -copysign(sqrt(s2), alpha.re);
-    const double rb = 1.0 / beta;
+    beta = -copysign(sqrt_nr(s2), alpha.re);
+    const double rb = rcp_nr(beta);
     const zc d = mk(alpha.re - beta, alpha.im);
-    const double rd = 1.0 / norm2(d);
+    const double rd = rcp_nr(norm2(d));
     tau = mk((beta - alpha.re) * rb, -alpha.im * rb);
     v2 = x * mk(d.re * rd, -d.im * rd);
   } else {
@@ -685,6 +705,31 @@ __device__ inline void rot2_right(const Rot2& g, zc& a, zc& b) {
   b.im = fma(-s.im, g.v2.re, fma(s.re, g.v2.im, b.im));
 }
 
+// Wilkinson's shift for the trailing 2 x 2 block [[a, b], [c, d]] of an active block (zlahqr): the
+// eigenvalue d - bc / (x + y) nearer to d, x = (a - d) / 2, y = +-sqrt(x^2 + bc) with Re(conj(x) y) >= 0.
+// It is the same eigenvalue zlahqr forms from sqrt(b) sqrt(c), with one complex square root and one
+// reciprocal.  The shift sets only the rate of convergence: a value that is not finite (range
+// exceeded) falls back to d.
+__device__ inline zc wilkinson_shift(zc a, zc b, zc c, zc d) {
+  const zc x = 0.5 * (a - d), p = b * c;
+  const zc q = x * x + p;
+  const double m = sqrt_nr(fmax(norm2(q), 1e-300));
+  const double tq = sqrt_nr(fmax(0.5 * (m + fabs(q.re)), 1e-300));
+  const double h = 0.5 * q.im * rcp_nr(tq);
+  zc y = (q.re >= 0.0) ? mk(tq, h) : mk(fabs(h), copysign(tq, q.im));
+  if (x.re * y.re + x.im * y.im < 0.0) y = -y;
+  const zc den = x + y;
+  const double nd = norm2(den);
+  zc t = d;
+  if (nd > 0.0) {
+    const double rn = rcp_nr(nd);
+    const zc r = p * conj(den);
+    t = d - mk(r.re * rn, r.im * rn);
+  }
+  if (!(fabs(t.re) + fabs(t.im) < 1e300)) t = d;
+  return t;
+}
+
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -742,21 +787,7 @@ __device__ __attribute__((always_inline)) inline bool small_eigvals_wave(zc* S, 
     } else if (kdefl % 10 == 0) {
       t = mk(0.75 * abs1(Sp(l + 1, l))) + Sp(l, l);
     } else {
-      t = Sp(i, i);
-      const zc u = zsqrt(Sp(i - 1, i)) * zsqrt(Sp(i, i - 1));
-      double s = abs1(u);
-      if (s != 0.0) {
-        const zc x = 0.5 * (Sp(i - 1, i - 1) - t);
-        const double sx = abs1(x);
-        s = fmax(s, sx);
-        const zc xs = x * (1.0 / s), us = u * (1.0 / s);
-        zc y = s * zsqrt(xs * xs + us * us);
-        if (sx > 0.0) {
-          const zc xn = x * (1.0 / sx);
-          if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
-        }
-        t = t - u * zdiv(u, x + y);
-      }
+      t = wilkinson_shift(Sp(i - 1, i - 1), Sp(i - 1, i), Sp(i, i - 1), Sp(i, i));
     }
     for (int kk = l; kk < i; ++kk) {
       zc v1, v2;
@@ -877,21 +908,7 @@ __device__ __attribute__((always_inline)) inline bool schur_wave(zc* S, zc* V, i
     } else if (kdefl % 10 == 0) {
       t = mk(0.75 * abs1(Sp(l + 1, l))) + Sp(l, l);
     } else {
-      t = Sp(i, i);
-      const zc u = zsqrt(Sp(i - 1, i)) * zsqrt(Sp(i, i - 1));
-      double s = abs1(u);
-      if (s != 0.0) {
-        const zc x = 0.5 * (Sp(i - 1, i - 1) - t);
-        const double sx = abs1(x);
-        s = fmax(s, sx);
-        const zc xs = x * (1.0 / s), us = u * (1.0 / s);
-        zc y = s * zsqrt(xs * xs + us * us);
-        if (sx > 0.0) {
-          const zc xn = x * (1.0 / sx);
-          if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
-        }
-        t = t - u * zdiv(u, x + y);
-      }
+      t = wilkinson_shift(Sp(i - 1, i - 1), Sp(i - 1, i), Sp(i, i - 1), Sp(i, i));
     }
     for (int kk = l; kk < i; ++kk) {
       zc v1, v2;
@@ -1259,6 +1276,49 @@ __device__ __attribute__((always_inline)) inline void ms_apply_line(zc* p, long 
   for (int q = 0; q < MS_S - 1; ++q) store(q, sb);
 }
 
+// The same left rotations on one column line, spread over the 16 lanes of a row of the wavefront so that
+// few lines (the leader's critical columns) do not run one rotation after another on one lane.  Lane j
+// applies bulge j's rotations: at step s they take elements (30 - 2j + s, 31 - 2j + s).  Lane j holds
+// the second element from step to step, and the first one it releases is the element that bulge j - 1
+// released one step earlier (bulges two rows apart), so each step passes one element from lane j - 1 to
+// lane j by a row shift.  Lane 0 takes fresh elements 32..46 instead.  An element released by bulge 15,
+// or at the last step, and every lane's final element are final for this window.  Each element sees the
+// same rotations in the same order as in ms_apply_line.  R: the window's rotations in LDS, step-major.
+template <bool FULL>
+__device__ __attribute__((always_inline)) inline void ms_apply_column_lanes(zc* p, int qlo, int qhi, const Rot2* R, int j) {
+  constexpr int Q0 = 2 * (MS_NB - 1);
+  static_assert(MS_NB == 16, "one bulge per lane of a 16-lane row");
+  auto in = [&](int q) { return FULL || (q >= qlo && q <= qhi); };
+  auto ld = [&](int q) { return in(q) ? p[q] : mk(0.0); };
+  auto shr1 = [](double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x111, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x111, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  };
+  zc fresh[MS_S - 1];
+  zc carry = ld(Q0 - 2 * j), nxt = ld(Q0 + 1 - 2 * j);
+  if (j == 0) {
+#pragma unroll
+    for (int s = 1; s < MS_S; ++s) fresh[s - 1] = ld(Q0 + 1 + s);
+  }
+#pragma unroll
+  for (int s = 0; s < MS_S; ++s) {
+    const Rot2 g = R[s * MS_NB + j];
+    rot2_left(g, carry, nxt);
+    const int q = Q0 - 2 * j + s;  // the element this lane releases
+    if ((j == MS_NB - 1 || s == MS_S - 1) && in(q)) p[q] = carry;
+    if (s + 1 < MS_S) {
+      const zc recv = mk(shr1(carry.re), shr1(carry.im));
+      carry = nxt;
+      nxt = (j == 0) ? fresh[s] : recv;
+    } else {
+      carry = nxt;
+    }
+  }
+  if (in(Q0 + MS_S - 2 * j)) p[Q0 + MS_S - 2 * j] = carry;
+}
+
 // Applies the single-shift rotations 0..wlen-1 (rotation j on elements j, j+1) to one line whose
 // element j lives at p[j * stride]; two elements are live at a time.
 template <bool LEFT>
@@ -1341,6 +1401,24 @@ __device__ inline void team_publish(int* ctl, int w) { wg_release_store(ctl, w +
 // Helper h (1 .. G-1) of matrix b: applies every published window to its lines, window after window,
 // until the leader stops.  Line l < n: row l of H (right rotations, rows above the window); n <= l < 2n:
 // row l - n of Z; 2n <= l < 3n: column l - 2n of H (left rotations, right of the critical columns).
+// Dense enumeration of one chase step's products.  The active bulges own na consecutive runs of
+// products whose lengths change by 2 from one bulge to the next: run0 + 2u (dir = +1: left products,
+// u counting from the leading bulge) or run0 - 2u (dir = -1: right products).  Returns the run u that
+// holds element e, i.e. P(u) <= e < P(u + 1) with P(u) = u run0 + dir u (u - 1), and its offset e - P(u)
+// in x.  The float estimate of the quadratic's root is exact to far below one for these sizes, and
+// one step each way corrects it.
+__device__ inline int chase_run(int e, int na, int run0, int dir, int& x) {
+  auto P = [&](int u) { return u * run0 + dir * u * (u - 1); };
+  const float b = (float)(run0 - dir);
+  const float uf = (dir > 0) ? 0.5f * (sqrtf(b * b + 4.0f * (float)e) - b)
+                             : 0.5f * (b - sqrtf(fmaxf(b * b - 4.0f * (float)e, 0.0f)));
+  int u = min(max((int)uf, 0), na - 1);
+  if (u + 1 < na && P(u + 1) <= e) ++u;
+  if (u > 0 && P(u) > e) --u;
+  x = e - P(u);
+  return u;
+}
+
 template <int NT>
 __device__ void hqr_help(int n, zc* H, int ldh, zc* Z, int ldz, const HqrTeam& t, int b, int h) {
   __shared__ int s_go, s_desc[DESC_INTS];
@@ -1371,9 +1449,23 @@ __device__ void hqr_help(int n, zc* H, int ldh, zc* Z, int ldz, const HqrTeam& t
     }
     __syncthreads();
     if (!__builtin_amdgcn_readfirstlane(s_go)) return;
-    // the slot's rotations are read through the scalar cache: drop what it holds of older windows
+    // the slot's rotations are read through the scalar cache: drop what it holds of older windows, then
+    // fetch the slot's 64-byte lines together (one dword of each, the waves taking turns, all loads in
+    // flight), so that the rotations streamed one ahead of their use hit instead of waiting on L2 each
     __builtin_amdgcn_s_dcache_inv();
     __builtin_amdgcn_s_waitcnt(0xC07F);
+    {
+      constexpr int kLines = MS_RING * 8 / 64, kWaves = NT / 64;
+      static_assert(kLines % kWaves == 0, "ring lines split evenly over the waves");
+      typedef __attribute__((address_space(4))) const unsigned KU32;
+      const unsigned* rp = reinterpret_cast<const unsigned*>(slots + (long)(w % NSLOT) * MS_RING);
+      asm volatile("" : "+s"(rp));
+      const KU32* ku = (const KU32*)rp;
+      unsigned acc = 0;
+#pragma unroll
+      for (int i = 0; i < kLines / kWaves; ++i) acc ^= ku[(i * kWaves + wid) * 16];
+      asm volatile("" ::"s"(acc));
+    }
     const int kbase = __builtin_amdgcn_readfirstlane(s_desc[0]), rlo = __builtin_amdgcn_readfirstlane(s_desc[1]);
     const int rhi = __builtin_amdgcn_readfirstlane(s_desc[2]);
     const bool full = __builtin_amdgcn_readfirstlane(s_desc[3]) != 0;
@@ -1571,9 +1663,9 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
             zc v1, v2;
             if (k == L) {
               const zc h11s = MBp(L, L) - shifts[j], h21 = MBp(L + 1, L);
-              const double sc = abs1(h11s) + abs1(h21);
-              v1 = h11s * (1.0 / sc);
-              v2 = h21 * (1.0 / sc);
+              const double rsc = rcp_nr(abs1(h11s) + abs1(h21));
+              v1 = h11s * rsc;
+              v2 = h21 * rsc;
             } else {
               v1 = MBp(k, k - 1);
               v2 = MBp(k + 1, k - 1);
@@ -1590,19 +1682,46 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
         }
         __syncthreads();
         const int nsteps = min(MS_S, T_total - t0);
+        const bool wfull = kbase >= L && kbase + MS_LEN - 1 <= I;
+        auto left_product = [&](int s, int t, int j, int x) {
+          const int k = L + t - 2 * j, c = k + x;
+          rot2_left(rots2[s * MS_NB + j], MBp(k, c), MBp(k + 1, c));
+          if (x == 0 && k > L) {
+            MBp(k, k - 1) = betas[j];
+            MBp(k + 1, k - 1) = mk(0.0);
+          }
+        };
+        auto right_product = [&](int s, int t, int j, int x) {
+          const int k = L + t - 2 * j;
+          rot2_right(rots2[s * MS_NB + j], MBp(rlo + x, k), MBp(rlo + x, k + 1));
+        };
         for (int s = 0; s < nsteps; ++s) {
           const int t = t0 + s;
+          // The bulges inside the active block at this step, j = jlo..jhi (k = L + t - 2j in [L, I-1]),
+          // own contiguous runs of products: rhi - k + 1 left products each (columns k..rhi), and
+          // k - rlo + 1 right products (rows rlo..k).  The runs are enumerated densely over the threads.
+          const int jlo = max(0, (L + t - I + 2) >> 1), na = min(nb - 1, t >> 1) - jlo + 1;
+          const int k0 = L + t - 2 * jlo;  // the leading active bulge's row
+          // With all bulges inside a window that lies inside the active block, bulge j's runs are
+          // 18 - s + 2j left and s + 31 - 2j right products, and wavefront w takes bulges w and 15 - w,
+          // whose runs add up to 66 - 2s and 2s + 32 lines.
+          const bool paired = wfull && jlo == 0 && na == MS_NB;
+          const int lane = tid & 63, nwav = nt >> 6;
           // left products on rows k, k+1 (columns k..rhi) and the bulge column of chase steps
-          for (int e = tid; e < MS_NB * MS_LEN; e += nt) {
-            const int j = e / MS_LEN, ci = e % MS_LEN;
-            const int k = L + t - 2 * j;
-            if (j >= nb || k < L || k > I - 1) continue;
-            const int c = k + ci;
-            if (c > rhi) continue;
-            rot2_left(rots2[s * MS_NB + j], MBp(k, c), MBp(k + 1, c));
-            if (ci == 0 && k > L) {
-              MBp(k, k - 1) = betas[j];
-              MBp(k + 1, k - 1) = mk(0.0);
+          if (paired) {
+            for (int pr = tid >> 6; pr < MS_NB / 2; pr += nwav) {
+              const int j2 = MS_NB - 1 - pr, c1 = 18 - s + 2 * pr, c2 = 18 - s + 2 * j2;
+              for (int xe = lane; xe < c1 + c2; xe += 64) {
+                if (xe < c1) left_product(s, t, pr, xe);
+                else left_product(s, t, j2, xe - c1);
+              }
+            }
+          } else if (na > 0) {
+            const int run0 = rhi - k0 + 1, total = na * run0 + na * (na - 1);
+            for (int e = tid; e < total; e += nt) {
+              int x;
+              const int j = jlo + chase_run(e, na, run0, 1, x);
+              left_product(s, t, j, x);
             }
           }
           __syncthreads();
@@ -1612,23 +1731,59 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
           // and then forms that reflector: a bulge that enters at the next step reads H(L, L) and
           // H(L+1, L), final after the left products above.
           const int rl = tid - (nt - 64);
-          if (rl >= 0) {
+          if (rl >= 0 && rl < MS_NB) {
+            // first, as the next step waits for it: rows k+1 and k+2 of bulge j and its next reflector
+            // from the two values just formed (k+1 > L), or from H(L, L), H(L+1, L) when it enters
             const int j = rl;
-            if (j < MS_NB) {
-              const int k = L + t - 2 * j;
-              if (j < nb && k >= L && k <= I - 1)
-                for (int r = max(k + 1, rlo); r <= min(k + 2, I); ++r)
-                  rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
-              reflector(j, t + 1, rots2[(s + 1) * MS_NB + j], betas[j]);
+            const int k = L + t - 2 * j;
+            zc a1 = mk(0.0), a2 = mk(0.0);
+            if (j < nb && k >= L && k <= I - 1) {
+              const Rot2 g = rots2[s * MS_NB + j];
+              zc b1 = MBp(k + 1, k + 1);
+              a1 = MBp(k + 1, k);
+              rot2_right(g, a1, b1);
+              MBp(k + 1, k) = a1;
+              MBp(k + 1, k + 1) = b1;
+              if (k + 2 <= I) {
+                zc b2 = MBp(k + 2, k + 1);
+                a2 = MBp(k + 2, k);
+                rot2_right(g, a2, b2);
+                MBp(k + 2, k) = a2;
+                MBp(k + 2, k + 1) = b2;
+              }
             }
-          } else {
-            for (int e = tid; e < MS_NB * MS_LEN; e += nt - 64) {
-              const int j = e / MS_LEN, ri = e % MS_LEN;
-              const int k = L + t - 2 * j;
-              if (j >= nb || k < L || k > I - 1) continue;
-              const int r = rlo + ri;
-              if (r > min(k, I)) continue;
-              rot2_right(rots2[s * MS_NB + j], MBp(r, k), MBp(r, k + 1));
+            Rot2 gn;
+            gn.t1 = gn.v2 = gn.t2 = mk(0.0);
+            zc bn = mk(0.0);
+            const int kn = k + 1;
+            if (j < nb && kn >= L && kn <= I - 1) {
+              zc v1 = a1, v2 = a2;
+              if (kn == L) {
+                const zc h11s = MBp(L, L) - shifts[j], h21 = MBp(L + 1, L);
+                const double rsc = rcp_nr(abs1(h11s) + abs1(h21));
+                v1 = h11s * rsc;
+                v2 = h21 * rsc;
+              }
+              gn = make_reflector2(v1, v2);
+              bn = v1;
+            }
+            rots2[(s + 1) * MS_NB + j] = gn;
+            betas[j] = bn;
+          }
+          if (paired) {
+            for (int pr = tid >> 6; pr < MS_NB / 2; pr += nwav) {
+              const int j2 = MS_NB - 1 - pr, c1 = s + 31 - 2 * pr, c2 = s + 31 - 2 * j2;
+              for (int xe = lane; xe < c1 + c2; xe += 64) {
+                if (xe < c1) right_product(s, t, pr, xe);
+                else right_product(s, t, j2, xe - c1);
+              }
+            }
+          } else if (na > 0) {
+            const int run0 = k0 - rlo + 1, total = na * run0 - na * (na - 1);
+            for (int e = tid; e < total; e += nt) {
+              int x;
+              const int j = jlo + chase_run(e, na, run0, -1, x);
+              right_product(s, t, j, x);
             }
           }
           __syncthreads();
@@ -1684,9 +1839,9 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
             return;
           }
         }
-        __builtin_amdgcn_s_dcache_inv();
-        __builtin_amdgcn_s_waitcnt(0xC07F);
-        {
+        if (!teamed) {
+          __builtin_amdgcn_s_dcache_inv();
+          __builtin_amdgcn_s_waitcnt(0xC07F);
           // the waves fetch the ring's 64-byte lines into the scalar cache together (one dword of
           // each, all loads in flight), so the deferred products start on hits
           constexpr int kLines = MS_RING * 8 / 64, kWaves = NT / 64;
@@ -1705,7 +1860,15 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
         // rows above it and all rows of Z (right rotations on columns kbase + q; the helpers' share
         // in a team)
         const int nfr = teamed ? chi - rhi : n - 1 - rhi, nfa = rlo, nz = n;
-        const int lines = teamed ? nfr : nfr + nfa + nz;
+        const int lines = teamed ? 0 : nfr + nfa + nz;
+        if (teamed) {
+          // the critical columns, 16 lanes each
+          for (int ln = tid >> 4; ln < nfr; ln += nt >> 4) {
+            zc* const p = H + (long)(rhi + 1 + ln) * ldh + kbase;
+            if (full) ms_apply_column_lanes<true>(p, 0, MS_LEN - 1, rots2, tid & 15);
+            else ms_apply_column_lanes<false>(p, rlo - kbase, rhi - kbase, rots2, tid & 15);
+          }
+        }
         constexpr unsigned long long kSign = 1ull << 63;
         for (int ln = tid; ln < lines; ln += nt) {
           const KDouble* R = scalar_view(reinterpret_cast<const double*>(rg));
@@ -1747,21 +1910,7 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
       } else if (kdefl % 10 == 0) {
         t = mk(0.75 * fabs(Hp(L + 1, L).re)) + Hp(L, L);
       } else {
-        t = Hp(I, I);
-        zc u = zsqrt(Hp(I - 1, I)) * zsqrt(Hp(I, I - 1));
-        double s = abs1(u);
-        if (s != 0.0) {
-          zc x = 0.5 * (Hp(I - 1, I - 1) - t);
-          const double sx = abs1(x);
-          s = fmax(s, abs1(x));
-          zc xs = x * (1.0 / s), us = u * (1.0 / s);
-          zc y = s * zsqrt(xs * xs + us * us);
-          if (sx > 0.0) {
-            zc xn = x * (1.0 / sx);
-            if (xn.re * y.re + xn.im * y.im < 0.0) y = -y;
-          }
-          t = t - u * zdiv(u, x + y);
-        }
+        t = wilkinson_shift(Hp(I - 1, I - 1), Hp(I - 1, I), Hp(I, I - 1), Hp(I, I));
       }
       s_t = t;
       s_m = L;
@@ -2106,8 +2255,9 @@ void zgeev_batched(int n, zc* A, int lda, long sA, zc* W, long sW, zc* V, int ld
     // leaders work alone
     int per_cu = 0;
     CWX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_hqr<EIG_THREADS>, EIG_THREADS, 0));
-    const long grid = 8L * ((batch + 7) / 8) * team.G;
-    if ((long)per_cu * cus < grid) team.G = 1;
+    // as many helpers as fit beside the leaders
+    const long slots = (long)per_cu * cus, lead = 8L * ((batch + 7) / 8);
+    if (slots < lead * team.G) team.G = (int)std::max<long>(1, slots / lead);
   }
   if (team.G > 1) {
     work.ensure_team(batch);
