@@ -84,8 +84,9 @@ class MLP:
     def nparams(self) -> int:
         return sum(self.dims[i] * self.dims[i + 1] + self.dims[i + 1] for i in range(len(self.dims) - 1))
 
-    def forward(self, X: DeviceArray, row0: int, rows: int) -> DeviceArray:
-        """Rows row0 .. row0+rows of X (BF16, dims[0] columns) through the network; returns self.Y."""
+    def forward(self, X: DeviceArray, row0: int, rows: int, keep: bool = True) -> DeviceArray:
+        """Rows row0 .. row0+rows of X (BF16, dims[0] columns) through the network; returns self.Y.
+        keep=False (inference): the pre-activations that only the backward pass reads are not written."""
         L = lib()
         h = self.ctx.h
         assert rows <= self.batch and X.dtype == np.uint16 and X.shape[1] == self.dims[0]
@@ -93,7 +94,7 @@ class MLP:
         for i in range(len(self.dims) - 1):
             nin, nout = self.dims[i], self.dims[i + 1]
             last = i == len(self.dims) - 2
-            outs = (_p(self.Z[i]), None if last else _p(self.H[i]), _p(self.Y) if last else None)
+            outs = (_p(self.Z[i]) if keep else None, None if last else _p(self.H[i]), _p(self.Y) if last else None)
             if nin % 4 == 0:
                 # one launch: product, bias and activation
                 check(L.cwx_mlp_dense_fwd(h, rows, nout, nin, Hin, ld_in, _p(self.Wb[i]), nin, _p(self.b[i]),
@@ -319,7 +320,7 @@ class Closure:
         out = np.zeros((len(X), cfg.d_out), np.float32)
         for r0 in range(0, len(X), batch):
             rows = min(batch, len(X) - r0)
-            Y = net.forward(dX, r0, rows)
+            Y = net.forward(dX, r0, rows, keep=False)
             out[r0:r0 + rows] = Y.numpy()[:rows]
         corr = out * self.t_std
         h = corr.shape[1] // 2

@@ -86,10 +86,35 @@ def test_activation_loss_and_adam_kernels(ctx):
     assert np.array_equal(wb.numpy(), bf(dw.numpy()))
 
 
-@pytest.mark.parametrize("dims", [[37, 29, 41, 23], [36, 28, 40, 22]])
+@pytest.mark.parametrize("M,N,K,ldx", [(1, 8, 8, 8), (129, 136, 72, 80), (300, 2700, 256, 256), (257, 256, 928, 936)])
+@pytest.mark.parametrize("act", [0, 1])
+def test_dense_layer_matches_fp32_accumulation(ctx, M, N, K, ldx, act):
+    """The fused dense layer (Z = X W^T + bias, H = act(Z)) at widths that are multiples of 8, which take the
+    LDS-staged double-depth kernel: row and column tails, a depth that is not a multiple of 32, a row
+    stride wider than K, against FP64 products of the same BF16 values."""
+    rng = np.random.default_rng(M + N + K + act)
+    X = co.bf16(rng.standard_normal((M, ldx)))
+    W = co.bf16(rng.standard_normal((N, K)))
+    b = rng.standard_normal(N).astype(np.float32)
+    dX, dW, db = dev(ctx, bf(X)), dev(ctx, bf(W)), dev(ctx, b)
+    Z = sa.DeviceArray(ctx, (M, N), np.float32)
+    H = sa.DeviceArray(ctx, (M, N), np.uint16)
+    Hf = sa.DeviceArray(ctx, (M, N), np.float32)
+    sa.check(sa.lib().cwx_mlp_dense_fwd(ctx.h, M, N, K, cl._p(dX), ldx, cl._p(dW), K, cl._p(db), act, cl._p(Z),
+                                        cl._p(H), cl._p(Hf)))
+    zref = X[:, :K].astype(np.float64) @ W.astype(np.float64).T + b
+    tol = 1e-5 * max(1.0, np.max(np.abs(zref))) * np.sqrt(K)
+    assert np.max(np.abs(Z.numpy() - zref)) <= tol
+    href = co.gelu(zref)[0] if act else zref
+    assert np.max(np.abs(Hf.numpy() - href)) <= 2 * tol
+    assert np.array_equal(H.numpy(), bf(Hf.numpy()))
+
+
+@pytest.mark.parametrize("dims", [[37, 29, 41, 23], [36, 28, 40, 22], [40, 32, 48, 24]])
 def test_network_gradients_match_oracle(ctx, dims):
     """Forward and backward of a three-layer network on the GPU against the oracle: odd widths take the
-    strided GEMM and a separate bias/activation pass, widths that are multiples of 4 the fused dense layer."""
+    strided GEMM and a separate bias/activation pass, widths that are multiples of 4 the fused dense layer
+    (multiples of 8: its LDS-staged form)."""
     rng = np.random.default_rng(11)
     B = 19
     net = cl.MLP(ctx, dims, B, seed=5)
