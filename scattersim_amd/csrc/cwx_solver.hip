@@ -599,35 +599,112 @@ __global__ void k_build_f(InstArgs ia, int which, zc* __restrict__ F) {
     F[e] = f_entry(a, which, (int)(e % n4), (int)(e / n4));
 }
 
-// A (6MF x 6MF) and b (ScatterVolume.cpp:126-131, 398-456): Gauss rows, [f2 | Gd0 Gc^-1 f3], -f1 EF;
-// one instance per blockIdx.y.
-__global__ void k_build_system(InstArgs ia, const zc* __restrict__ Abr, zc* __restrict__ A, zc* __restrict__ b) {
+// ---- the boundary system by elimination of the reflected waves ----
+// A x = b (ScatterVolume.cpp:398-456) has, for every order m, five equations in which the reflected
+// amplitudes R_m = (Rx, Ry, Rz)_m appear: the Gauss row (Sx, Sy, -Sz0) and rows m of the four f2 blocks
+// (identity, identity, (0, Sz0, Sy), (-Sz0, 0, -Sx)), each of the last four also holding row m of one
+// block of M = Gd0 S^-1 f3 against the transmitted amplitudes T.  A unitary Q_m^H (three Householder
+// reflectors of the 5 x 3 coefficient block C_m) turns them into three rows that give R_m from T and two
+// rows W_m (the left null space of C_m) that hold T alone.  Those 2 MF rows and the MF Gauss rows of T
+// form a 3MF x 3MF system for T; R_m follows as G_m z_m with G_m = U_m^-1 (Q_m^H)(0:3, :) and z_m the
+// five right-hand sides less the T terms.  In exact arithmetic this is the solution of A x = b; the
+// elimination is orthogonal and C_m has full column rank (Sx^2 + Sy^2 + Sz0^2 = n0^2), so it is stable.
+// Per order and instance: W_m (2 x 5) then G_m (3 x 5), row by row, 25 complex.
+constexpr int kBndW = 25;
+
+__global__ void k_bnd_qr(InstArgs ia, zc* __restrict__ Wq) {
   const int i = blockIdx.y;
   const BoundaryArgs a = boundary_args(ia, i);
-  const int MF = a.MF, n6 = 6 * MF, n4 = 4 * MF, n3 = 3 * MF;
+  const int MF = a.MF;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= MF) return;
+  const zc sx = a.Sx[m], sy = a.Sy[m], s0 = a.Sz0[m], z = mk(0.0), one = mk(1.0);
+  // [C_m | I_5]: coefficients of (Rx, Ry, Rz) in the Gauss row and the four f2 rows
+  zc g[5][8] = {{sx, sy, -s0, one, z, z, z, z},
+                {one, z, z, z, one, z, z, z},
+                {z, one, z, z, z, one, z, z},
+                {z, s0, sy, z, z, z, one, z},
+                {-s0, z, -sx, z, z, z, z, one}};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // zlarfg on column k, rows k..4; H^H = I - conj(tau) v v^H applied to every column
+    double xn2 = 0.0;
+#pragma unroll
+    for (int r = k + 1; r < 5; ++r) xn2 += norm2(g[r][k]);
+    const zc alpha = g[k][k];
+    if (xn2 == 0.0 && alpha.im == 0.0) continue;
+    const double beta = -copysign(sqrt(norm2(alpha) + xn2), alpha.re);
+    const zc tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+    const zc sc = zdiv(one, alpha - mk(beta));
+    zc v[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = (r < k) ? z : (r == k ? one : g[r][k] * sc);
+    const zc tc = conj(tau);
+#pragma unroll
+    for (int c = k; c < 8; ++c) {
+      zc d = z;
+#pragma unroll
+      for (int r = k; r < 5; ++r) d = d + conj(v[r]) * g[r][c];
+      d = tc * d;
+#pragma unroll
+      for (int r = k; r < 5; ++r) g[r][c] = g[r][c] - v[r] * d;
+    }
+    g[k][k] = mk(beta);
+#pragma unroll
+    for (int r = k + 1; r < 5; ++r) g[r][k] = z;
+  }
+  zc* w = Wq + ((long)i * MF + m) * kBndW;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) w[r * 5 + c] = g[3 + r][3 + c];
+  // G = U^-1 (Q^H)(0:3, :) by back substitution on the upper triangular U = g(0:3, 0:3)
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    zc x2 = zdiv(g[2][3 + c], g[2][2]);
+    zc x1 = zdiv(g[1][3 + c] - g[1][2] * x2, g[1][1]);
+    zc x0 = zdiv(g[0][3 + c] - g[0][1] * x1 - g[0][2] * x2, g[0][0]);
+    w[10 + c] = x0;
+    w[15 + c] = x1;
+    w[20 + c] = x2;
+  }
+}
+
+// The reduced 3MF x 3MF matrix per instance (column-major, leading dimension n3, instances sA apart):
+// rows 0..MF-1 the Gauss rows of T (ScatterVolume.cpp:398-413), row MF + a MF + m the null-space row a
+// of order m against the rows of M = Gd0 S^-1 f3 (Abr, 4MF x 3MF column-major).
+__global__ void k_bnd_matrix(InstArgs ia, const zc* __restrict__ Abr, const zc* __restrict__ Wq, zc* __restrict__ Ar,
+                             long sA) {
+  const int i = blockIdx.y;
+  const BoundaryArgs a = boundary_args(ia, i);
+  const int MF = a.MF, n4 = 4 * MF, n3 = 3 * MF;
   Abr += (long)i * n4 * n3;
-  A += (long)i * n6 * n6;
-  b += (long)i * n6;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n6 * n6; e += (long)gridDim.x * blockDim.x) {
-    const int r = (int)(e % n6), c = (int)(e / n6);
+  Ar += (long)i * sA;
+  Wq += (long)i * MF * kBndW;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n3 * n3; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n3), c = (int)(e / n3);
     zc v = mk(0.0);
     if (r < MF) {
-      const int m = r;
-      if (c == m) v = a.Sx[m];
-      else if (c == MF + m) v = a.Sy[m];
-      else if (c == 2 * MF + m) v = -a.Sz0[m];
-    } else if (r < 2 * MF) {
-      const int m = r - MF;
-      if (c == 3 * MF + m) v = a.Sx[m];
-      else if (c == 4 * MF + m) v = a.Sy[m];
-      else if (c == 5 * MF + m) v = a.Sz1[m];
+      if (c == r) v = a.Sx[r];
+      else if (c == MF + r) v = a.Sy[r];
+      else if (c == 2 * MF + r) v = a.Sz1[r];
     } else {
-      const int rr = r - 2 * MF;
-      if (c < n3) v = f_entry(a, 2, rr, c);
-      else v = Abr[(long)(c - n3) * n4 + rr];
+      const int rr = r - MF, row = rr / MF, m = rr % MF;
+      const zc* w = Wq + (long)m * kBndW + row * 5;
+      const zc* col = Abr + (long)c * n4 + m;
+#pragma unroll
+      for (int j = 1; j < 5; ++j) v = v + w[j] * col[(long)(j - 1) * MF];
     }
-    A[e] = v;
+    Ar[(long)c * n3 + r] = v;
   }
+}
+
+// b = [0; -f1 EF] (ScatterVolume.cpp:454) into x, one instance per blockIdx.y.
+__global__ void k_bnd_b(InstArgs ia, zc* __restrict__ b) {
+  const int i = blockIdx.y;
+  const BoundaryArgs a = boundary_args(ia, i);
+  const int MF = a.MF, n6 = 6 * MF, n3 = 3 * MF;
+  b += (long)i * n6;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n6; r += gridDim.x * blockDim.x) {
     zc v = mk(0.0);
     if (r >= 2 * MF) {
@@ -639,6 +716,58 @@ __global__ void k_build_system(InstArgs ia, const zc* __restrict__ Abr, zc* __re
       v = -v;
     }
     b[r] = v;
+  }
+}
+
+// Right-hand sides of the reduced system from full right-hand sides b (rows 2MF..6MF hold -f1 EF, the Gauss
+// rows are zero): t(m) = 0 for the Gauss rows of T, t(MF + a MF + m) = sum_j W_m(a, j) b_m(j).  Instance
+// blockIdx.z, right-hand side blockIdx.y: b at xs + (z nrhs + y) 6MF, t at t + (z nrhs + y) 3MF.
+__global__ void k_bnd_rhs(InstArgs ia, const zc* __restrict__ Wq, const zc* __restrict__ xs, int nrhs,
+                          zc* __restrict__ t) {
+  const int i = blockIdx.z, k = blockIdx.y;
+  const int MF = ia.MF, n6 = 6 * MF, n3 = 3 * MF;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= MF) return;
+  const zc* b = xs + ((long)i * nrhs + k) * n6;
+  zc* tt = t + ((long)i * nrhs + k) * n3;
+  const zc* w = Wq + ((long)i * MF + m) * kBndW;
+  tt[m] = mk(0.0);
+#pragma unroll
+  for (int row = 0; row < 2; ++row) {
+    zc v = mk(0.0);
+#pragma unroll
+    for (int j = 1; j < 5; ++j) v = v + w[row * 5 + j] * b[(j + 1) * MF + m];
+    tt[MF + row * MF + m] = v;
+  }
+}
+
+// x = [R; T] from the reduced solution T (t) and y = M T (Y, 4MF per right-hand side): R_m = G_m z_m with
+// z_m = (0, b_m(j) - y_m(j)).  Each thread reads its order's right-hand side rows before it overwrites them.
+__global__ void k_bnd_back(InstArgs ia, const zc* __restrict__ Wq, const zc* __restrict__ Y, const zc* __restrict__ t,
+                           zc* __restrict__ xs, int nrhs) {
+  const int i = blockIdx.z, k = blockIdx.y;
+  const int MF = ia.MF, n6 = 6 * MF, n4 = 4 * MF, n3 = 3 * MF;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= MF) return;
+  zc* x = xs + ((long)i * nrhs + k) * n6;
+  const zc* y = Y + ((long)i * nrhs + k) * n4;
+  const zc* tt = t + ((long)i * nrhs + k) * n3;
+  const zc* w = Wq + ((long)i * MF + m) * kBndW + 10;
+  zc zv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) zv[j] = x[(j + 2) * MF + m] - y[j * MF + m];
+  zc R[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zc v = mk(0.0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v = v + w[c * 5 + 1 + j] * zv[j];
+    R[c] = v;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c * MF + m] = R[c];
+    x[(3 + c) * MF + m] = tt[c * MF + m];
   }
 }
 
@@ -1042,7 +1171,7 @@ struct SetBufs {
   DevBuf<int> d_fslot, d_flist, d_finfo;
   std::vector<int> flist;
   // chain, boundary system, beta
-  DevBuf<zc> Gd_local, Gc_local, beta_local, Y, Abr, A;
+  DevBuf<zc> Gd_local, Gc_local, beta_local, Y, Abr, A, Wq;
   DevBuf<int> d_info_lu, infoS, infoA, d_nonfinite;
   DevBuf<zc>* gam = nullptr;
   DevBuf<zc>* Gd = nullptr;
@@ -1145,6 +1274,7 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
   b.Y.ensure((size_t)count * n4 * n3);
   b.Abr.ensure((size_t)count * n4 * n3);
   b.A.ensure((size_t)count * n6 * n6);
+  b.Wq.ensure((size_t)count * MF * kBndW);
   b.x = res.x;
   b.x->alloc((size_t)count * n6);
   b.beta = res.full ? &res.full->beta : &b.beta_local;
@@ -1359,23 +1489,41 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, b.Y.p, n4, (long)n4 * n3, count, s);
   zgemm_set(OpN, OpN, n4, n3, n4, mk(1.0), pl.mats(b.oGd0), n4, strided<zc>(b.Y.p, (long)n4 * n3), n4, mk(0.0),
             strided<zc>(b.Abr.p, (long)n4 * n3), n4, count, s);
-  hipLaunchKernelGGL(k_build_system, dim3(grid_for((long)n6 * n6), count), dim3(256), 0, s, ia, b.Abr.p, b.A.p, xp);
-  zgetrf_batched(n6, b.A.p, n6, (long)n6 * n6, pivA, n6, b.infoA.p, count, s);
-  zgetrs_batched(n6, 1, b.A.p, n6, (long)n6 * n6, pivA, n6, xp, n6, n6, count, s);
+  // the reflected amplitudes eliminated order by order: a 3MF x 3MF system for T (A's storage holds it)
+  const unsigned gmf = cdiv(MF, 128);
+  hipLaunchKernelGGL(k_bnd_qr, dim3(gmf, count), dim3(128), 0, s, ia, b.Wq.p);
+  hipLaunchKernelGGL(k_bnd_matrix, dim3(grid_for((long)n3 * n3), count), dim3(256), 0, s, ia, b.Abr.p, b.Wq.p, b.A.p,
+                     (long)n6 * n6);
+  hipLaunchKernelGGL(k_bnd_b, dim3(grid_for(n6), count), dim3(256), 0, s, ia, xp);
+  zgetrf_batched(n3, b.A.p, n3, (long)n6 * n6, pivA, n6, b.infoA.p, count, s);
+  {
+    // S^-1 f3 (b.Y) is spent once Abr holds Gd0 S^-1 f3: the reduced right-hand sides and M T go there
+    zc* tr = b.Y.p;
+    zc* ym = tr + (long)count * n3;
+    hipLaunchKernelGGL(k_bnd_rhs, dim3(gmf, 1, count), dim3(128), 0, s, ia, b.Wq.p, xp, 1, tr);
+    zgetrs_batched(n3, 1, b.A.p, n3, (long)n6 * n6, pivA, n6, tr, n3, n3, count, s);
+    zgemm_batched(OpN, OpN, n4, 1, n3, mk(1.0), b.Abr.p, n4, (long)n4 * n3, tr, n3, n3, mk(0.0), ym, n4, n4, count, s);
+    hipLaunchKernelGGL(k_bnd_back, dim3(gmf, 1, count), dim3(128), 0, s, ia, b.Wq.p, ym, tr, xp, 1);
+  }
   if (res.K > 0) {
     // b_k = [0; -f1 EF_k] for every further field, then the same factors (ScatterVolume.cpp:739-740 per field)
     CWX_CHECK(count == 1, "several incident fields are solved for one instance at a time");
     const int K = res.K;
-    DevBuf<zc> F1((size_t)n4 * n3);
+    const int KB = std::min(K, 4096);
+    DevBuf<zc> F1((size_t)n4 * n3), tr((size_t)n3 * KB), ym((size_t)n4 * KB);
     hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), 1), dim3(256), 0, s, ia, 1, F1.p);
     hipLaunchKernelGGL(k_fill, dim3(grid_for((long)n6 * K)), dim3(256), 0, s, (long)n6 * K, res.xs);
-    for (int k0 = 0; k0 < K; k0 += 65535) {
-      const int kb = std::min(65535, K - k0);
-      zgemm_batched(OpN, OpN, n4, kb, n3, mk(-1.0), F1.p, n4, 0, res.EFs + (long)k0 * n3, n3, 0, mk(0.0),
-                    res.xs + (long)k0 * n6 + n2, n6, 0, 1, s);
-      zgetrs_batched(n6, kb, b.A.p, n6, (long)n6 * n6, pivA, n6, res.xs + (long)k0 * n6, n6, 0, 1, s);
+    for (int k0 = 0; k0 < K; k0 += KB) {
+      const int kb = std::min(KB, K - k0);
+      zc* xk = res.xs + (long)k0 * n6;
+      zgemm_batched(OpN, OpN, n4, kb, n3, mk(-1.0), F1.p, n4, 0, res.EFs + (long)k0 * n3, n3, 0, mk(0.0), xk + n2,
+                    n6, 0, 1, s);
+      hipLaunchKernelGGL(k_bnd_rhs, dim3(gmf, kb, 1), dim3(128), 0, s, ia, b.Wq.p, xk, kb, tr.p);
+      zgetrs_batched(n3, kb, b.A.p, n3, 0, pivA, 0, tr.p, n3, 0, 1, s);
+      zgemm_batched(OpN, OpN, n4, kb, n3, mk(1.0), b.Abr.p, n4, 0, tr.p, n3, 0, mk(0.0), ym.p, n4, 0, 1, s);
+      hipLaunchKernelGGL(k_bnd_back, dim3(gmf, kb, 1), dim3(128), 0, s, ia, b.Wq.p, ym.p, tr.p, xk, kb);
     }
-    CWX_HIP(hipStreamSynchronize(s));  // F1 is released on return
+    CWX_HIP(hipStreamSynchronize(s));  // F1, tr and ym are released on return
   }
 
   // ---- 6. beta recurrence, Q_check / Q_hat, Ez rows ----
