@@ -386,6 +386,32 @@ def load(path: str) -> Closure:
     return c
 
 
+def time_forward(ctx, rows: int, path: str = None):
+    """The shipped network's inference forward pass on `rows` random feature rows: 3 untimed and 10 timed
+    passes on the library's stream; prints ms per pass and TFLOP/s."""
+    import os
+    import time
+    net = load(path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "closure_c4.npz"))
+    cfg = net.cfg
+    d0 = _pad4(cfg.d_in)
+    dims = [d0, *cfg.hidden, cfg.d_out]
+    w = list(net.weights)
+    w[0] = _pad_cols(np.asarray(w[0], np.float32), d0)
+    mlp = MLP(ctx, dims, rows, weights=w)
+    X = _bf16_bits(np.random.default_rng(0).standard_normal((rows, d0)).astype(np.float32))
+    dX = DeviceArray.from_numpy(ctx, X)
+    for _ in range(3):
+        mlp.forward(dX, 0, rows, keep=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        mlp.forward(dX, 0, rows, keep=False)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / 10
+    fl = 2.0 * rows * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    print(f"forward {t * 1e3:.3f} ms, {fl / t / 1e12:.1f} TFLOP/s (dims {dims}, {rows} rows)", flush=True)
+
+
 def main(argv=None):
     """Generate the closure's training pairs with the solver, train on the GPU, evaluate on held-out layers
     and save the network:  python -m scattersim_amd.closure --count 20000 --test 1000 --out c4.npz"""
@@ -400,9 +426,17 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="where the trained network is saved")
+    ap.add_argument("--time-forward", type=int, default=0, metavar="ROWS",
+                    help="only time the shipped network's inference forward pass at ROWS rows (3 + 10 passes)")
     a = ap.parse_args(argv)
     ctx = Context(0)
+    if a.time_forward:
+        time_forward(ctx, a.time_forward)
+        ctx.close()
+        return
+    if not a.out:
+        ap.error("--out is required for training")
     cfg = ClosureConfig()
     t0 = time.time()
     data = make_dataset(ctx, cfg, a.count + a.test, seed=a.seed, log=lambda s: print(s, flush=True))
