@@ -1703,9 +1703,10 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
           const int jlo = max(0, (L + t - I + 2) >> 1), na = min(nb - 1, t >> 1) - jlo + 1;
           const int k0 = L + t - 2 * jlo;  // the leading active bulge's row
           // With all bulges inside a window that lies inside the active block, bulge j's runs are
-          // 18 - s + 2j left and s + 31 - 2j right products, and wavefront w takes bulges w and 15 - w,
-          // whose runs add up to 66 - 2s and 2s + 32 lines.
-          const bool paired = wfull && jlo == 0 && na == MS_NB;
+          // 18 - s + 2j left and s + 31 - 2j right products, and with eight wavefronts wavefront w takes
+          // bulges w and 15 - w, whose runs add up to 66 - 2s and 2s + 32 lines (with four, the dense
+          // enumeration needs fewer passes).
+          const bool paired = nt >= 512 && wfull && jlo == 0 && na == MS_NB;
           const int lane = tid & 63, nwav = nt >> 6;
           // left products on rows k, k+1 (columns k..rhi) and the bulge column of chase steps
           if (paired) {
