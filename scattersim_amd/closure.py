@@ -333,8 +333,9 @@ class Closure:
 
 
 def train(ctx, cfg: ClosureConfig, data: Dataset, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0,
-          log=None) -> Closure:
-    """Train the closure on data with Adam (mini-batches, reshuffled every epoch); returns the network."""
+          log=None, schedule: str = "const") -> Closure:
+    """Train the closure on data with Adam (mini-batches, reshuffled every epoch); returns the network.
+    schedule "cosine" decays the step size from lr to lr / 20 over the epochs along a half cosine."""
     x_mean = data.X.mean(axis=0)
     x_std = data.X.std(axis=0) + 1e-12
     t_std = np.full(data.T.shape[1], data.T.std() + 1e-30, np.float32)
@@ -346,6 +347,9 @@ def train(ctx, cfg: ClosureConfig, data: Dataset, epochs: int, batch: int = 256,
     n = len(Xn)
     hist = []
     for ep in range(epochs):
+        lr_ep = lr
+        if schedule == "cosine":
+            lr_ep = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs, 1))))
         perm = rng.permutation(n)
         dX = DeviceArray.from_numpy(ctx, _bf16_bits(Xn[perm]))
         dT = DeviceArray.from_numpy(ctx, Tn[perm])
@@ -354,7 +358,7 @@ def train(ctx, cfg: ClosureConfig, data: Dataset, epochs: int, batch: int = 256,
             net.forward(dX, r0, batch)
             net.mse(dT, r0, batch)
             net.backward(dX, r0, batch, net.dY)
-            net.step(lr)
+            net.step(lr_ep)
             tot += float(net.loss.numpy()[0])
             nb += 1
         hist.append(tot / max(nb, 1))
@@ -426,6 +430,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--eps-orders", type=int, default=ClosureConfig.eps_orders,
+                    help="permittivity Fourier coefficients per axis in the input")
+    ap.add_argument("--schedule", choices=["const", "cosine"], default="const")
     ap.add_argument("--out", help="where the trained network is saved")
     ap.add_argument("--time-forward", type=int, default=0, metavar="ROWS",
                     help="only time the shipped network's inference forward pass at ROWS rows (3 + 10 passes)")
@@ -437,7 +444,7 @@ def main(argv=None):
         return
     if not a.out:
         ap.error("--out is required for training")
-    cfg = ClosureConfig()
+    cfg = ClosureConfig(eps_orders=a.eps_orders)
     t0 = time.time()
     data = make_dataset(ctx, cfg, a.count + a.test, seed=a.seed, log=lambda s: print(s, flush=True))
     t_data = time.time() - t0
@@ -447,7 +454,7 @@ def main(argv=None):
     tr = Dataset(X=data.X[:n], T=data.T[:n], x_low=data.x_low[:n], x_high=data.x_high[:n])
     t0 = time.time()
     net = train(ctx, cfg, tr, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=0,
-                log=lambda s: print(s, flush=True))
+                log=lambda s: print(s, flush=True), schedule=a.schedule)
     t_train = time.time() - t0
     te = slice(n, n + a.test)
     pred = net.predict(ctx, data.X[te], data.x_low[te])
@@ -456,6 +463,7 @@ def main(argv=None):
     e_base = rel_err_rows(base, data.x_high[te])
     save_bf16(net, a.out)
     print(json.dumps({"pairs": n, "test": a.test, "data_s": t_data, "train_s": t_train, "epochs": a.epochs,
+                      "eps_orders": cfg.eps_orders, "schedule": a.schedule,
                       "params": sum(int(np.prod(w.shape)) for w in net.weights),
                       "held_out_rel_err_low_order": float(e_base.mean()),
                       "held_out_rel_err_closure": float(e_net.mean()),

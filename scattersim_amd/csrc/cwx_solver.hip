@@ -505,6 +505,103 @@ __global__ void k_build_gdgc(InstArgs ia, int reduced, const zc* __restrict__ V,
   }
 }
 
+// ---- the transfer chain in the reduced mode's block form ----
+// In the reduced mode the mode matrix of a layer is V = M Pi R with M = [[A, A], [B, -B]], A = (a_p) the
+// eigenvectors of P Q in sorted pair order, B = Q A Gamma^-1 (Gamma the down members), Pi the interleave
+// (column 2i + s is block column s n2 + i; s = 0 down, 1 up) and R the diagonal of zgeev's normalisation
+// (k_build_gdgc).  M^-1 = 1/2 [[A^-1, B^-1], [A^-1, -B^-1]], so with X = Vr_g^-1 Vr_{g-1} and
+// Y = Vr_g^-1 Q_g^-1 QA_{g-1} (Vr the eigenvectors in zgeev's order, QA = Q Vr) the transfer matrix is
+//   T_g(2i+s, 2j+t) = Dd_g^-1 R_g^-1 (2i+s) . 1/2 (X(p_i, q_j) +- gam_g(2i) Y(p_i, q_j) / gam_{g-1}(2j))
+//                     . R_{g-1} Dc_{g-1} (2j+t)
+// (+ for s = t), with p_i, q_j the pairs of sorted index i in layer g and j in layer g - 1, Dd and Dc the
+// phase diagonals of Gd = V Dd and Gc = V Dc.  That is T_g = Gd_g^-1 Gc_{g-1} (ScatterVolume.cpp:327-334)
+// from two LUs of order 2MF instead of one of order 4MF, to rounding: k_build_gdgc also sets the imaginary
+// part of each column's largest entry to exactly zero.
+__device__ inline zc mode_rot(const PairScale& sc, bool neg, int n2) {
+  zc vk = sc.vk;
+  if (neg && sc.k >= n2) vk = -vk;
+  return conj(vk) * (sc.inv_norm / absz(vk));
+}
+
+__global__ void k_t_block(InstArgs ia, const int* __restrict__ tg, MatSet Xs, MatSet Ys, MatSet Ts,
+                          const zc* __restrict__ gam, const int* __restrict__ src, const PairScale* __restrict__ ps,
+                          const double* __restrict__ dz) {
+  const int t = blockIdx.y, g = tg[t];
+  const int n4 = 4 * ia.MF, n2 = n4 / 2;
+  const double k = ia.k[ia.linst[g]];
+  const zc* __restrict__ X = Xs.at(t);
+  const zc* __restrict__ Y = Ys.at(t);
+  zc* __restrict__ T = Ts.at(t);
+  const zc* gg = gam + (long)g * n4;
+  const zc* gp = gam + (long)(g - 1) * n4;
+  const int* sg = src + (long)g * n4;
+  const int* sp = src + (long)(g - 1) * n4;
+  const PairScale* psg = ps + (long)g * n2;
+  const PairScale* psp = ps + (long)(g - 1) * n2;
+  const double dzg = dz[g], dzp = dz[g - 1];
+  constexpr int kMask = (1 << 30) - 1;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n4 * n4; e += (long)gridDim.x * blockDim.x) {
+    const int row = (int)(e % n4), col = (int)(e / n4);
+    const int i = row >> 1, si = row & 1, j = col >> 1, tj = col & 1;
+    const int p = sg[2 * i] & kMask, q = sp[2 * j] & kMask;
+    const zc x = X[(long)q * n2 + p];
+    const zc y = Y[(long)q * n2 + p] * zdiv(gg[2 * i], gp[2 * j]);
+    zc v = (si == tj) ? x + y : x - y;
+    v = 0.5 * v;
+    // row factor Dd_g^-1 R_g^-1: R = conj(vk) c with |R| = inv_norm, so R^-1 = vk / (|vk| inv_norm)
+    const PairScale& a = psg[p];
+    zc vk = a.vk;
+    if (si && a.k >= n2) vk = -vk;
+    zc rf = vk * (1.0 / (absz(vk) * a.inv_norm));
+    if (si) {
+      const zc gr = gg[row];
+      rf = rf * zexp(mk(-gr.im * k * dzg, gr.re * k * dzg));  // 1 / exp(-i k gamma dz)
+    }
+    // column factor R_{g-1} Dc_{g-1}
+    zc cf = mode_rot(psp[q], tj != 0, n2);
+    if (!tj) {
+      const zc gc = gp[col];
+      cf = cf * zexp(mk(-gc.im * k * dzp, gc.re * k * dzp));  // exp(i k gamma dz)
+    }
+    T[e] = rf * v * cf;
+  }
+}
+
+// beta_0 = Gd_0^-1 r in the block form: with r = [u; w] (2MF each), a1 = Vr^-1 u and a2 = Vr^-1 Q^-1 w already
+// solved, beta_0(2i+s) = Dd^-1 R^-1 (2i+s) . 1/2 (a1(p_i) +- gam(2i) a2(p_i)) (+ for the down member s = 0).
+// Instance blockIdx.y: layer first[i]; a1, a2 (2MF each) and beta_0 (4MF) from the lists.
+__global__ void k_beta0_block(InstArgs ia, const int* __restrict__ first, MatSet A1, MatSet A2, MatSet B0,
+                              const zc* __restrict__ gam, const int* __restrict__ src,
+                              const PairScale* __restrict__ ps, const double* __restrict__ dz) {
+  const int ii = blockIdx.y, g = first[ii];
+  const int n4 = 4 * ia.MF, n2 = n4 / 2;
+  const double k = ia.k[ia.linst[g]];
+  const zc* a1 = A1.at(ii);
+  const zc* a2 = A2.at(ii);
+  zc* b0 = B0.at(ii);
+  const zc* gg = gam + (long)g * n4;
+  const int* sg = src + (long)g * n4;
+  const PairScale* psg = ps + (long)g * n2;
+  const double dzg = dz[g];
+  constexpr int kMask = (1 << 30) - 1;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n4; row += gridDim.x * blockDim.x) {
+    const int i = row >> 1, si = row & 1;
+    const int p = sg[2 * i] & kMask;
+    const zc y = gg[2 * i] * a2[p];
+    zc v = si ? a1[p] - y : a1[p] + y;
+    v = 0.5 * v;
+    const PairScale& a = psg[p];
+    zc vk = a.vk;
+    if (si && a.k >= n2) vk = -vk;
+    zc rf = vk * (1.0 / (absz(vk) * a.inv_norm));
+    if (si) {
+      const zc gr = gg[row];
+      rf = rf * zexp(mk(-gr.im * k * dzg, gr.re * k * dzg));
+    }
+    b0[row] = rf * v;
+  }
+}
+
 // count[b] = number of non-finite entries of x[b] (n entries per instance); one workgroup per instance
 __global__ __launch_bounds__(256) void k_count_nonfinite(int n, const zc* __restrict__ x, int* __restrict__ count) {
   __shared__ int s_c;
@@ -1184,6 +1281,11 @@ struct SetBufs {
   std::vector<Level> levels, bsteps;
   size_t oTLU = 0, oTP = 0, oTB = 0, oCS = 0, oCD = 0, oGd0 = 0, oLU0 = 0, oPiv0 = 0, oB0 = 0;
   int nT = 0, ncS = 0;
+  // the chain's block form (reduced mode): lists over the T items and over the instances' first layers
+  size_t oKV = 0, oKPV = 0, oKQ = 0, oKPQ = 0, oKX = 0, oKZ = 0, oKVp = 0, oKQAp = 0, oKT = 0;
+  size_t oK0V = 0, oK0PV = 0, oK0Q = 0, oK0PQ = 0, oK0B = 0, oK0A1 = 0, oK0A2 = 0;
+  DevBuf<int> d_tg, d_first;
+  bool used_block = false;
 };
 
 // Host work of a set solve: instance tables, DFT matrices, every buffer and the pointer lists, all on the
@@ -1267,7 +1369,7 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
   wk.ipiv.ensure((size_t)Ltot * n4 + (size_t)count * (n4 + n6));
   wk.w2.ensure((size_t)count * nn);
   wk.w3.ensure((size_t)std::max(1, Ltot) * nn);
-  b.d_info_lu.ensure(Ltot);
+  b.d_info_lu.ensure(2 * (size_t)Ltot);
   b.infoS.ensure(count);
   b.infoA.ensure(count);
   b.d_nonfinite.ensure(count);
@@ -1362,6 +1464,64 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
       b.bsteps.push_back(Level{b.pl.add(Av), b.pl.add(Bv), b.pl.add(Cv), (int)Av.size()});
     }
   }
+  if (b.eig_mode == 1) {
+    // the block form: in wk.w0, X at [0, nT mm) (P's place), Q at [Ltot mm, 2 Ltot mm) factored in place, the
+    // factors of V at [2 Ltot mm, 3 Ltot mm), Y at [3 Ltot mm, ...); pivots of V then Q at the start of ipiv
+    const long mm = (long)n2 * n2;
+    zc* w0 = wk.w0.p;
+    zc* Qb = w0 + (long)Ltot * mm;
+    zc* LUV = w0 + 2L * Ltot * mm;
+    zc* Zb = w0 + 3L * Ltot * mm;
+    int* pivV = piv;
+    int* pivQ = piv + (long)Ltot * n2;
+    std::vector<void*> kV, kPV, kQ, kPQ, kX, kZ, kVp, kQAp, kT;
+    std::vector<int> tg;
+    for (int i = 0; i < count; ++i)
+      for (int l = 1; l < is.L[i]; ++l) {
+        const long g = is.first[i] + l, t = (long)tg.size();
+        kV.push_back(LUV + g * mm);
+        kPV.push_back(pivV + g * n2);
+        kQ.push_back(Qb + g * mm);
+        kPQ.push_back(pivQ + g * n2);
+        kX.push_back(w0 + t * mm);
+        kZ.push_back(Zb + t * mm);
+        kVp.push_back(b.V.p + (g - 1) * mm);
+        kQAp.push_back(b.QA.p + (g - 1) * mm);
+        kT.push_back(Tm + g * nn);
+        tg.push_back((int)g);
+      }
+    b.oKV = b.pl.add(kV);
+    b.oKPV = b.pl.add(kPV);
+    b.oKQ = b.pl.add(kQ);
+    b.oKPQ = b.pl.add(kPQ);
+    b.oKX = b.pl.add(kX);
+    b.oKZ = b.pl.add(kZ);
+    b.oKVp = b.pl.add(kVp);
+    b.oKQAp = b.pl.add(kQAp);
+    b.oKT = b.pl.add(kT);
+    upload(b.d_tg, tg, s);
+    std::vector<void*> k0V, k0PV, k0Q, k0PQ, k0B, k0A1, k0A2;
+    std::vector<int> first;
+    for (int i = 0; i < count; ++i) {
+      const long f = is.first[i];
+      k0V.push_back(LUV + f * mm);
+      k0PV.push_back(pivV + f * n2);
+      k0Q.push_back(Qb + f * mm);
+      k0PQ.push_back(pivQ + f * n2);
+      k0B.push_back(keep_beta ? b.beta->p + f * n4 : nullptr);
+      k0A1.push_back(w0 + (long)i * n4);       // beta_0's right-hand side, then a1 and a2, in X's place
+      k0A2.push_back(w0 + (long)i * n4 + n2);
+      first.push_back((int)f);
+    }
+    b.oK0V = b.pl.add(k0V);
+    b.oK0PV = b.pl.add(k0PV);
+    b.oK0Q = b.pl.add(k0Q);
+    b.oK0PQ = b.pl.add(k0PQ);
+    b.oK0B = b.pl.add(k0B);
+    b.oK0A1 = b.pl.add(k0A1);
+    b.oK0A2 = b.pl.add(k0A2);
+    upload(b.d_first, first, s);
+  }
   b.pl.commit(s);
   CWX_HIP(hipGetLastError());
 }
@@ -1451,7 +1611,11 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   DevBuf<zc>& Gc = *b.Gc;
   hipLaunchKernelGGL(k_build_gdgc, dim3(grid_for(nn), Ltot), dim3(256), 0, s, ia, eig_mode, b.V.p, b.QA.p,
                      b.pscale.p, b.Vf.p, b.d_fslot.p, gam.p, b.d_src.p, is.dz.p, Gd.p, Gc.p);
-  if (!capture) {
+  // the reduced mode's chain in block form, unless layers were redone through the full Phi (their modes lack
+  // the block structure, and that Phi has overwritten P and Q)
+  const bool block = eig_mode == 1 && b.flist.empty();
+  b.used_block = block;
+  if (!capture && !block) {
     b.V.release();
     b.QA.release();
     b.W.release();
@@ -1462,18 +1626,38 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   // ---- 4. LU of every Gd, T_g = Gd_g^-1 Gc_{g-1} (g not first in its instance), and per instance
   //         S = Gc_last T_last ... T_first+1 by a pairwise product tree (ScatterVolume.cpp:327-336) ----
   zc* LUd = wk.w0.p;  // Phi / P,Q no longer needed
-  hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * nn)), dim3(256), 0, s, (long)Ltot * nn, Gd.p, LUd);
   int* piv = wk.ipiv.p;
   int* pivS = piv + (long)Ltot * n4;
   int* pivA = pivS + (long)count * n4;
-  zgetrf_batched(n4, LUd, n4, nn, piv, n4, b.d_info_lu.p, Ltot, s);
   zc* Tm = wk.w1.p;  // T_g at slot g
   zc* Sm = wk.w2.p;
   const PtrLists& pl = b.pl;
-  if (b.nT > 0) {
-    hipLaunchKernelGGL(k_copy, dim3(grid_for((long)(Ltot - 1) * nn)), dim3(256), 0, s, (long)(Ltot - 1) * nn, Gc.p,
-                       Tm + nn);
-    zgetrs_set(n4, n4, pl.mats(b.oTLU), n4, pl.ints(b.oTP), pl.mats(b.oTB), n4, b.nT, s);
+  if (block) {
+    // LU of every layer's eigenvectors Vr and of its Q; X = Vr_g^-1 Vr_{g-1}, Y = Vr_g^-1 Q_g^-1 QA_{g-1}, then T_g
+    zc* Qb = wk.w0.p + (long)Ltot * mm;
+    zc* LUV = wk.w0.p + 2L * Ltot * mm;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * mm)), dim3(256), 0, s, (long)Ltot * mm, b.V.p, LUV);
+    zgetrf_batched(n2, LUV, n2, mm, piv, n2, b.d_info_lu.p, Ltot, s);
+    zgetrf_batched(n2, Qb, n2, mm, piv + (long)Ltot * n2, n2, b.d_info_lu.p + Ltot, Ltot, s);
+    if (b.nT > 0) {
+      hipLaunchKernelGGL(k_copy_set, dim3(grid_for(mm), (unsigned)b.nT), dim3(256), 0, s, mm, pl.mats(b.oKVp),
+                         pl.mats(b.oKX));
+      hipLaunchKernelGGL(k_copy_set, dim3(grid_for(mm), (unsigned)b.nT), dim3(256), 0, s, mm, pl.mats(b.oKQAp),
+                         pl.mats(b.oKZ));
+      zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKX), n2, b.nT, s);
+      zgetrs_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKZ), n2, b.nT, s);
+      zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKZ), n2, b.nT, s);
+      hipLaunchKernelGGL(k_t_block, dim3(grid_for(nn), (unsigned)b.nT), dim3(256), 0, s, ia, b.d_tg.p,
+                         pl.mats(b.oKX), pl.mats(b.oKZ), pl.mats(b.oKT), gam.p, b.d_src.p, b.pscale.p, is.dz.p);
+    }
+  } else {
+    hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * nn)), dim3(256), 0, s, (long)Ltot * nn, Gd.p, LUd);
+    zgetrf_batched(n4, LUd, n4, nn, piv, n4, b.d_info_lu.p, Ltot, s);
+    if (b.nT > 0) {
+      hipLaunchKernelGGL(k_copy, dim3(grid_for((long)(Ltot - 1) * nn)), dim3(256), 0, s, (long)(Ltot - 1) * nn, Gc.p,
+                         Tm + nn);
+      zgetrs_set(n4, n4, pl.mats(b.oTLU), n4, pl.ints(b.oTP), pl.mats(b.oTB), n4, b.nT, s);
+    }
   }
   for (const Level& lv : b.levels)
     zgemm_set(OpN, OpN, n4, n4, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
@@ -1529,7 +1713,18 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   // ---- 6. beta recurrence, Q_check / Q_hat, Ez rows ----
   if (b.keep_beta) {
     hipLaunchKernelGGL(k_beta0_rhs, dim3(grid_for(n4), count), dim3(256), 0, s, ia, xp, pl.mats(b.oB0));
-    zgetrs_set(n4, 1, pl.mats(b.oLU0), n4, pl.ints(b.oPiv0), pl.mats(b.oB0), n4, count, s);
+    if (block) {
+      // beta_0 = Gd_0^-1 r from the factors of Vr_0 and Q_0
+      hipLaunchKernelGGL(k_copy_set, dim3(grid_for(n4), count), dim3(256), 0, s, (long)n4, pl.mats(b.oK0B),
+                         pl.mats(b.oK0A1));
+      zgetrs_set(n2, 1, pl.mats(b.oK0V), n2, pl.ints(b.oK0PV), pl.mats(b.oK0A1), n2, count, s);
+      zgetrs_set(n2, 1, pl.mats(b.oK0Q), n2, pl.ints(b.oK0PQ), pl.mats(b.oK0A2), n2, count, s);
+      zgetrs_set(n2, 1, pl.mats(b.oK0V), n2, pl.ints(b.oK0PV), pl.mats(b.oK0A2), n2, count, s);
+      hipLaunchKernelGGL(k_beta0_block, dim3(grid_for(n4), count), dim3(256), 0, s, ia, b.d_first.p,
+                         pl.mats(b.oK0A1), pl.mats(b.oK0A2), pl.mats(b.oK0B), gam.p, b.d_src.p, b.pscale.p, is.dz.p);
+    } else {
+      zgetrs_set(n4, 1, pl.mats(b.oLU0), n4, pl.ints(b.oPiv0), pl.mats(b.oB0), n4, count, s);
+    }
     for (const Level& lv : b.bsteps)
       zgemm_set(OpN, OpN, n4, 1, n4, mk(1.0), pl.mats(lv.oA), n4, pl.mats(lv.oB), n4, mk(0.0), pl.mats(lv.oC), n4,
                 lv.batch, s);
@@ -1549,9 +1744,11 @@ void set_status(Context& ctx, SetBufs& b, const std::vector<VolumeOptions>& opts
   const InstSet& is = b.is;
   const int count = b.count, Ltot = b.Ltot, n6 = b.n6;
   if (b.full) finish_solution(*b.full, opts[0], is, b.F.p + (long)Ltot * b.MF, s);
-  std::vector<int> hinfo(Ltot), hlu(Ltot), hS(count), hA(count), hnf(count), hun(Ltot, 0);
+  // hlu: the LU of every Gd, or in the block form that of every Vr (then every Q behind it)
+  const int nlu = b.used_block ? 2 * Ltot : Ltot;
+  std::vector<int> hinfo(Ltot), hlu(nlu), hS(count), hA(count), hnf(count), hun(Ltot, 0);
   CWX_HIP(hipMemcpyAsync(hinfo.data(), b.d_info.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
-  CWX_HIP(hipMemcpyAsync(hlu.data(), b.d_info_lu.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
+  CWX_HIP(hipMemcpyAsync(hlu.data(), b.d_info_lu.p, sizeof(int) * nlu, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipMemcpyAsync(hS.data(), b.infoS.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipMemcpyAsync(hA.data(), b.infoA.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   CWX_HIP(hipMemcpyAsync(hnf.data(), b.d_nonfinite.p, sizeof(int) * count, hipMemcpyDeviceToHost, s));
@@ -1583,7 +1780,11 @@ void set_status(Context& ctx, SetBufs& b, const std::vector<VolumeOptions>& opts
       const int g = is.first[i] + l;
       if (hlu[g] != 0) {
         inf = hlu[g];
-        m = "layer " + std::to_string(l) + ": singular Gd (zero pivot " + std::to_string(hlu[g]) + ")";
+        m = "layer " + std::to_string(l) + (b.used_block ? ": singular mode matrix" : ": singular Gd") +
+            " (zero pivot " + std::to_string(hlu[g]) + ")";
+      } else if (b.used_block && hlu[Ltot + g] != 0) {
+        inf = hlu[Ltot + g];
+        m = "layer " + std::to_string(l) + ": singular Q block (zero pivot " + std::to_string(hlu[Ltot + g]) + ")";
       }
     }
     if (!inf && hS[i]) { inf = hS[i]; m = "singular transfer product"; }
