@@ -56,9 +56,11 @@ const char* cwx_last_error(void);
  *   "qr_threads"       256 or 512 threads per QR workgroup when they do
  *   "batch_chunk_mib"  device memory of one chunk of cwx_solve_batch_dev when max_bytes is 0
  *   "zgemm_kernel"     the FP64 complex GEMM: 1 (built-in) the matrix-core kernel with 64x64 tiles, 2 and 3
- *                      its 128x64 and 64x128 tilings, 0 the vector kernel.  The only key that changes
- *                      results, in the last bits (each kernel sums in its own order); a process that
- *                      compares bits across calls keeps one setting
+ *                      its 128x64 and 64x128 tilings, 0 the vector kernel
+ *   "chain_block"      the reduced eigen mode's transfer chain: 1 (built-in) in block form from the order-2MF
+ *                      eigenvectors and Q, 0 through LUs of the order-4MF Gd (the reference's formulation)
+ * zgemm_kernel and chain_block are the keys that change results, in the last bits (each form sums in its
+ * own order); a process that compares bits across calls keeps one setting.
  * cwx_tuning_table returns the path of the table in use ("" if none).  Host-only calls: no device work. */
 #define CWX_PARAM_TABLE INT64_MIN
 int cwx_set_param(const char* name, int64_t value);

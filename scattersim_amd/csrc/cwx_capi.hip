@@ -139,6 +139,9 @@ int cwx_set_param(const char* name, int64_t value) {
       CWX_CHECK(unset || value == -1 || value == 256 || value == 512, "qr_threads takes 256 or 512");
     } else if (k == "batch_chunk_mib") {
       CWX_CHECK(unset || value == -1 || value >= 64, "batch_chunk_mib takes at least 64");
+    } else if (k == "chain_block") {
+      CWX_CHECK(unset || (value >= -1 && value <= 1),
+                "chain_block takes -1 (built-in choice), 0 (the order-4MF chain) or 1 (the reduced mode's block form)");
     } else if (k == "zgemm_kernel") {
       CWX_CHECK(unset || (value >= -1 && value <= 3),
                 "zgemm_kernel takes -1 (built-in choice), 0 (vector) or 1-3 (matrix cores, 64x64, 128x64, 64x128)");

@@ -1613,7 +1613,7 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
                      b.pscale.p, b.Vf.p, b.d_fslot.p, gam.p, b.d_src.p, is.dz.p, Gd.p, Gc.p);
   // the reduced mode's chain in block form, unless layers were redone through the full Phi (their modes lack
   // the block structure, and that Phi has overwritten P and Q)
-  const bool block = eig_mode == 1 && b.flist.empty();
+  const bool block = eig_mode == 1 && b.flist.empty() && tune_lookup("chain_block", n4, Ltot, 1) != 0;
   b.used_block = block;
   if (!capture && !block) {
     b.V.release();

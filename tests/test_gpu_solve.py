@@ -83,6 +83,26 @@ def test_three_layer_oblique(ctx, eig_mode):
     compare(ctx, sample, p, eig_mode=eig_mode)
 
 
+def test_reduced_chain_block_form_matches_order_4mf_chain(ctx):
+    """The reduced mode's transfer chain in block form (LUs of the order-2MF eigenvectors and Q, T_g assembled
+    from their products) against the same solve through LUs of the order-4MF Gd (chain_block 0, the
+    reference's formulation, ScatterVolume.cpp:327-334) and against the oracle: eight layers at oblique
+    incidence, interior betas included (the fields of the .cw records inside every layer)."""
+    sample = disks_sample(16, 8, 4.0, 4, 1.0, lambda r: 1.3 + 0.3 * r.random() + 0.02j * r.random(), seed=21)
+    p = o.VolumeParams(lam=0.6, direction=(0.25, -0.1, 1.0), ex=(1, 0.3), ey=(0.1, 0.2), size=(4.0, 4.0, 2.0),
+                       coef=(7, 7), n=(1.0, 1.2))
+    L = sa.lib()
+    try:
+        sa.check(L.cwx_set_param(b"chain_block", 0))
+        s4, _ = compare(ctx, sample, p, eig_mode=1)
+        x4 = s4.x.copy()
+        sa.check(L.cwx_set_param(b"chain_block", 1))
+        blk, _ = compare(ctx, sample, p, eig_mode=1)
+    finally:
+        sa.check(L.cwx_set_param(b"chain_block", sa._capi.CWX_PARAM_TABLE))
+    assert rel_err(blk.x, x4) < 1e-11
+
+
 def test_external_and_npz(ctx):
     sample = disks_sample(12, 2, 3.0, 3, 1.0, 1.5, seed=9)
     p = o.VolumeParams(lam=0.8, size=(3.0, 3.0, 1.0), coef=(3, 3), external=True)
