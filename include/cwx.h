@@ -59,8 +59,14 @@ const char* cwx_last_error(void);
  *                      its 128x64 and 64x128 tilings, 0 the vector kernel
  *   "chain_block"      the reduced eigen mode's transfer chain: 1 (built-in) in block form from the order-2MF
  *                      eigenvectors and Q, 0 through LUs of the order-4MF Gd (the reference's formulation)
- * zgemm_kernel and chain_block are the keys that change results, in the last bits (each form sums in its
- * own order); a process that compares bits across calls keeps one setting.
+ *   "lu_panel"         zgetrf's panel kernel: 2 (built-in) whole rows per lane, 1 one column pair per
+ *                      wavefront (both: 32-column panels up to 512 rows, 16 columns up to 1024), 0 rows per
+ *                      lane pair (32 columns); the same pivots
+ *   "getrs_inv"        the solves with many right-hand sides in the transfer chain and for S^-1 f3: 1
+ *                      (built-in) with the inverses of the factors' 64 x 64 diagonal blocks applied as
+ *                      matrix-core products, 0 by substitution in 32 x 32 diagonal blocks
+ * zgemm_kernel, chain_block, lu_panel and getrs_inv are the keys that change results, in the last bits (each
+ * form sums in its own order); a process that compares bits across calls keeps one setting.
  * cwx_tuning_table returns the path of the table in use ("" if none).  Host-only calls: no device work. */
 #define CWX_PARAM_TABLE INT64_MIN
 int cwx_set_param(const char* name, int64_t value);

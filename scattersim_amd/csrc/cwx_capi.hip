@@ -142,6 +142,13 @@ int cwx_set_param(const char* name, int64_t value) {
     } else if (k == "chain_block") {
       CWX_CHECK(unset || (value >= -1 && value <= 1),
                 "chain_block takes -1 (built-in choice), 0 (the order-4MF chain) or 1 (the reduced mode's block form)");
+    } else if (k == "lu_panel") {
+      CWX_CHECK(unset || (value >= -1 && value <= 2),
+                "lu_panel takes -1 (built-in choice), 0 (rows per lane pair), 1 (a column pair per wavefront) or 2 "
+                "(whole rows per lane)");
+    } else if (k == "getrs_inv") {
+      CWX_CHECK(unset || (value >= -1 && value <= 1),
+                "getrs_inv takes -1 (built-in choice), 0 (substitution by diagonal blocks) or 1 (their inverses)");
     } else if (k == "zgemm_kernel") {
       CWX_CHECK(unset || (value >= -1 && value <= 3),
                 "zgemm_kernel takes -1 (built-in choice), 0 (vector) or 1-3 (matrix cores, 64x64, 128x64, 64x128)");

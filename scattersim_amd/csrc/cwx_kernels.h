@@ -93,6 +93,17 @@ inline void zgetrf_batched(int n, zc* A, int lda, long sA, int* ipiv, long sP, i
   zgetrf_set(n, strided(A, sA), lda, strided(ipiv, sP), info, batch, s);
 }
 
+// Inverses of the 64 x 64 diagonal blocks of the factors of zgetrf: trtri_diag_size(n) elements per matrix
+// (block j of L^-1, then block j of U^-1 after all of L's, each 64 x 64 column-major).
+int trtri_diag_blocks(int n);
+long trtri_diag_size(int n);
+void ztrtri_diag_set(int n, MatSet LU, int lda, MatSet Dinv, int batch, hipStream_t s);
+// Solve A X = B as zgetrs does, with the diagonal blocks applied as products by their inverses (Dinv from
+// ztrtri_diag_set) on the matrix cores; Y (n x nrhs per matrix, leading dimension ldy) is scratch.  The
+// rounding differs from zgetrs in the last bits.
+void zgetrs_inv_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet Dinv, MatSet B, int ldb, MatSet Y,
+                    int ldy, int batch, hipStream_t s);
+
 // Solve A X = B with the factors of zgetrf (LAPACK zgetrs, no transpose).
 void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int ldb, int batch, hipStream_t s);
 inline void zgetrs_batched(int n, int nrhs, const zc* LU, int lda, long sA, const int* ipiv, long sP, zc* B,

@@ -12,6 +12,33 @@ namespace cwx {
 
 namespace {
 
+// Pointers into global memory: loads and stores through them are global_* instructions (flat ones count
+// against the LDS wait counter too, so an LDS wait would also wait for every load in flight).
+typedef __attribute__((address_space(1))) const zc gzc;
+typedef __attribute__((address_space(1))) zc gzw;
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) dv2 gdv2;
+// An element of a global-memory view: reads and writes are 16-byte global vector accesses.
+struct GRef {
+  gzw* p;
+  __device__ operator zc() const {
+    const dv2 v = *(const gdv2*)p;
+    return mk(v.x, v.y);
+  }
+  __device__ GRef& operator=(zc v) {
+    *(gdv2*)p = dv2{v.re, v.im};
+    return *this;
+  }
+  __device__ GRef& operator=(const GRef& o) { return *this = (zc)o; }
+};
+// A complex array in global memory, indexed like a pointer.
+struct GP {
+  gzw* p;
+  __device__ GRef operator[](long i) const { return GRef{p + i}; }
+  __device__ GP operator+(long i) const { return GP{p + i}; }
+};
+__device__ inline GP gview(const zc* p) { return GP{(gzw*)p}; }
+
 // ------------------------------------------------------------------------------------------------
 // GEMM: 64x64 output tile per 256-lane workgroup, K staged through LDS in steps of 16, each lane
 // owning a 4x4 block of outputs at stride 16 (conflict-free broadcast reads from LDS).
@@ -95,49 +122,63 @@ __global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int
 }
 
 // ------------------------------------------------------------------------------------------------
-// Matrix-vector products (n <= 4 right-hand sides, op(A) = A): 64 rows per workgroup, the four
-// wavefronts take interleaved quarters of K and combine their partial sums in LDS (a fixed order, so
-// results do not depend on scheduling).
+// Matrix-vector products (n <= 4 right-hand sides, op(A) = A): 64 rows per workgroup, sixteen
+// wavefronts take the columns c = w (mod 16), four columns in flight per wavefront, and the partial
+// sums are combined in LDS in a fixed order (results do not depend on scheduling or on the batch).
+// A 900-row matrix thus keeps 15 workgroups of 16 wavefronts busy instead of 15 of four, whose
+// one-column-at-a-time loads were latency-bound.
 // ------------------------------------------------------------------------------------------------
-constexpr int GV_MAXN = 4;
+constexpr int GV_MAXN = 4, GV_WAVES = 16, GV_UNROLL = 4;
 
-__global__ __launch_bounds__(256) void k_zgemv(int m, int n, int k, zc alpha, MatSet Aset, int lda, MatSet Bset,
-                                               int ldb, zc beta, MatSet Cset, int ldc) {
-  __shared__ zc part[4][GV_MAXN][64];
+__global__ __launch_bounds__(64 * GV_WAVES) void k_zgemv(int m, int n, int k, zc alpha, MatSet Aset, int lda,
+                                                         MatSet Bset, int ldb, zc beta, MatSet Cset, int ldc) {
+  __shared__ zc part[GV_WAVES][64];
   const int b = blockIdx.y;
-  const zc* __restrict__ A = Aset.at(b);
-  const zc* __restrict__ B = Bset.at(b);
+  const GP A = gview(Aset.at(b));
+  const GP B = gview(Bset.at(b));
   zc* __restrict__ C = Cset.at(b);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = blockIdx.x * 64 + lane;
-  zc acc[GV_MAXN][2];
+  zc acc[GV_MAXN][GV_UNROLL];
 #pragma unroll
-  for (int j = 0; j < GV_MAXN; ++j) acc[j][0] = acc[j][1] = mk(0.0);
+  for (int j = 0; j < GV_MAXN; ++j)
+#pragma unroll
+    for (int u = 0; u < GV_UNROLL; ++u) acc[j][u] = mk(0.0);
   if (r < m) {
+    constexpr int step = GV_WAVES * GV_UNROLL;
     int c = wv;
-    for (; c + 4 < k; c += 8) {
-      const zc a0 = A[(long)c * lda + r], a1 = A[(long)(c + 4) * lda + r];
+    for (; c + (GV_UNROLL - 1) * GV_WAVES < k; c += step) {
+      zc a[GV_UNROLL];
+#pragma unroll
+      for (int u = 0; u < GV_UNROLL; ++u) a[u] = A[(long)(c + u * GV_WAVES) * lda + r];
 #pragma unroll
       for (int j = 0; j < GV_MAXN; ++j)
         if (j < n) {
-          fma_acc(acc[j][0], a0, B[(long)j * ldb + c]);
-          fma_acc(acc[j][1], a1, B[(long)j * ldb + c + 4]);
+#pragma unroll
+          for (int u = 0; u < GV_UNROLL; ++u) fma_acc(acc[j][u], a[u], B[(long)j * ldb + c + u * GV_WAVES]);
         }
     }
-    for (; c < k; c += 4) {
+    for (; c < k; c += GV_WAVES) {
       const zc a0 = A[(long)c * lda + r];
 #pragma unroll
       for (int j = 0; j < GV_MAXN; ++j)
         if (j < n) fma_acc(acc[j][0], a0, B[(long)j * ldb + c]);
     }
   }
+  const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
 #pragma unroll
-  for (int j = 0; j < GV_MAXN; ++j) part[wv][j][lane] = acc[j][0] + acc[j][1];
-  __syncthreads();
-  if (wv == 0 && r < m) {
-    const bool beta0 = (beta.re == 0.0 && beta.im == 0.0);
-    for (int j = 0; j < n; ++j) {
-      const zc sum = (part[0][j][lane] + part[1][j][lane]) + (part[2][j][lane] + part[3][j][lane]);
+  for (int j = 0; j < GV_MAXN; ++j) {
+    if (j >= n) break;
+    if (j > 0) __syncthreads();  // the previous column's partials have been read
+    part[wv][lane] = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
+    __syncthreads();
+    if (wv == 0 && r < m) {
+      zc t[GV_WAVES / 2];
+#pragma unroll
+      for (int w = 0; w < GV_WAVES / 2; ++w) t[w] = part[2 * w][lane] + part[2 * w + 1][lane];
+#pragma unroll
+      for (int w = 0; w < GV_WAVES / 4; ++w) t[w] = t[2 * w] + t[2 * w + 1];
+      const zc sum = (t[0] + t[1]) + (t[2] + t[3]);
       zc* cp = C + (long)j * ldc + r;
       zc v = alpha * sum;
       if (!beta0) v = v + beta * (*cp);
@@ -162,8 +203,6 @@ constexpr int MK = 16;
 
 // Operands are read through global-address-space pointers: a flat load would also count against the LDS
 // wait counter, so every LDS read's wait would wait for the prefetched panel too.
-typedef __attribute__((address_space(1))) const zc gzc;
-typedef double dv2 __attribute__((ext_vector_type(2)));
 __device__ inline dv2 gload2(gzc* p) { return *(const __attribute__((address_space(1))) dv2*)(p); }
 
 template <int TM, int TN>
@@ -776,6 +815,298 @@ __global__ __launch_bounds__(LP_THREADS) void k_lu_panel_reg(int n, int j0, int 
   if (nb > 16) lp_gather_swaps(A, lda, j0, spiv, 16, nb, 0, 16, rowmap, R);
 }
 
+// ------------------------------------------------------------------------------------------------
+// LU panel with one column pair per wavefront (same pivots as k_lu_panel: largest |re| + |im| of the
+// interchanged column, ties to the first, as izamax).  Wavefront w holds panel columns 2w and 2w + 1 of
+// every row: lane l, register i holds physical row l + 64 i, so once a column pair is factored its
+// wavefront has no more work and the later columns' updates are not predicated per lane.  Rows are not
+// moved: each carries its logical position (the row order after the interchanges so far); a pivot is
+// chosen among the rows whose position is not yet taken, an interchange swaps two positions, and every
+// row is stored at its final position at the end.  Per column: the owning wavefront finds the pivot
+// (cross-lane reductions, no barrier), scales the column and publishes the multipliers in LDS
+// (two buffers, used alternately); one barrier; every wavefront with later columns reads them and
+// updates its columns.  NR registers of rows cover panels of up to 64 NR rows; 2 WAVES columns at most.
+// The update does not skip the rows above the pivot (their multipliers are zero): with a pivot row that
+// is not finite they become NaN too, where LAPACK leaves them (the factors are not finite either way).
+// ------------------------------------------------------------------------------------------------
+// Lane src's value in every lane (src uniform): scalar reads, no LDS round trip.
+__device__ inline zc shfl_z(zc v, int src) {
+  const double re = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.re), src),
+                                     __builtin_amdgcn_readlane(__double2loint(v.re), src));
+  const double im = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.im), src),
+                                     __builtin_amdgcn_readlane(__double2loint(v.im), src));
+  return mk(re, im);
+}
+
+// (value, position) maximum over the wavefront, ties to the smaller position; every lane gets the result.
+// Rotations by 1, 2, 4, 8 inside each row of 16 lanes (DPP row_ror), then across rows by lane exchanges.
+__device__ inline void wave_argmax(double& v, int& idx) {
+  auto ror = [](double x, int sel) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    unsigned lo, hi;
+    switch (sel) {
+      case 1: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x121, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x121, 0xF, 0xF, false); break;
+      case 2: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x122, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x122, 0xF, 0xF, false); break;
+      case 4: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x124, 0xF, 0xF, false);
+              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x124, 0xF, 0xF, false); break;
+      default: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x128, 0xF, 0xF, false);
+               hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false); break;
+    }
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  };
+  auto rori = [](int x, int sel) {
+    switch (sel) {
+      case 1: return __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false);
+      case 2: return __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false);
+      case 4: return __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);
+      default: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+    }
+  };
+#pragma unroll
+  for (int sh = 1; sh <= 8; sh <<= 1) {
+    const double ov = ror(v, sh);
+    const int oi = rori(idx, sh);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+#pragma unroll
+  for (int sh = 16; sh < 64; sh <<= 1) {
+    const double ov = __shfl_xor(v, sh);
+    const int oi = __shfl_xor(idx, sh);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+}
+
+// 1 / p from the hardware reciprocal estimate and two Newton steps, p scaled by a power of two so that
+// |p|^2 neither overflows nor underflows (within a few ulps of the division; p = 0 gives infinities).
+__device__ inline zc zrcp_scaled(zc p) {
+  const int e = ilogb(fmax(fabs(p.re), fabs(p.im)));
+  const double a = ldexp(p.re, -e), b = ldexp(p.im, -e);
+  const double d = fma(a, a, b * b);
+  double r = __builtin_amdgcn_rcp(d);
+  double t = fma(-d, r, 1.0);
+  r = fma(r, t, r);
+  t = fma(-d, r, 1.0);
+  r = fma(r, t, r);
+  return mk(ldexp(a * r, -e), ldexp(-b * r, -e));
+}
+
+// c ? a : b by components (a select of the structure itself would keep x out of registers)
+__device__ inline zc zsel(bool c, zc a, zc b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
+
+// The owning wavefront's part of column jj (its column CO): pivot, multipliers into LDS and into x.
+// Row loops are branch-free (selects), so that their loads and stores are issued together.
+template <int CO, int NR>
+__device__ inline void cw_owner_step(zc (&x)[NR][2], const int (&lp)[NR], int jj, int lane, zc* Lc, int* s_pr,
+                                     int* ipiv, int j0, int* info, int b) {
+  double best = -1.0;
+  int bl = 0x7fffffff;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const double a = (lp[i] >= jj) ? abs1(x[i][CO]) : -1.0;
+    const int li = (lp[i] >= jj) ? lp[i] : 0x7fffffff;
+    const bool take = a > best || (a == best && li < bl);
+    best = take ? a : best;
+    bl = take ? li : bl;
+  }
+  wave_argmax(best, bl);
+  const int pr = bl;
+  zc P = mk(0.0);
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const unsigned long long m = __ballot(lp[i] == pr);
+    if (m) P = shfl_z(x[i][CO], __ffsll((long long)m) - 1);
+  }
+  const zc rinv = (best != 0.0) ? zrcp_scaled(P) : mk(0.0);
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const bool below = lp[i] >= jj && lp[i] != pr;
+    const zc l = x[i][CO] * rinv;
+    Lc[lane + 64 * i] = zsel(below, l, mk(0.0));
+    x[i][CO] = zsel(below, l, x[i][CO]);
+  }
+  if (lane == 0) {
+    *s_pr = pr;
+    ipiv[j0 + jj] = j0 + pr;
+    if (best == 0.0 && info[b] == 0) info[b] = j0 + jj + 1;
+  }
+}
+
+template <int NR, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_lu_panel_cw(int n, int j0, int nb, MatSet Aset, int lda, IntSet pset,
+                                                            int* info) {
+  __shared__ zc Lc[2][64 * NR];
+  __shared__ int s_pr[2];
+  const int b = blockIdx.x;
+  const GP A = gview(Aset.at(b));
+  int* ipiv = pset.at(b);
+  const int R = n - j0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ca = 2 * w, cb = 2 * w + 1;  // this wavefront's panel columns
+  zc x[NR][2];
+  int lp[NR];  // logical position of each register's row; -1 past the panel's end
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const int r = lane + 64 * i;
+    const bool ok = r < R;
+    lp[i] = ok ? r : -1;
+    x[i][0] = (ok && ca < nb) ? A[(long)(j0 + ca) * lda + j0 + r] : mk(0.0);
+    x[i][1] = (ok && cb < nb) ? A[(long)(j0 + cb) * lda + j0 + r] : mk(0.0);
+  }
+#pragma unroll 1
+  for (int jj = 0; jj < nb; ++jj) {
+    const int buf = jj & 1;
+    if (w == (jj >> 1)) {
+      if (jj & 1)
+        cw_owner_step<1, NR>(x, lp, jj, lane, Lc[buf], &s_pr[buf], ipiv, j0, info, b);
+      else
+        cw_owner_step<0, NR>(x, lp, jj, lane, Lc[buf], &s_pr[buf], ipiv, j0, info, b);
+    }
+    __syncthreads();
+    const int pr = s_pr[buf];
+    const bool liveA = ca > jj && ca < nb, liveB = cb > jj && cb < nb;
+    if (liveA || liveB) {
+      // the pivot row's entries in this wavefront's columns, then the rank-1 update below it
+      zc u0 = mk(0.0), u1 = mk(0.0);
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const unsigned long long m = __ballot(lp[i] == pr);
+        if (m) {
+          const int src = __ffsll((long long)m) - 1;
+          u0 = shfl_z(x[i][0], src);
+          u1 = shfl_z(x[i][1], src);
+        }
+      }
+      // The multipliers are zero in the rows that are not below the pivot (the owner wrote zeros there), so
+      // every row takes x - l u: unchanged in value in those rows while the pivot row is finite.  Multipliers
+      // are read four registers at a time.
+#pragma unroll
+      for (int i0 = 0; i0 < NR; i0 += 4) {
+        zc l[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) l[k] = Lc[buf][lane + 64 * (i0 + k)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (liveA) x[i0 + k][0] = x[i0 + k][0] - l[k] * u0;
+          if (liveB) x[i0 + k][1] = x[i0 + k][1] - l[k] * u1;
+        }
+      }
+    }
+    // positions jj and pr are interchanged
+#pragma unroll
+    for (int i = 0; i < NR; ++i) lp[i] = (lp[i] == jj) ? pr : ((lp[i] == pr) ? jj : lp[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < NR; ++i)
+    if (lp[i] >= 0) {
+      if (ca < nb) A[(long)(j0 + ca) * lda + j0 + lp[i]] = x[i][0];
+      if (cb < nb) A[(long)(j0 + cb) * lda + j0 + lp[i]] = x[i][1];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// LU panel with whole rows in registers (same pivots as k_lu_panel).  Lane t holds panel rows t + NT q
+// (q < NR), all NB columns, and the column loop is unrolled, so every column index is static: a column's
+// update touches only the columns after it, with no per-column predication.  Rows are not moved: each
+// carries its logical position (LAPACK's row order after the interchanges so far), a pivot is chosen
+// among the rows whose position is not yet taken (largest |re| + |im|, ties to the smaller position, as
+// izamax on the interchanged column), an interchange swaps two positions, and every row is stored at its
+// final position at the end.  Per column: a wavefront reduction and a reduction over the wavefronts'
+// results in LDS (one barrier), then the pivot row's lane publishes the row (second barrier), and every
+// row below it takes its multiplier and updates its later columns.  LDS areas alternate between columns,
+// so two barriers per column suffice.  The update does not skip the rows above the pivot (their
+// multipliers are zero): with a pivot row that is not finite they become NaN too, where LAPACK leaves
+// them (the factors are not finite either way).
+// ------------------------------------------------------------------------------------------------
+template <int NT, int NR, int NB>
+__global__ __launch_bounds__(NT) void k_lu_panel_rw(int n, int j0, int nb, MatSet Aset, int lda, IntSet pset,
+                                                    int* info) {
+  constexpr int NW = NT / 64;
+  __shared__ double s_v[2][NW];
+  __shared__ int s_i[2][NW];
+  __shared__ zc s_row[2][NB];
+  const int b = blockIdx.x;
+  const GP A = gview(Aset.at(b));
+  int* ipiv = pset.at(b);
+  const int R = n - j0, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  zc x[NR][NB];
+  int lp[NR];  // logical position of each row held; -1 past the panel's end
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int r = tid + NT * q;
+    lp[q] = r < R ? r : -1;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[q][c] = (r < R && c < nb) ? A[(long)(j0 + c) * lda + j0 + r] : mk(0.0);
+  }
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj) {
+    if (jj < nb) {
+    const int buf = jj & 1;
+    double a = -1.0;
+    int idx = 0x7fffffff;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double aq = (lp[q] >= jj) ? abs1(x[q][jj]) : -1.0;
+      const int iq = (lp[q] >= jj) ? lp[q] : 0x7fffffff;
+      const bool take = aq > a || (aq == a && iq < idx);
+      a = take ? aq : a;
+      idx = take ? iq : idx;
+    }
+    wave_argmax(a, idx);
+    if (lane == 0) {
+      s_v[buf][w] = a;
+      s_i[buf][w] = idx;
+    }
+    __syncthreads();
+    double best = s_v[buf][0];
+    int pr = s_i[buf][0];
+#pragma unroll
+    for (int k = 1; k < NW; ++k) {
+      const double ov = s_v[buf][k];
+      const int oi = s_i[buf][k];
+      const bool take = ov > best || (ov == best && oi < pr);
+      best = take ? ov : best;
+      pr = take ? oi : pr;
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      if (lp[q] == pr) {
+#pragma unroll
+        for (int c = jj; c < NB; ++c) s_row[buf][c] = x[q][c];
+      }
+    if (tid == 0) {
+      ipiv[j0 + jj] = j0 + pr;
+      if (best == 0.0 && info[b] == 0) info[b] = j0 + jj + 1;
+    }
+    __syncthreads();
+    const zc rinv = (best != 0.0) ? zrcp_scaled(s_row[buf][jj]) : mk(0.0);
+    zc le[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const bool below = lp[q] >= jj && lp[q] != pr;
+      const zc l = x[q][jj] * rinv;
+      le[q] = zsel(below, l, mk(0.0));
+      x[q][jj] = zsel(below, l, x[q][jj]);
+      lp[q] = (lp[q] == jj) ? pr : ((lp[q] == pr) ? jj : lp[q]);
+    }
+#pragma unroll
+    for (int c = jj + 1; c < NB; ++c) {
+      const zc u = s_row[buf][c];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) x[q][c] = x[q][c] - le[q] * u;
+    }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q)
+    if (lp[q] >= 0) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nb) A[(long)(j0 + c) * lda + j0 + lp[q]] = x[q][c];
+    }
+}
+
 // Apply the panel's swaps to the columns outside [j0, j0+nb) and, for columns right of the panel,
 // solve U12 = L11^{-1} A12 (L11 unit lower triangular).  One lane per column.  The interchanges
 // (applied in order) are composed once per workgroup into a gather over the rows they touch: rows
@@ -787,7 +1118,7 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
   __shared__ int trow[2 * LU_NB], tsrc[2 * LU_NB];
   __shared__ int s_nt;
   const int b = blockIdx.y;
-  zc* A = Aset.at(b);
+  const GP A = gview(Aset.at(b));
   const int* ipiv = pset.at(b);
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     int r = e % nb, c = e / nb;
@@ -827,13 +1158,20 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
   // columns [0, j0) and [j0+nb, n)
   if (c >= j0) c += nb;
   if (c >= n) return;
-  zc* col = A + (long)c * lda;
+  const GP col = A + (long)c * lda;
   const int ne = s_nt - nb;  // pivot rows below the panel
   zc x[LU_NB], w[LU_NB];      // the panel's rows, then those pivot rows
+  // every load from a valid row (indices clamped, the extra values dropped by selects), all in flight
+  // together rather than one conditional load after another
 #pragma unroll
   for (int t = 0; t < LU_NB; ++t) {
-    x[t] = (t < nb) ? col[tsrc[t]] : mk(0.0);
-    w[t] = (t < ne) ? col[tsrc[nb + t]] : mk(0.0);
+    x[t] = col[tsrc[min(t, nb - 1)]];
+    w[t] = col[tsrc[ne > 0 ? nb + min(t, ne - 1) : 0]];
+  }
+#pragma unroll
+  for (int t = 0; t < LU_NB; ++t) {
+    x[t] = zsel(t < nb, x[t], mk(0.0));
+    w[t] = zsel(t < ne, w[t], mk(0.0));
   }
 #pragma unroll
   for (int t = 0; t < LU_NB; ++t)
@@ -877,7 +1215,7 @@ __global__ __launch_bounds__(SWP_THREADS) void k_laswp(int n, int nrhs, int ch, 
   int* src = reinterpret_cast<int*>(swp_lds + sizeof(zc) * (size_t)ch * n);
   int* pv = src + n;
   const int b = blockIdx.x;
-  zc* B = Bset.at(b);
+  const GP B = gview(Bset.at(b));
   const int* ipiv = pset.at(b);
   int below = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -923,8 +1261,8 @@ __global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, Ma
                                                    MatSet Bset, int ldb) {
   __shared__ zc T[LU_NB][LU_NB];
   const int b = blockIdx.y;
-  const zc* LU = LUset.at(b);
-  zc* B = Bset.at(b);
+  const GP LU = gview(LUset.at(b));
+  const GP B = gview(Bset.at(b));
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     int r = e % nb, c = e / nb;
     T[r][c] = LU[(long)(j0 + c) * lda + j0 + r];
@@ -932,10 +1270,12 @@ __global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, Ma
   __syncthreads();
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
-  zc* col = B + (long)c * ldb + j0;
+  const GP col = B + (long)c * ldb + j0;
   zc x[LU_NB];
 #pragma unroll
-  for (int r = 0; r < LU_NB; ++r) x[r] = (r < nb) ? col[r] : mk(0.0);
+  for (int r = 0; r < LU_NB; ++r) x[r] = col[min(r, nb - 1)];  // all loads in flight together
+#pragma unroll
+  for (int r = 0; r < LU_NB; ++r) x[r] = zsel(r < nb, x[r], mk(0.0));
   if (lower) {
 #pragma unroll
     for (int r = 0; r < LU_NB; ++r) {
@@ -963,20 +1303,27 @@ __global__ __launch_bounds__(64) void k_trsm_block(int j0, int nb, int lower, Ma
 // diagonal block with its rows in registers (lane r holds row r), then all lanes update the rows still
 // to come with the block's 32 values.  Replaces two launches per block of the many-column path.
 constexpr int GS_THREADS = 512, GS_MAXN = 4000;
-// Lane src's value in every lane (src uniform): scalar reads, no LDS round trip.
-__device__ inline zc shfl_z(zc v, int src) {
-  const double re = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.re), src),
-                                     __builtin_amdgcn_readlane(__double2loint(v.re), src));
-  const double im = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.im), src),
-                                     __builtin_amdgcn_readlane(__double2loint(v.im), src));
-  return mk(re, im);
+
+// sum_q LU(r, j0 + q) xs[j0 + q], q < nb: every load issued before the first is used (addresses clamped to
+// the block, the terms past nb dropped by selects), two partial sums.
+__device__ inline zc gs_dot(const GP LU, int lda, int j0, int nb, int r, const zc* xs) {
+  zc a[LU_NB];
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q) a[q] = LU[(long)(j0 + min(q, nb - 1)) * lda + r];
+  zc acc0 = mk(0.0), acc1 = mk(0.0);
+#pragma unroll
+  for (int q = 0; q < LU_NB; q += 2) {
+    fma_acc(acc0, zsel(q < nb, a[q], mk(0.0)), xs[j0 + min(q, nb - 1)]);
+    fma_acc(acc1, zsel(q + 1 < nb, a[q + 1], mk(0.0)), xs[j0 + min(q + 1, nb - 1)]);
+  }
+  return acc0 + acc1;
 }
 
 __global__ __launch_bounds__(GS_THREADS) void k_getrs1(int n, MatSet LUset, int lda, MatSet Bset) {
   extern __shared__ __align__(16) unsigned char gs_lds[];
   zc* xs = reinterpret_cast<zc*>(gs_lds);
-  const zc* LU = LUset.at(blockIdx.x);
-  zc* B = Bset.at(blockIdx.x);
+  const GP LU = gview(LUset.at(blockIdx.x));
+  const GP B = gview(Bset.at(blockIdx.x));
   const int tid = threadIdx.x, lane = tid & 63;
   for (int r = tid; r < n; r += GS_THREADS) xs[r] = B[r];
   __syncthreads();
@@ -984,9 +1331,14 @@ __global__ __launch_bounds__(GS_THREADS) void k_getrs1(int n, MatSet LUset, int 
   for (int j0 = 0; j0 < n; j0 += LU_NB) {
     const int nb = min(LU_NB, n - j0);
     if (tid < 64) {
+      // loads from rows inside the block for every lane (selected afterwards), so that they are all in
+      // flight together instead of one conditional load after another
+      const int rl = j0 + min(lane, nb - 1);
       zc lrow[LU_NB];
 #pragma unroll
-      for (int q = 0; q < LU_NB; ++q) lrow[q] = (q < lane && lane < nb) ? LU[(long)(j0 + q) * lda + j0 + lane] : mk(0.0);
+      for (int q = 0; q < LU_NB; ++q) lrow[q] = LU[(long)(j0 + min(q, nb - 1)) * lda + rl];
+#pragma unroll
+      for (int q = 0; q < LU_NB; ++q) lrow[q] = zsel(q < lane && lane < nb, lrow[q], mk(0.0));
       zc xr = (lane < nb) ? xs[j0 + lane] : mk(0.0);
 #pragma unroll
       for (int q = 0; q < LU_NB - 1; ++q)
@@ -994,25 +1346,19 @@ __global__ __launch_bounds__(GS_THREADS) void k_getrs1(int n, MatSet LUset, int 
       if (lane < nb) xs[j0 + lane] = xr;
     }
     __syncthreads();
-    for (int r = j0 + nb + tid; r < n; r += GS_THREADS) {
-      // all loads of a row are independent and in flight together; two partial sums
-      zc acc0 = mk(0.0), acc1 = mk(0.0);
-#pragma unroll
-      for (int q = 0; q < LU_NB; q += 2) {
-        if (q < nb) fma_acc(acc0, LU[(long)(j0 + q) * lda + r], xs[j0 + q]);
-        if (q + 1 < nb) fma_acc(acc1, LU[(long)(j0 + q + 1) * lda + r], xs[j0 + q + 1]);
-      }
-      xs[r] = xs[r] - (acc0 + acc1);
-    }
+    for (int r = j0 + nb + tid; r < n; r += GS_THREADS) xs[r] = xs[r] - gs_dot(LU, lda, j0, nb, r, xs);
     __syncthreads();
   }
   // U (upper, with its diagonal), blocks from the bottom
   for (int j1 = n; j1 > 0; j1 -= LU_NB) {
     const int j0 = max(0, j1 - LU_NB), nb = j1 - j0;
     if (tid < 64) {
+      const int rl = j0 + min(lane, nb - 1);
       zc urow[LU_NB];
 #pragma unroll
-      for (int q = 0; q < LU_NB; ++q) urow[q] = (lane <= q && q < nb) ? LU[(long)(j0 + q) * lda + j0 + lane] : mk(0.0);
+      for (int q = 0; q < LU_NB; ++q) urow[q] = LU[(long)(j0 + min(q, nb - 1)) * lda + rl];
+#pragma unroll
+      for (int q = 0; q < LU_NB; ++q) urow[q] = zsel(lane <= q && q < nb, urow[q], mk(0.0));
       zc xr = (lane < nb) ? xs[j0 + lane] : mk(0.0);
 #pragma unroll
       for (int q = LU_NB - 1; q >= 0; --q) {
@@ -1026,13 +1372,7 @@ __global__ __launch_bounds__(GS_THREADS) void k_getrs1(int n, MatSet LUset, int 
     }
     __syncthreads();
     for (int r = tid; r < j0; r += GS_THREADS) {
-      zc acc0 = mk(0.0), acc1 = mk(0.0);
-#pragma unroll
-      for (int q = 0; q < LU_NB; q += 2) {
-        if (q < nb) fma_acc(acc0, LU[(long)(j0 + q) * lda + r], xs[j0 + q]);
-        if (q + 1 < nb) fma_acc(acc1, LU[(long)(j0 + q + 1) * lda + r], xs[j0 + q + 1]);
-      }
-      xs[r] = xs[r] - (acc0 + acc1);
+      xs[r] = xs[r] - gs_dot(LU, lda, j0, nb, r, xs);
     }
     __syncthreads();
   }
@@ -1043,6 +1383,70 @@ __global__ void k_fill_int(int n, int* p, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Inverses of the 64 x 64 diagonal blocks of an LU factorisation, for the solves that apply them as
+// matrix products.  Workgroup (j, f, b): block j of matrix b, f = 0 the unit lower factor, f = 1 the
+// upper one.  The block (transposed) and the inverse being built both sit in LDS (2 x 65 KiB, so one
+// workgroup per compute unit); a block cut by the matrix's end is completed with the identity, whose
+// inverse it keeps.  Lane (c, p) of the 256 (c = tid / 4, p = tid % 4) owns rows p, p + 4, ..., p + 60
+// of column c of the inverse: column elimination, one barrier per row.  The output is column-major,
+// leading dimension 64: block j of L^-1 at j 64^2, of U^-1 at (nblk + j) 64^2 of the matrix's area.
+// ------------------------------------------------------------------------------------------------
+constexpr int TI_NB = 64, TI_LD = TI_NB + 1;
+constexpr size_t TI_LDS = 2 * sizeof(zc) * TI_NB * TI_LD;
+
+__global__ __launch_bounds__(256) void k_trtri_diag(int n, MatSet LUset, int lda, MatSet Dset) {
+  extern __shared__ __align__(16) unsigned char ti_lds[];
+  zc* Tc = reinterpret_cast<zc*>(ti_lds);  // Tc[c * TI_LD + r] = block(r, c)
+  zc* Xs = Tc + TI_NB * TI_LD;             // Xs[c * TI_LD + r] = inverse(r, c)
+  const int j = blockIdx.x, upper = blockIdx.y, b = blockIdx.z;
+  const int nblk = (n + TI_NB - 1) / TI_NB;
+  const int j0 = j * TI_NB, nbj = min(TI_NB, n - j0);
+  const GP LU = gview(LUset.at(b));
+  const GP D = gview(Dset.at(b)) + (long)(upper * nblk + j) * TI_NB * TI_NB;
+  for (int e = threadIdx.x; e < TI_NB * TI_NB; e += blockDim.x) {
+    const int r = e & (TI_NB - 1), c = e / TI_NB;
+    zc v = (r == c) ? mk(1.0) : mk(0.0);
+    if (r < nbj && c < nbj) v = LU[(long)(j0 + c) * lda + j0 + r];
+    Tc[c * TI_LD + r] = v;
+    Xs[c * TI_LD + r] = (r == c) ? mk(1.0) : mk(0.0);
+  }
+  __syncthreads();
+  const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
+  zc* xc = Xs + c * TI_LD;
+  if (!upper) {
+    // row q of every column is final after step q - 1: rows r > q take x_r -= L[r][q] x_q
+#pragma unroll 1
+    for (int q = 0; q < TI_NB - 1; ++q) {
+      const zc xq = xc[q];
+      const zc* tq = Tc + q * TI_LD;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = part + 4 * i;
+        if (r > q) xc[r] = xc[r] - tq[r] * xq;
+      }
+      __syncthreads();
+    }
+  } else {
+    // x_q /= U[q][q] by the lane that owns row q, then rows r < q take x_r -= U[r][q] x_q
+#pragma unroll 1
+    for (int q = TI_NB - 1; q >= 0; --q) {
+      const zc* tq = Tc + q * TI_LD;
+      if (part == (q & 3)) xc[q] = zdiv(xc[q], tq[q]);
+      __syncthreads();
+      const zc xq = xc[q];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = part + 4 * i;
+        if (r < q) xc[r] = xc[r] - tq[r] * xq;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) D[(long)c * TI_NB + part + 4 * i] = xc[part + 4 * i];
+}
+
 }  // namespace
 
 void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda, MatSet B, int ldb, zc beta,
@@ -1051,8 +1455,8 @@ void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda,
   CWX_CHECK(batch <= 65535, "zgemm: batch too large");
   CWX_CHECK(k > 0, "zgemm: k must be positive");
   if (n <= GV_MAXN && opa == OpN && opb == OpN) {
-    hipLaunchKernelGGL(k_zgemv, dim3(cdiv(m, 64), batch), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc);
+    hipLaunchKernelGGL(k_zgemv, dim3(cdiv(m, 64), batch), dim3(64 * GV_WAVES), 0, s, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc);
     CWX_HIP(hipGetLastError());
     return;
   }
@@ -1094,9 +1498,25 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
   CWX_CHECK(batch <= 65535, "zgetrf: batch too large");
   // a fill kernel rather than hipMemsetAsync keeps the sequence capturable in a HIP graph
   hipLaunchKernelGGL(k_fill_int, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, info, 0);
-  for (int j0 = 0; j0 < n; j0 += LU_NB) {
-    const int nb = std::min(LU_NB, n - j0);
-    if (n - j0 <= 256)
+  // panels with whole rows per lane (2, the default), one column pair per wavefront (1) or rows per lane
+  // pair (0); the first two keep 32 columns up to 512 rows and 16 columns up to 1024 rows
+  const int kind = (int)tune_lookup("lu_panel", n, 0, 2);
+  for (int j0 = 0, nb = 0; j0 < n; j0 += nb) {
+    const int R = n - j0;
+    nb = std::min((kind != 0 && R > 512 && R <= 1024) ? 16 : LU_NB, R);
+    if (kind == 2 && R <= 256)
+      hipLaunchKernelGGL((k_lu_panel_rw<256, 1, 32>), dim3(batch), dim3(256), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (kind == 2 && R <= 512)
+      hipLaunchKernelGGL((k_lu_panel_rw<512, 1, 32>), dim3(batch), dim3(512), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (kind == 2 && R <= 1024)
+      hipLaunchKernelGGL((k_lu_panel_rw<512, 2, 16>), dim3(batch), dim3(512), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (kind == 1 && R <= 256)
+      hipLaunchKernelGGL((k_lu_panel_cw<4, 16>), dim3(batch), dim3(1024), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (kind == 1 && R <= 512)
+      hipLaunchKernelGGL((k_lu_panel_cw<8, 16>), dim3(batch), dim3(1024), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (kind == 1 && R <= 1024)
+      hipLaunchKernelGGL((k_lu_panel_cw<16, 8>), dim3(batch), dim3(512), 0, s, n, j0, nb, A, lda, ipiv, info);
+    else if (n - j0 <= 256)
       hipLaunchKernelGGL(k_lu_panel_reg<1>, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     else if (n - j0 <= 512)
       hipLaunchKernelGGL(k_lu_panel_reg<2>, dim3(batch), dim3(LP_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
@@ -1116,23 +1536,65 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
   CWX_HIP(hipGetLastError());
 }
 
+// The row interchanges of zgetrf on B (LAPACK zlaswp, forward).
+void laswp_set(int n, int nrhs, MatSet B, int ldb, IntSet ipiv, int batch, hipStream_t s) {
+  // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map (the device's
+  // whole LDS allowance when one column does not fit in 64 KiB)
+  const size_t need = (2 * sizeof(int) + sizeof(zc)) * (size_t)n;
+  const size_t budget = need <= 64 * 1024 ? 64 * 1024 : lds_limit();
+  CWX_CHECK(need <= budget, "zgetrs: n too large for the LDS row map");
+  const size_t avail = budget - 2 * sizeof(int) * (size_t)n;
+  const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
+  const size_t lds = sizeof(zc) * (size_t)ch * n + 2 * sizeof(int) * (size_t)n;
+  // enough workgroups to spread the column chunks over the device (about 2048 in all)
+  const int chunks = (nrhs + ch - 1) / ch;
+  const int gy = std::max(1, std::min({chunks, 65535, 2048 / batch}));
+  hipLaunchKernelGGL(k_laswp, dim3(batch, gy), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
+}
+
+int trtri_diag_blocks(int n) { return (n + TI_NB - 1) / TI_NB; }
+
+long trtri_diag_size(int n) { return 2L * trtri_diag_blocks(n) * TI_NB * TI_NB; }
+
+void ztrtri_diag_set(int n, MatSet LU, int lda, MatSet Dinv, int batch, hipStream_t s) {
+  if (n <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "ztrtri_diag: batch too large");
+  hipLaunchKernelGGL(k_trtri_diag, dim3(trtri_diag_blocks(n), 2, batch), dim3(256), TI_LDS, s, n, LU, lda, Dinv);
+  CWX_HIP(hipGetLastError());
+}
+
+void zgetrs_inv_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet Dinv, MatSet B, int ldb, MatSet Y,
+                    int ldy, int batch, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
+  laswp_set(n, nrhs, B, ldb, ipiv, batch, s);
+  const int nblk = trtri_diag_blocks(n);
+  const long bs = (long)TI_NB * TI_NB;
+  // forward, Y = L^-1 B: Y_j = (L_jj)^-1 B_j, then the rows below take B -= L(:, j) Y_j
+  for (int j = 0; j < nblk; ++j) {
+    const int j0 = j * TI_NB, nb = std::min(TI_NB, n - j0);
+    zgemm_set(OpN, OpN, nb, nrhs, nb, mk(1.0), Dinv.shifted(j * bs), TI_NB, B.shifted(j0), ldb, mk(0.0),
+              Y.shifted(j0), ldy, batch, s);
+    const int rest = n - j0 - nb;
+    if (rest > 0)
+      zgemm_set(OpN, OpN, rest, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda + j0 + nb), lda, Y.shifted(j0), ldy,
+                mk(1.0), B.shifted(j0 + nb), ldb, batch, s);
+  }
+  // backward, X = U^-1 Y into B: X_j = (U_jj)^-1 Y_j, then the rows above take Y -= U(:, j) X_j
+  for (int j = nblk - 1; j >= 0; --j) {
+    const int j0 = j * TI_NB, nb = std::min(TI_NB, n - j0);
+    zgemm_set(OpN, OpN, nb, nrhs, nb, mk(1.0), Dinv.shifted((nblk + j) * bs), TI_NB, Y.shifted(j0), ldy, mk(0.0),
+              B.shifted(j0), ldb, batch, s);
+    if (j0 > 0)
+      zgemm_set(OpN, OpN, j0, nrhs, nb, mk(-1.0), LU.shifted((long)j0 * lda), lda, B.shifted(j0), ldb, mk(1.0), Y,
+                ldy, batch, s);
+  }
+}
+
 void zgetrs_set(int n, int nrhs, MatSet LU, int lda, IntSet ipiv, MatSet B, int ldb, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   CWX_CHECK(batch <= 65535, "zgetrs: batch too large");
-  {
-    // columns staged per chunk: as many as fit 64 KiB of LDS beside the n-entry source map (the device's
-    // whole LDS allowance when one column does not fit in 64 KiB)
-    const size_t need = (2 * sizeof(int) + sizeof(zc)) * (size_t)n;
-    const size_t budget = need <= 64 * 1024 ? 64 * 1024 : lds_limit();
-    CWX_CHECK(need <= budget, "zgetrs: n too large for the LDS row map");
-    const size_t avail = budget - 2 * sizeof(int) * (size_t)n;
-    const int ch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, avail / (sizeof(zc) * (size_t)n)));
-    const size_t lds = sizeof(zc) * (size_t)ch * n + 2 * sizeof(int) * (size_t)n;
-    // enough workgroups to spread the column chunks over the device (about 2048 in all)
-    const int chunks = (nrhs + ch - 1) / ch;
-    const int gy = std::max(1, std::min({chunks, 65535, 2048 / batch}));
-    hipLaunchKernelGGL(k_laswp, dim3(batch, gy), dim3(SWP_THREADS), lds, s, n, nrhs, ch, B, ldb, ipiv);
-  }
+  laswp_set(n, nrhs, B, ldb, ipiv, batch, s);
   if (nrhs == 1 && n <= GS_MAXN) {
     hipLaunchKernelGGL(k_getrs1, dim3(batch), dim3(GS_THREADS), sizeof(zc) * (size_t)n, s, n, LU, lda, B);
     CWX_HIP(hipGetLastError());

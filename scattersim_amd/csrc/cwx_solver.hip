@@ -1282,7 +1282,7 @@ struct SetBufs {
   size_t oTLU = 0, oTP = 0, oTB = 0, oCS = 0, oCD = 0, oGd0 = 0, oLU0 = 0, oPiv0 = 0, oB0 = 0;
   int nT = 0, ncS = 0;
   // the chain's block form (reduced mode): lists over the T items and over the instances' first layers
-  size_t oKV = 0, oKPV = 0, oKQ = 0, oKPQ = 0, oKX = 0, oKZ = 0, oKVp = 0, oKQAp = 0, oKT = 0;
+  size_t oKV = 0, oKPV = 0, oKQ = 0, oKPQ = 0, oKX = 0, oKZ = 0, oKVp = 0, oKQAp = 0, oKT = 0, oKDV = 0, oKDQ = 0;
   size_t oK0V = 0, oK0PV = 0, oK0Q = 0, oK0PQ = 0, oK0B = 0, oK0A1 = 0, oK0A2 = 0;
   DevBuf<int> d_tg, d_first;
   bool used_block = false;
@@ -1368,7 +1368,14 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
   b.Gc->alloc((size_t)Ltot * nn);
   wk.ipiv.ensure((size_t)Ltot * n4 + (size_t)count * (n4 + n6));
   wk.w2.ensure((size_t)count * nn);
-  wk.w3.ensure((size_t)std::max(1, Ltot) * nn);
+  {
+    // the product tree's pool; before the tree, in the reduced mode, the inverses of the diagonal blocks of
+    // Q and V and the solves' scratch; after it, those of S and the scratch of S^-1 f3
+    size_t w3 = (size_t)std::max(1, Ltot) * nn;
+    if (b.eig_mode == 1) w3 = std::max(w3, (size_t)Ltot * (2 * (size_t)trtri_diag_size(n2) + (size_t)mm));
+    w3 = std::max(w3, (size_t)count * ((size_t)trtri_diag_size(n4) + (size_t)n4 * n3));
+    wk.w3.ensure(w3);
+  }
   b.d_info_lu.ensure(2 * (size_t)Ltot);
   b.infoS.ensure(count);
   b.infoA.ensure(count);
@@ -1466,19 +1473,25 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
   }
   if (b.eig_mode == 1) {
     // the block form: in wk.w0, X at [0, nT mm) (P's place), Q at [Ltot mm, 2 Ltot mm) factored in place, the
-    // factors of V at [2 Ltot mm, 3 Ltot mm), Y at [3 Ltot mm, ...); pivots of V then Q at the start of ipiv
+    // factors of V at [2 Ltot mm, 3 Ltot mm), Y at [3 Ltot mm, ...); pivots of Q then V at the start of ipiv
+    // (Q and V are factored as one batch of 2 Ltot matrices)
     const long mm = (long)n2 * n2;
     zc* w0 = wk.w0.p;
     zc* Qb = w0 + (long)Ltot * mm;
     zc* LUV = w0 + 2L * Ltot * mm;
     zc* Zb = w0 + 3L * Ltot * mm;
-    int* pivV = piv;
-    int* pivQ = piv + (long)Ltot * n2;
-    std::vector<void*> kV, kPV, kQ, kPQ, kX, kZ, kVp, kQAp, kT;
+    int* pivQ = piv;
+    int* pivV = piv + (long)Ltot * n2;
+    // inverses of the diagonal blocks of the factors of Q then V (for the solves by products), in wk.w3
+    const long dsz = trtri_diag_size(n2);
+    zc* Dinv = wk.w3.p;
+    std::vector<void*> kV, kPV, kQ, kPQ, kX, kZ, kVp, kQAp, kT, kDV, kDQ;
     std::vector<int> tg;
     for (int i = 0; i < count; ++i)
       for (int l = 1; l < is.L[i]; ++l) {
         const long g = is.first[i] + l, t = (long)tg.size();
+        kDQ.push_back(Dinv + g * dsz);
+        kDV.push_back(Dinv + (Ltot + g) * dsz);
         kV.push_back(LUV + g * mm);
         kPV.push_back(pivV + g * n2);
         kQ.push_back(Qb + g * mm);
@@ -1499,6 +1512,8 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
     b.oKVp = b.pl.add(kVp);
     b.oKQAp = b.pl.add(kQAp);
     b.oKT = b.pl.add(kT);
+    b.oKDV = b.pl.add(kDV);
+    b.oKDQ = b.pl.add(kDQ);
     upload(b.d_tg, tg, s);
     std::vector<void*> k0V, k0PV, k0Q, k0PQ, k0B, k0A1, k0A2;
     std::vector<int> first;
@@ -1614,6 +1629,8 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   // the reduced mode's chain in block form, unless layers were redone through the full Phi (their modes lack
   // the block structure, and that Phi has overwritten P and Q)
   const bool block = eig_mode == 1 && b.flist.empty() && tune_lookup("chain_block", n4, Ltot, 1) != 0;
+  // solves with many right-hand sides apply the factors' diagonal blocks by their inverses
+  const bool inv = tune_lookup("getrs_inv", n4, Ltot, 1) != 0;
   b.used_block = block;
   if (!capture && !block) {
     b.V.release();
@@ -1637,16 +1654,38 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
     zc* Qb = wk.w0.p + (long)Ltot * mm;
     zc* LUV = wk.w0.p + 2L * Ltot * mm;
     hipLaunchKernelGGL(k_copy, dim3(grid_for((long)Ltot * mm)), dim3(256), 0, s, (long)Ltot * mm, b.V.p, LUV);
-    zgetrf_batched(n2, LUV, n2, mm, piv, n2, b.d_info_lu.p, Ltot, s);
-    zgetrf_batched(n2, Qb, n2, mm, piv + (long)Ltot * n2, n2, b.d_info_lu.p + Ltot, Ltot, s);
+    // Q_g at Qb + g mm and V_g at Qb + (Ltot + g) mm: one batch (pivots and infos in the same order)
+    const long dsz = trtri_diag_size(n2);
+    if (2L * Ltot <= 65535) {
+      zgetrf_batched(n2, Qb, n2, mm, piv, n2, b.d_info_lu.p, 2 * Ltot, s);
+      if (inv) ztrtri_diag_set(n2, strided(Qb, mm), n2, strided(wk.w3.p, dsz), 2 * Ltot, s);
+    } else {
+      zgetrf_batched(n2, Qb, n2, mm, piv, n2, b.d_info_lu.p, Ltot, s);
+      zgetrf_batched(n2, LUV, n2, mm, piv + (long)Ltot * n2, n2, b.d_info_lu.p + Ltot, Ltot, s);
+      if (inv) {
+        ztrtri_diag_set(n2, strided(Qb, mm), n2, strided(wk.w3.p, dsz), Ltot, s);
+        ztrtri_diag_set(n2, strided(LUV, mm), n2, strided(wk.w3.p + Ltot * dsz, dsz), Ltot, s);
+      }
+    }
     if (b.nT > 0) {
       hipLaunchKernelGGL(k_copy_set, dim3(grid_for(mm), (unsigned)b.nT), dim3(256), 0, s, mm, pl.mats(b.oKVp),
                          pl.mats(b.oKX));
       hipLaunchKernelGGL(k_copy_set, dim3(grid_for(mm), (unsigned)b.nT), dim3(256), 0, s, mm, pl.mats(b.oKQAp),
                          pl.mats(b.oKZ));
-      zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKX), n2, b.nT, s);
-      zgetrs_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKZ), n2, b.nT, s);
-      zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKZ), n2, b.nT, s);
+      if (inv) {
+        // the diagonal blocks applied as products by their inverses; scratch behind the inverses in w3
+        const MatSet Yt = strided(wk.w3.p + 2L * Ltot * dsz, mm);
+        zgetrs_inv_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKDV), pl.mats(b.oKX), n2, Yt, n2,
+                       b.nT, s);
+        zgetrs_inv_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKDQ), pl.mats(b.oKZ), n2, Yt, n2,
+                       b.nT, s);
+        zgetrs_inv_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKDV), pl.mats(b.oKZ), n2, Yt, n2,
+                       b.nT, s);
+      } else {
+        zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKX), n2, b.nT, s);
+        zgetrs_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKZ), n2, b.nT, s);
+        zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKZ), n2, b.nT, s);
+      }
       hipLaunchKernelGGL(k_t_block, dim3(grid_for(nn), (unsigned)b.nT), dim3(256), 0, s, ia, b.d_tg.p,
                          pl.mats(b.oKX), pl.mats(b.oKZ), pl.mats(b.oKT), gam.p, b.d_src.p, b.pscale.p, is.dz.p);
     }
@@ -1670,7 +1709,15 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
   zc* xp = b.x->p;
   zgetrf_batched(n4, Sm, n4, nn, pivS, n4, b.infoS.p, count, s);
   hipLaunchKernelGGL(k_build_f, dim3(grid_for((long)n4 * n3), count), dim3(256), 0, s, ia, 3, b.Y.p);
-  zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, b.Y.p, n4, (long)n4 * n3, count, s);
+  if (inv) {
+    // the pool is free once the tree is done: inverses of S's diagonal blocks, then the solve's scratch
+    const long dsz4 = trtri_diag_size(n4);
+    ztrtri_diag_set(n4, strided(Sm, nn), n4, strided(wk.w3.p, dsz4), count, s);
+    zgetrs_inv_set(n4, n3, strided(Sm, nn), n4, strided(pivS, n4), strided(wk.w3.p, dsz4),
+                   strided(b.Y.p, (long)n4 * n3), n4, strided(wk.w3.p + count * dsz4, (long)n4 * n3), n4, count, s);
+  } else {
+    zgetrs_batched(n4, n3, Sm, n4, nn, pivS, n4, b.Y.p, n4, (long)n4 * n3, count, s);
+  }
   zgemm_set(OpN, OpN, n4, n3, n4, mk(1.0), pl.mats(b.oGd0), n4, strided<zc>(b.Y.p, (long)n4 * n3), n4, mk(0.0),
             strided<zc>(b.Abr.p, (long)n4 * n3), n4, count, s);
   // the reflected amplitudes eliminated order by order: a 3MF x 3MF system for T (A's storage holds it)
@@ -1744,7 +1791,7 @@ void set_status(Context& ctx, SetBufs& b, const std::vector<VolumeOptions>& opts
   const InstSet& is = b.is;
   const int count = b.count, Ltot = b.Ltot, n6 = b.n6;
   if (b.full) finish_solution(*b.full, opts[0], is, b.F.p + (long)Ltot * b.MF, s);
-  // hlu: the LU of every Gd, or in the block form that of every Vr (then every Q behind it)
+  // hlu: the LU of every Gd, or in the block form that of every Q (then every Vr behind it)
   const int nlu = b.used_block ? 2 * Ltot : Ltot;
   std::vector<int> hinfo(Ltot), hlu(nlu), hS(count), hA(count), hnf(count), hun(Ltot, 0);
   CWX_HIP(hipMemcpyAsync(hinfo.data(), b.d_info.p, sizeof(int) * Ltot, hipMemcpyDeviceToHost, s));
@@ -1778,13 +1825,14 @@ void set_status(Context& ctx, SetBufs& b, const std::vector<VolumeOptions>& opts
     }
     for (int l = 0; l < is.L[i] && !inf; ++l) {
       const int g = is.first[i] + l;
-      if (hlu[g] != 0) {
-        inf = hlu[g];
+      const int ilu = b.used_block ? Ltot + g : g;  // Gd, or Vr in the block form
+      if (hlu[ilu] != 0) {
+        inf = hlu[ilu];
         m = "layer " + std::to_string(l) + (b.used_block ? ": singular mode matrix" : ": singular Gd") +
-            " (zero pivot " + std::to_string(hlu[g]) + ")";
-      } else if (b.used_block && hlu[Ltot + g] != 0) {
-        inf = hlu[Ltot + g];
-        m = "layer " + std::to_string(l) + ": singular Q block (zero pivot " + std::to_string(hlu[Ltot + g]) + ")";
+            " (zero pivot " + std::to_string(hlu[ilu]) + ")";
+      } else if (b.used_block && hlu[g] != 0) {
+        inf = hlu[g];
+        m = "layer " + std::to_string(l) + ": singular Q block (zero pivot " + std::to_string(hlu[g]) + ")";
       }
     }
     if (!inf && hS[i]) { inf = hS[i]; m = "singular transfer product"; }
@@ -1826,9 +1874,12 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
   const long n4 = 4L * st0.MF, n2 = 2L * st0.MF, n6 = 6L * st0.MF;
   const long nn = n4 * n4;
   const long n_eig = opts[0].eig_mode ? n2 : n4;
-  // bytes per layer: Phi or P,Q + LU (w0), PQ + T (w1), pool (w3), Gd, Gc, eigen work (Z, X), V, QA
-  const size_t per_layer = sizeof(zc) * (size_t)(nn * 5 + n_eig * n_eig * 4 + (long)s1 * s2 * 2);
-  const size_t per_inst = sizeof(zc) * (size_t)(nn + 2 * n4 * 3 * st0.MF + n6 * n6);
+  // bytes per layer: Phi or P,Q + LU (w0), PQ + T (w1), pool (w3, or the inverses of the diagonal blocks of Q
+  // and V and the solves' scratch where that is more), Gd, Gc, eigen work (Z, X), V, QA
+  const long w3_layer = std::max(nn, opts[0].eig_mode ? 2 * trtri_diag_size((int)n2) + n2 * n2 : 0L);
+  const size_t per_layer = sizeof(zc) * (size_t)(nn * 4 + w3_layer + n_eig * n_eig * 4 + (long)s1 * s2 * 2);
+  const size_t per_inst =
+      sizeof(zc) * (size_t)(nn + 2 * n4 * 3 * st0.MF + n6 * n6 + trtri_diag_size((int)n4) + n4 * 3 * st0.MF);
   if (max_bytes == 0) max_bytes = (size_t)tune_lookup("batch_chunk_mib", 4 * opts[0].coef[0] * opts[0].coef[1], count,
                                                       64 << 10) << 20;
   info.assign(count, 0);

@@ -103,6 +103,35 @@ def test_zgesv(ctx, n, nrhs):
     assert np.max(np.abs(X - np.linalg.solve(A, B))) <= 1e-10 * np.max(np.abs(X))
 
 
+@pytest.mark.parametrize("n,batch", [(40, 3), (200, 2), (450, 4), (675, 1), (900, 1), (1100, 1)])
+@pytest.mark.parametrize("panel", [0, 1, 2])
+def test_zgetrf_pivots_and_factors_match_lapack(ctx, n, batch, panel):
+    """zgetrf's pivots equal LAPACK's (scipy lu_factor: izamax on |re| + |im|, first of equals) and its packed
+    factors agree to rounding, with each panel kernel (lu_panel 2: whole rows per lane, 1: a column pair per
+    wavefront, both with 32- or 16-column panels by height; 0: rows per lane pair), across the panel
+    heights they choose between."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(n + batch)
+    A = np.stack([crand(rng, n, n) for _ in range(batch)])
+    A[0][:, 3] = 1.0  # a column of equal entries: the first of them is the pivot
+    dA = sa.DeviceArray.from_numpy(ctx, np.ascontiguousarray(A.transpose(0, 2, 1)))
+    dP = sa.DeviceArray(ctx, (batch, n), np.int32)
+    L = sa.lib()
+    try:
+        sa.check(L.cwx_set_param(b"lu_panel", panel))
+        sa.check(L.cwx_zgetrf_batched_dev(ctx.h, n, batch, dA.ptr, n * n, dP.ptr, n, None))
+    finally:
+        sa.check(L.cwx_set_param(b"lu_panel", sa._capi.CWX_PARAM_TABLE))
+    LU = dA.numpy().transpose(0, 2, 1)
+    piv = dP.numpy()
+    for k in range(batch):
+        lu, pv = sl.lu_factor(A[k])
+        assert np.array_equal(piv[k], pv), k
+        assert np.max(np.abs(LU[k] - lu)) <= 1e-12 * np.max(np.abs(lu)) * np.sqrt(n), k
+    dA.free()
+    dP.free()
+
+
 def test_zgesv_needs_pivoting(ctx):
     # zero leading pivot: solvable only with row interchanges
     A = np.array([[0, 1, 2], [1, 0, 3], [4, -3, 8]], dtype=np.complex128)

@@ -103,6 +103,27 @@ def test_reduced_chain_block_form_matches_order_4mf_chain(ctx):
     assert rel_err(blk.x, x4) < 1e-11
 
 
+@pytest.mark.parametrize("eig_mode", [0, 1])
+def test_solves_by_inverse_diagonal_blocks_match_substitution(ctx, eig_mode):
+    """The transfer chain's solves (reduced mode) and S^-1 f3 (both modes) with the factors' 64 x 64
+    diagonal blocks applied as products by their inverses (getrs_inv 1), against substitution in 32 x 32
+    diagonal blocks (getrs_inv 0) and against the oracle.  15x15-order problems are not needed: 7x7 orders
+    give 98- and 196-row factors, whose last diagonal blocks are cut by the matrix's end."""
+    sample = disks_sample(16, 6, 4.0, 4, 1.0, lambda r: 1.3 + 0.3 * r.random() + 0.02j * r.random(), seed=23)
+    p = o.VolumeParams(lam=0.6, direction=(0.2, 0.1, 1.0), ex=(1, 0.2), ey=(0.1, 0.3), size=(4.0, 4.0, 2.0),
+                       coef=(7, 7), n=(1.0, 1.2))
+    L = sa.lib()
+    try:
+        sa.check(L.cwx_set_param(b"getrs_inv", 0))
+        s0, _ = compare(ctx, sample, p, eig_mode=eig_mode)
+        x0 = s0.x.copy()
+        sa.check(L.cwx_set_param(b"getrs_inv", 1))
+        s1, _ = compare(ctx, sample, p, eig_mode=eig_mode)
+    finally:
+        sa.check(L.cwx_set_param(b"getrs_inv", sa._capi.CWX_PARAM_TABLE))
+    assert rel_err(s1.x, x0) < 1e-11
+
+
 def test_external_and_npz(ctx):
     sample = disks_sample(12, 2, 3.0, 3, 1.0, 1.5, seed=9)
     p = o.VolumeParams(lam=0.8, size=(3.0, 3.0, 1.0), coef=(3, 3), external=True)
