@@ -892,6 +892,11 @@ __device__ inline zc zrcp_scaled(zc p) {
   return mk(ldexp(a * r, -e), ldexp(-b * r, -e));
 }
 
+// x - a b with four fused multiply-adds
+__device__ inline zc zfms(zc x, zc a, zc b) {
+  return mk(fma(-a.re, b.re, fma(a.im, b.im, x.re)), fma(-a.re, b.im, fma(-a.im, b.re, x.im)));
+}
+
 // c ? a : b by components (a select of the structure itself would keep x out of registers)
 __device__ inline zc zsel(bool c, zc a, zc b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
 
@@ -1120,9 +1125,10 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
   const int b = blockIdx.y;
   const GP A = gview(Aset.at(b));
   const int* ipiv = pset.at(b);
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    int r = e % nb, c = e / nb;
-    L11[r][c] = A[(long)(j0 + c) * lda + j0 + r];
+  // the whole LU_NB x LU_NB block, zero outside nb x nb
+  for (int e = threadIdx.x; e < LU_NB * LU_NB; e += blockDim.x) {
+    const int r = e % LU_NB, c = e / LU_NB;
+    L11[r][c] = (r < nb && c < nb) ? (zc)A[(long)(j0 + c) * lda + j0 + r] : mk(0.0);
   }
   {
     // Lane t holds touched row t (my_row) and the original row whose value ends in it (my_src): the
@@ -1182,12 +1188,13 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
       if (t < nb) col[j0 + t] = x[t];
     return;
   }
+  // one uniform branch per column; inside, every row below is updated (L11 is zero past nb)
 #pragma unroll
-  for (int r = 0; r < LU_NB; ++r) {
+  for (int r = 0; r < LU_NB; ++r)
+    if (r < nb) {
 #pragma unroll
-    for (int q = r + 1; q < LU_NB; ++q)
-      if (q < nb) x[q] = x[q] - L11[q][r] * x[r];
-  }
+      for (int q = r + 1; q < LU_NB; ++q) x[q] = x[q] - L11[q][r] * x[r];
+    }
 #pragma unroll
   for (int r = 0; r < LU_NB; ++r)
     if (r < nb) col[j0 + r] = x[r];
@@ -1404,12 +1411,21 @@ __global__ __launch_bounds__(256) void k_trtri_diag(int n, MatSet LUset, int lda
   const int j0 = j * TI_NB, nbj = min(TI_NB, n - j0);
   const GP LU = gview(LUset.at(b));
   const GP D = gview(Dset.at(b)) + (long)(upper * nblk + j) * TI_NB * TI_NB;
-  for (int e = threadIdx.x; e < TI_NB * TI_NB; e += blockDim.x) {
-    const int r = e & (TI_NB - 1), c = e / TI_NB;
-    zc v = (r == c) ? mk(1.0) : mk(0.0);
-    if (r < nbj && c < nbj) v = LU[(long)(j0 + c) * lda + j0 + r];
-    Tc[c * TI_LD + r] = v;
-    Xs[c * TI_LD + r] = (r == c) ? mk(1.0) : mk(0.0);
+  {
+    constexpr int PER = TI_NB * TI_NB / 256;
+    zc v[PER];  // all loads in flight together (addresses clamped into the block, selected below)
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + 256 * k, r = e & (TI_NB - 1), c = e / TI_NB;
+      v[k] = LU[(long)(j0 + min(c, nbj - 1)) * lda + j0 + min(r, nbj - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + 256 * k, r = e & (TI_NB - 1), c = e / TI_NB;
+      const zc id = mk(r == c ? 1.0 : 0.0);
+      Tc[c * TI_LD + r] = zsel(r < nbj && c < nbj, v[k], id);
+      Xs[c * TI_LD + r] = id;
+    }
   }
   __syncthreads();
   const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
