@@ -55,6 +55,9 @@ const char* cwx_last_error(void);
  *   "lahr2_team"       helper workgroups per matrix of each Hessenberg panel in that case (0: none)
  *   "qr_threads"       256 or 512 threads per QR workgroup when they do
  *   "batch_chunk_mib"  device memory of one chunk of cwx_solve_batch_dev when max_bytes is 0
+ *   "alloc_cache"      1 (built-in): device buffers the library releases are kept (per device, at most
+ *                      32 GiB) and reused by later calls instead of being returned with hipFree; 0 returns
+ *                      them at once (and empties the cache).  Memory from cwx_dev_alloc is never cached
  *   "zgemm_kernel"     the FP64 complex GEMM: 1 (built-in) the matrix-core kernel with 64x64 tiles, 2 and 3
  *                      its 128x64 and 64x128 tilings, 0 the vector kernel
  *   "chain_block"      the reduced eigen mode's transfer chain: 1 (built-in) in block form from the order-2MF

@@ -4,7 +4,9 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <random>
 #include <string>
@@ -22,6 +24,114 @@ struct cwx_context {
 struct cwx_solution {
   cwx::Solution s;
 };
+
+namespace cwx {
+
+// ---- device memory cache of the library's buffers (declared in cwx_common.h) ----
+namespace {
+constexpr int kCacheDevices = 64;
+constexpr size_t kCacheMaxBytes = size_t(32) << 30;  // per device
+struct DevCache {
+  std::mutex m;
+  std::multimap<size_t, void*> ready;           // reusable now, by rounded size
+  std::vector<std::pair<void*, size_t>> pending;  // released since the last device synchronisation
+  size_t bytes = 0;                             // held in ready and pending
+};
+// never destroyed, so that buffers released during the process's exit still find it
+DevCache* const g_cache = new DevCache[kCacheDevices];
+std::atomic<bool> g_cache_on{true};
+thread_local int t_capturing = 0;
+
+size_t cache_round(size_t b) {
+  const size_t g = b < (size_t(1) << 20) ? 512 : (size_t(2) << 20);
+  return (b + g - 1) / g * g;
+}
+
+// Returns every cached block of c to the runtime (c.m held).  Pending blocks only outside a capture.
+void cache_flush_locked(DevCache& c) {
+  for (auto& kv : c.ready) (void)hipFree(kv.second);
+  c.ready.clear();
+  if (!t_capturing) {
+    for (auto& b : c.pending) (void)hipFree(b.first);
+    c.pending.clear();
+  }
+  c.bytes = 0;
+  for (auto& b : c.pending) c.bytes += b.second;
+}
+}  // namespace
+
+capture_scope::capture_scope() { ++t_capturing; }
+capture_scope::~capture_scope() { --t_capturing; }
+
+void* dev_alloc(size_t bytes, int* dev) {
+  CWX_HIP(hipGetDevice(dev));
+  CWX_CHECK(*dev >= 0 && *dev < kCacheDevices, "device index out of range");
+  const size_t rb = cache_round(bytes);
+  DevCache& c = g_cache[*dev];
+  std::lock_guard<std::mutex> lk(c.m);
+  if (g_cache_on.load()) {
+    if (!c.pending.empty() && !t_capturing) {
+      // work queued before those releases may still use them: one synchronisation covers them all
+      CWX_HIP(hipDeviceSynchronize());
+      for (auto& b : c.pending) c.ready.emplace(b.second, b.first);
+      c.pending.clear();
+    }
+    auto it = c.ready.find(rb);
+    if (it != c.ready.end()) {
+      void* p = it->second;
+      c.ready.erase(it);
+      c.bytes -= rb;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, rb) != hipSuccess) {
+    (void)hipGetLastError();
+    if (!t_capturing) CWX_HIP(hipDeviceSynchronize());
+    cache_flush_locked(c);
+    CWX_HIP(hipMalloc(&p, rb));
+  }
+  return p;
+}
+
+void dev_free(void* p, size_t bytes, int dev) {
+  if (!p) return;
+  const size_t rb = cache_round(bytes);
+  if (dev < 0 || dev >= kCacheDevices) {
+    (void)hipFree(p);
+    return;
+  }
+  DevCache& c = g_cache[dev];
+  std::lock_guard<std::mutex> lk(c.m);
+  if ((g_cache_on.load() && c.bytes + rb <= kCacheMaxBytes) || t_capturing) {
+    c.pending.emplace_back(p, rb);
+    c.bytes += rb;
+    return;
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(dev);
+  (void)hipFree(p);
+  (void)hipSetDevice(cur);
+}
+
+// Empties every device's cache and turns it on or off (cwx_set_param "alloc_cache").
+void dev_cache_set(bool on) {
+  g_cache_on.store(on);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (int d = 0; d < kCacheDevices; ++d) {
+    DevCache& c = g_cache[d];
+    std::lock_guard<std::mutex> lk(c.m);
+    if (c.ready.empty() && c.pending.empty()) continue;
+    if (hipSetDevice(d) != hipSuccess) continue;
+    if (!t_capturing) (void)hipDeviceSynchronize();
+    cache_flush_locked(c);
+  }
+  (void)hipSetDevice(cur);
+}
+
+}  // namespace cwx
 
 namespace {
 
@@ -149,6 +259,12 @@ int cwx_set_param(const char* name, int64_t value) {
     } else if (k == "getrs_inv") {
       CWX_CHECK(unset || (value >= -1 && value <= 1),
                 "getrs_inv takes -1 (built-in choice), 0 (substitution by diagonal blocks) or 1 (their inverses)");
+    } else if (k == "alloc_cache") {
+      CWX_CHECK(unset || (value >= -1 && value <= 1),
+                "alloc_cache takes -1 or 1 (keep released device buffers for reuse, the default) or 0 (return "
+                "them at once; the cache is emptied)");
+      cwx::dev_cache_set(unset || value != 0);
+      return CWX_OK;
     } else if (k == "zgemm_kernel") {
       CWX_CHECK(unset || (value >= -1 && value <= 3),
                 "zgemm_kernel takes -1 (built-in choice), 0 (vector) or 1-3 (matrix cores, 64x64, 128x64, 64x128)");

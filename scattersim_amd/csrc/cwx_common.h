@@ -98,29 +98,44 @@ inline size_t lds_limit() {
   return cached[dev];
 }
 
-// Device buffer owned by the library (freed on destruction).
+// Device memory of the library's buffers goes through a per-device cache: a released block is kept and
+// handed to the next request of the same rounded size instead of being returned with hipFree (which
+// synchronises the device and unmaps, a few hundred microseconds per large block).  Blocks released since
+// the last synchronisation are reused only after one device-wide synchronisation (the guarantee hipFree
+// gave for each), taken at the next allocation; during a stream capture by this thread (capture_scope) no
+// synchronisation is made and those blocks wait.  The cache holds at most a bounded number of bytes per
+// device and is emptied when an allocation fails; cwx_set_param("alloc_cache", 0) empties and disables it.
+void* dev_alloc(size_t bytes, int* dev);
+void dev_free(void* p, size_t bytes, int dev);
+struct capture_scope {  // marks this thread as capturing a stream for the lifetime of the object
+  capture_scope();
+  ~capture_scope();
+};
+
+// Device buffer owned by the library (released on destruction).
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  int dev = 0;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), dev(o.dev) { o.p = nullptr; o.n = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; dev = o.dev; o.p = nullptr; o.n = 0; }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
-    if (count) CWX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    if (count) p = static_cast<T*>(dev_alloc(count * sizeof(T), &dev));
     n = count;
   }
   void ensure(size_t count) { if (count > n) alloc(count); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, n * sizeof(T), dev);
     p = nullptr;
     n = 0;
   }

@@ -1985,6 +1985,7 @@ int solve_batch_graph(Context& ctx, const std::vector<VolumeOptions>& opts, cons
       CWX_HIP(hipStreamSynchronize(s));
       CWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
+        capture_scope cs;  // no device synchronisation by the buffer cache while the stream is captured
         set_launch(ctx, ng->b, ng->samples.p, s1, s2, res);
       } catch (...) {
         hipGraph_t dead = nullptr;

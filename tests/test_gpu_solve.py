@@ -124,6 +124,27 @@ def test_solves_by_inverse_diagonal_blocks_match_substitution(ctx, eig_mode):
     assert rel_err(s1.x, x0) < 1e-11
 
 
+def test_buffer_cache_does_not_change_results(ctx):
+    """Solves with the library's device-buffer cache off (alloc_cache 0: every buffer returned at once) and on
+    (released buffers reused by the next solve, after one device synchronisation) give the same bits,
+    including a solve whose buffers come back from the cache while the previous solution is still held."""
+    sample = disks_sample(16, 4, 4.0, 4, 1.0, lambda r: 1.3 + 0.2 * r.random() + 0.01j * r.random(), seed=29)
+    p = o.VolumeParams(lam=0.6, direction=(0.1, 0.2, 1.0), size=(4.0, 4.0, 1.5), coef=(5, 5), n=(1.0, 1.2))
+    L = sa.lib()
+    try:
+        sa.check(L.cwx_set_param(b"alloc_cache", 0))
+        x0 = gpu_solve(ctx, sample, p, eig_mode=1).x.copy()
+        sa.check(L.cwx_set_param(b"alloc_cache", 1))
+        a = gpu_solve(ctx, sample, p, eig_mode=1)
+        xa = a.x.copy()
+        a.free()
+        b = gpu_solve(ctx, sample, p, eig_mode=1)  # takes the blocks a released
+        c = gpu_solve(ctx, sample, p, eig_mode=1)  # while b is alive
+        assert np.array_equal(xa, x0) and np.array_equal(b.x, x0) and np.array_equal(c.x, x0)
+    finally:
+        sa.check(L.cwx_set_param(b"alloc_cache", sa._capi.CWX_PARAM_TABLE))
+
+
 def test_external_and_npz(ctx):
     sample = disks_sample(12, 2, 3.0, 3, 1.0, 1.5, seed=9)
     p = o.VolumeParams(lam=0.8, size=(3.0, 3.0, 1.0), coef=(3, 3), external=True)
