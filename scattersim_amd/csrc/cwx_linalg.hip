@@ -1200,6 +1200,120 @@ __global__ __launch_bounds__(64) void k_lu_swap_trsm(int n, int j0, int nb, MatS
     if (r < nb) col[j0 + r] = x[r];
 }
 
+// The same step with four lanes per column (256 threads, 64 columns per workgroup): lane p of a column's
+// quad holds the touched rows p, p + 4, ... of the panel (and of the pivot rows below it), so each lane
+// issues a quarter of the loads and stores, and in the forward substitution the row that becomes final
+// is passed to the other three lanes by a DPP quad broadcast (the row index is a compile-time constant,
+// so are all register indices).  L11 sits in LDS column-major and zero outside nb x nb; the rows at or
+// above the diagonal take a zero multiplier.
+template <int Q>
+__device__ inline double quad_bcast(double x) {
+  constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);  // quad_perm [Q, Q, Q, Q]
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int Q>
+__device__ inline zc quad_bcast_z(zc v) { return mk(quad_bcast<Q>(v.re), quad_bcast<Q>(v.im)); }
+
+template <int R>
+__device__ inline void st4_step(zc (&x)[LU_NB / 4], const zc (*Lc)[LU_NB], int part) {
+  const zc xr = quad_bcast_z<R & 3>(x[R >> 2]);
+#pragma unroll
+  for (int k = R >> 2; k < LU_NB / 4; ++k) {
+    const int t = part + 4 * k;
+    const zc l = zsel(t > R, Lc[R][t], mk(0.0));
+    x[k] = x[k] - l * xr;
+  }
+  if constexpr (R + 1 < LU_NB) st4_step<R + 1>(x, Lc, part);
+}
+
+__global__ __launch_bounds__(256) void k_lu_swap_trsm4(int n, int j0, int nb, MatSet Aset, int lda, IntSet pset) {
+  __shared__ zc Lc[LU_NB][LU_NB];  // Lc[c][r] = L11(r, c)
+  __shared__ int trow[2 * LU_NB], tsrc[2 * LU_NB];
+  __shared__ int s_nt;
+  const int b = blockIdx.y;
+  const GP A = gview(Aset.at(b));
+  const int* ipiv = pset.at(b);
+  const int tid = threadIdx.x;
+  {
+    zc v[LU_NB * LU_NB / 256];  // loads in flight together, addresses clamped into the block
+#pragma unroll
+    for (int k = 0; k < LU_NB * LU_NB / 256; ++k) {
+      const int e = tid + 256 * k, r = e % LU_NB, c = e / LU_NB;
+      v[k] = A[(long)(j0 + min(c, nb - 1)) * lda + j0 + min(r, nb - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < LU_NB * LU_NB / 256; ++k) {
+      const int e = tid + 256 * k, r = e % LU_NB, c = e / LU_NB;
+      Lc[c][r] = zsel(r < nb && c < nb, v[k], mk(0.0));
+    }
+  }
+  if (tid < 64) {
+    // the composition of the panel's interchanges, as in k_lu_swap_trsm
+    const int lane = tid;
+    int my_row = (lane < nb) ? j0 + lane : -1, my_src = my_row;
+    const int pv = (lane < nb) ? ipiv[j0 + lane] : 0;
+    int nt = nb;
+    for (int jj = 0; jj < nb; ++jj) {
+      const int p = __builtin_amdgcn_readlane(pv, jj);
+      if (p == j0 + jj) continue;
+      int tp = p - j0;
+      if (p >= j0 + nb) {
+        const unsigned long long hit = __ballot(my_row == p);
+        if (hit) {
+          tp = __ffsll((long long)hit) - 1;
+        } else {
+          tp = nt++;
+          if (lane == tp) my_row = my_src = p;
+        }
+      }
+      const int sj = __builtin_amdgcn_readlane(my_src, jj), sp = __builtin_amdgcn_readlane(my_src, tp);
+      if (lane == jj) my_src = sp;
+      else if (lane == tp) my_src = sj;
+    }
+    trow[lane] = my_row;
+    tsrc[lane] = my_src;
+    if (lane == 0) s_nt = nt;
+  }
+  __syncthreads();
+  const int part = tid & 3;
+  int c = blockIdx.x * 64 + (tid >> 2);
+  if (c >= j0) c += nb;  // columns [0, j0) and [j0 + nb, n)
+  const bool active = c < n;
+  const GP col = A + (long)(active ? c : 0) * lda;
+  const int ne = s_nt - nb;  // pivot rows below the panel
+  zc x[LU_NB / 4], w[LU_NB / 4];
+#pragma unroll
+  for (int k = 0; k < LU_NB / 4; ++k) {
+    const int t = part + 4 * k;
+    x[k] = col[tsrc[min(t, nb - 1)]];
+    w[k] = col[tsrc[ne > 0 ? nb + min(t, ne - 1) : 0]];
+  }
+#pragma unroll
+  for (int k = 0; k < LU_NB / 4; ++k) {
+    const int t = part + 4 * k;
+    x[k] = zsel(t < nb, x[k], mk(0.0));
+    w[k] = zsel(t < ne, w[k], mk(0.0));
+  }
+  // every lane of a column has its sources before any lane stores (the touched rows are shared by the
+  // column's four lanes)
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int k = 0; k < LU_NB / 4; ++k) {
+    const int t = part + 4 * k;
+    if (t < ne) col[trow[nb + t]] = w[k];
+  }
+  if (c >= j0) st4_step<0>(x, Lc, part);
+#pragma unroll
+  for (int k = 0; k < LU_NB / 4; ++k) {
+    const int t = part + 4 * k;
+    if (t < nb) col[j0 + t] = x[k];
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // getrs pieces: row interchanges on B, and small diagonal-block triangular solves.
 // ------------------------------------------------------------------------------------------------
@@ -1541,7 +1655,9 @@ void zgetrf_set(int n, MatSet A, int lda, IntSet ipiv, int* info, int batch, hip
     else
       hipLaunchKernelGGL(k_lu_panel, dim3(batch), dim3(LU_THREADS), 0, s, n, j0, nb, A, lda, ipiv, info);
     const int others = n - nb;
-    if (others > 0)
+    if (others > 0 && kind != 0)
+      hipLaunchKernelGGL(k_lu_swap_trsm4, dim3(cdiv(others, 64), batch), dim3(256), 0, s, n, j0, nb, A, lda, ipiv);
+    else if (others > 0)
       hipLaunchKernelGGL(k_lu_swap_trsm, dim3(cdiv(others, 64), batch), dim3(64), 0, s, n, j0, nb, A, lda, ipiv);
     const int rest = n - j0 - nb;
     if (rest > 0)
