@@ -1283,6 +1283,8 @@ struct SetBufs {
   int nT = 0, ncS = 0;
   // the chain's block form (reduced mode): lists over the T items and over the instances' first layers
   size_t oKV = 0, oKPV = 0, oKQ = 0, oKPQ = 0, oKX = 0, oKZ = 0, oKVp = 0, oKQAp = 0, oKT = 0, oKDV = 0, oKDQ = 0;
+  // the two independent solves of the chain (V_g X = Vr_{g-1}, Q_g Z = QA_{g-1}) as one batch: V's then Q's
+  size_t oK2LU = 0, oK2P = 0, oK2D = 0, oK2B = 0;
   size_t oK0V = 0, oK0PV = 0, oK0Q = 0, oK0PQ = 0, oK0B = 0, oK0A1 = 0, oK0A2 = 0;
   DevBuf<int> d_tg, d_first;
   bool used_block = false;
@@ -1372,7 +1374,7 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
     // the product tree's pool; before the tree, in the reduced mode, the inverses of the diagonal blocks of
     // Q and V and the solves' scratch; after it, those of S and the scratch of S^-1 f3
     size_t w3 = (size_t)std::max(1, Ltot) * nn;
-    if (b.eig_mode == 1) w3 = std::max(w3, (size_t)Ltot * (2 * (size_t)trtri_diag_size(n2) + (size_t)mm));
+    if (b.eig_mode == 1) w3 = std::max(w3, (size_t)Ltot * (2 * (size_t)trtri_diag_size(n2) + 2 * (size_t)mm));
     w3 = std::max(w3, (size_t)count * ((size_t)trtri_diag_size(n4) + (size_t)n4 * n3));
     wk.w3.ensure(w3);
   }
@@ -1514,6 +1516,14 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
     b.oKT = b.pl.add(kT);
     b.oKDV = b.pl.add(kDV);
     b.oKDQ = b.pl.add(kDQ);
+    auto cat = [](std::vector<void*> a, const std::vector<void*>& c) {
+      a.insert(a.end(), c.begin(), c.end());
+      return a;
+    };
+    b.oK2LU = b.pl.add(cat(kV, kQ));
+    b.oK2P = b.pl.add(cat(kPV, kPQ));
+    b.oK2D = b.pl.add(cat(kDV, kDQ));
+    b.oK2B = b.pl.add(cat(kX, kZ));
     upload(b.d_tg, tg, s);
     std::vector<void*> k0V, k0PV, k0Q, k0PQ, k0B, k0A1, k0A2;
     std::vector<int> first;
@@ -1672,18 +1682,30 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
                          pl.mats(b.oKX));
       hipLaunchKernelGGL(k_copy_set, dim3(grid_for(mm), (unsigned)b.nT), dim3(256), 0, s, mm, pl.mats(b.oKQAp),
                          pl.mats(b.oKZ));
+      // X = V_g^-1 Vr_{g-1} and Z = Q_g^-1 QA_{g-1} as one batch of 2 nT solves (twice the workgroups per
+      // launch), then Y = V_g^-1 Z
+      const bool pair = 2L * b.nT <= 65535;
       if (inv) {
         // the diagonal blocks applied as products by their inverses; scratch behind the inverses in w3
         const MatSet Yt = strided(wk.w3.p + 2L * Ltot * dsz, mm);
-        zgetrs_inv_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKDV), pl.mats(b.oKX), n2, Yt, n2,
-                       b.nT, s);
-        zgetrs_inv_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKDQ), pl.mats(b.oKZ), n2, Yt, n2,
-                       b.nT, s);
+        if (pair) {
+          zgetrs_inv_set(n2, n2, pl.mats(b.oK2LU), n2, pl.ints(b.oK2P), pl.mats(b.oK2D), pl.mats(b.oK2B), n2, Yt, n2,
+                         2 * b.nT, s);
+        } else {
+          zgetrs_inv_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKDV), pl.mats(b.oKX), n2, Yt, n2,
+                         b.nT, s);
+          zgetrs_inv_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKDQ), pl.mats(b.oKZ), n2, Yt, n2,
+                         b.nT, s);
+        }
         zgetrs_inv_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKDV), pl.mats(b.oKZ), n2, Yt, n2,
                        b.nT, s);
       } else {
-        zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKX), n2, b.nT, s);
-        zgetrs_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKZ), n2, b.nT, s);
+        if (pair) {
+          zgetrs_set(n2, n2, pl.mats(b.oK2LU), n2, pl.ints(b.oK2P), pl.mats(b.oK2B), n2, 2 * b.nT, s);
+        } else {
+          zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKX), n2, b.nT, s);
+          zgetrs_set(n2, n2, pl.mats(b.oKQ), n2, pl.ints(b.oKPQ), pl.mats(b.oKZ), n2, b.nT, s);
+        }
         zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKZ), n2, b.nT, s);
       }
       hipLaunchKernelGGL(k_t_block, dim3(grid_for(nn), (unsigned)b.nT), dim3(256), 0, s, ia, b.d_tg.p,
@@ -1876,7 +1898,7 @@ int solve_batch(Context& ctx, const std::vector<VolumeOptions>& opts, const zc* 
   const long n_eig = opts[0].eig_mode ? n2 : n4;
   // bytes per layer: Phi or P,Q + LU (w0), PQ + T (w1), pool (w3, or the inverses of the diagonal blocks of Q
   // and V and the solves' scratch where that is more), Gd, Gc, eigen work (Z, X), V, QA
-  const long w3_layer = std::max(nn, opts[0].eig_mode ? 2 * trtri_diag_size((int)n2) + n2 * n2 : 0L);
+  const long w3_layer = std::max(nn, opts[0].eig_mode ? 2 * trtri_diag_size((int)n2) + 2 * n2 * n2 : 0L);
   const size_t per_layer = sizeof(zc) * (size_t)(nn * 4 + w3_layer + n_eig * n_eig * 4 + (long)s1 * s2 * 2);
   const size_t per_inst =
       sizeof(zc) * (size_t)(nn + 2 * n4 * 3 * st0.MF + n6 * n6 + trtri_diag_size((int)n4) + n4 * 3 * st0.MF);
