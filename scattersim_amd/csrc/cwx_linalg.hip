@@ -841,41 +841,57 @@ __device__ inline zc shfl_z(zc v, int src) {
 // (value, position) maximum over the wavefront, ties to the smaller position; every lane gets the result.
 // Rotations by 1, 2, 4, 8 inside each row of 16 lanes (DPP row_ror), then across rows by lane exchanges.
 __device__ inline void wave_argmax(double& v, int& idx) {
-  auto ror = [](double x, int sel) {
+  // The largest value by DPP alone (rotations inside rows of 16, then row broadcasts 15 and 31: lane 63
+  // ends with the maximum), then the smallest position among the lanes that hold it: usually one lane, read
+  // directly; with ties, a minimum over those lanes (rotations and lane exchanges).
+  auto dpp64 = [](double x, double old, int ctrl, int rows) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
     unsigned lo, hi;
-    switch (sel) {
-      case 1: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x121, 0xF, 0xF, false);
-              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x121, 0xF, 0xF, false); break;
-      case 2: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x122, 0xF, 0xF, false);
-              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x122, 0xF, 0xF, false); break;
-      case 4: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x124, 0xF, 0xF, false);
-              hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x124, 0xF, 0xF, false); break;
-      default: lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, 0x128, 0xF, 0xF, false);
-               hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false); break;
+    switch (ctrl) {
+      case 0x121: lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x121, 0xF, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x121, 0xF, 0xF, false);
+                  break;
+      case 0x122: lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x122, 0xF, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x122, 0xF, 0xF, false);
+                  break;
+      case 0x124: lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x124, 0xF, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x124, 0xF, 0xF, false);
+                  break;
+      case 0x128: lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x128, 0xF, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x128, 0xF, 0xF, false);
+                  break;
+      case 0x142: lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x142, 0xA, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x142, 0xA, 0xF, false);
+                  break;
+      default:    lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, 0x143, 0xC, 0xF, false);
+                  hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), 0x143, 0xC, 0xF, false);
+                  break;
     }
+    (void)rows;
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
   };
-  auto rori = [](int x, int sel) {
-    switch (sel) {
-      case 1: return __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false);
-      case 2: return __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false);
-      case 4: return __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);
-      default: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
-    }
-  };
+  double m = v;
+  m = fmax(m, dpp64(m, m, 0x121, 0));
+  m = fmax(m, dpp64(m, m, 0x122, 0));
+  m = fmax(m, dpp64(m, m, 0x124, 0));
+  m = fmax(m, dpp64(m, m, 0x128, 0));
+  m = fmax(m, dpp64(m, m, 0x142, 0));
+  m = fmax(m, dpp64(m, m, 0x143, 0));
+  const double vmax = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 63),
+                                       __builtin_amdgcn_readlane(__double2loint(m), 63));
+  const unsigned long long tie = __ballot(v == vmax);
+  int imin;
+  if (__popcll(tie) == 1) {
+    imin = __builtin_amdgcn_readlane(idx, __ffsll((long long)tie) - 1);
+  } else {
+    int k = (v == vmax) ? idx : 0x7fffffff;
 #pragma unroll
-  for (int sh = 1; sh <= 8; sh <<= 1) {
-    const double ov = ror(v, sh);
-    const int oi = rori(idx, sh);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    for (int sh = 1; sh < 64; sh <<= 1) k = min(k, __shfl_xor(k, sh));
+    imin = k;
   }
-#pragma unroll
-  for (int sh = 16; sh < 64; sh <<= 1) {
-    const double ov = __shfl_xor(v, sh);
-    const int oi = __shfl_xor(idx, sh);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
+  v = vmax;
+  idx = imin;
 }
 
 // 1 / p from the hardware reciprocal estimate and two Newton steps, p scaled by a power of two so that
