@@ -1523,19 +1523,49 @@ __global__ void k_fill_int(int n, int* p, int v) {
 // ------------------------------------------------------------------------------------------------
 // Inverses of the 64 x 64 diagonal blocks of an LU factorisation, for the solves that apply them as
 // matrix products.  Workgroup (j, f, b): block j of matrix b, f = 0 the unit lower factor, f = 1 the
-// upper one.  The block (transposed) and the inverse being built both sit in LDS (2 x 65 KiB, so one
-// workgroup per compute unit); a block cut by the matrix's end is completed with the identity, whose
-// inverse it keeps.  Lane (c, p) of the 256 (c = tid / 4, p = tid % 4) owns rows p, p + 4, ..., p + 60
-// of column c of the inverse: column elimination, one barrier per row.  The output is column-major,
-// leading dimension 64: block j of L^-1 at j 64^2, of U^-1 at (nblk + j) 64^2 of the matrix's area.
+// upper one.  The block sits in LDS (column-major, 65 KiB, two workgroups per compute unit); a block cut
+// by the matrix's end is completed with the identity, whose inverse it keeps.  Lane (c, p) of the 256
+// (c = tid / 4, p = tid % 4) holds rows p, p + 4, ..., p + 60 of column c of the inverse in registers:
+// column elimination with the final row passed inside the quad by DPP, no barrier.  The output is
+// column-major, leading dimension 64: block j of L^-1 at j 64^2, of U^-1 at (nblk + j) 64^2 of the
+// matrix's area.
 // ------------------------------------------------------------------------------------------------
 constexpr int TI_NB = 64, TI_LD = TI_NB + 1;
-constexpr size_t TI_LDS = 2 * sizeof(zc) * TI_NB * TI_LD;
+constexpr size_t TI_LDS = sizeof(zc) * (TI_NB * TI_LD + TI_NB);
+
+template <int Q>
+__device__ inline void ti_lower(zc (&x)[16], const zc* Tc, int part) {
+  // row Q is final: the rows below take x_r -= L[r][Q] x_Q (a zero multiplier at and above the diagonal)
+  const zc xq = quad_bcast_z<Q & 3>(x[Q >> 2]);
+  const zc* tq = Tc + Q * TI_LD;
+#pragma unroll
+  for (int k = Q >> 2; k < 16; ++k) {
+    const int r = part + 4 * k;
+    x[k] = x[k] - zsel(r > Q, tq[r], mk(0.0)) * xq;
+  }
+  __builtin_amdgcn_sched_barrier(0);  // one row's LDS reads at a time (registers)
+  if constexpr (Q + 2 < TI_NB) ti_lower<Q + 1>(x, Tc, part);
+}
+
+template <int Q>
+__device__ inline void ti_upper(zc (&x)[16], const zc* Tc, const zc* rdiag, int part) {
+  // x_Q times 1 / U[Q][Q] in the lane that owns row Q, then the rows above take x_r -= U[r][Q] x_Q
+  x[Q >> 2] = zsel(part == (Q & 3), x[Q >> 2] * rdiag[Q], x[Q >> 2]);
+  const zc xq = quad_bcast_z<Q & 3>(x[Q >> 2]);
+  const zc* tq = Tc + Q * TI_LD;
+#pragma unroll
+  for (int k = 0; k <= (Q >> 2); ++k) {
+    const int r = part + 4 * k;
+    x[k] = x[k] - zsel(r < Q, tq[r], mk(0.0)) * xq;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (Q > 0) ti_upper<Q - 1>(x, Tc, rdiag, part);
+}
 
 __global__ __launch_bounds__(256) void k_trtri_diag(int n, MatSet LUset, int lda, MatSet Dset) {
   extern __shared__ __align__(16) unsigned char ti_lds[];
   zc* Tc = reinterpret_cast<zc*>(ti_lds);  // Tc[c * TI_LD + r] = block(r, c)
-  zc* Xs = Tc + TI_NB * TI_LD;             // Xs[c * TI_LD + r] = inverse(r, c)
+  zc* rdiag = Tc + TI_NB * TI_LD;          // 1 / U[q][q]
   const int j = blockIdx.x, upper = blockIdx.y, b = blockIdx.z;
   const int nblk = (n + TI_NB - 1) / TI_NB;
   const int j0 = j * TI_NB, nbj = min(TI_NB, n - j0);
@@ -1552,45 +1582,24 @@ __global__ __launch_bounds__(256) void k_trtri_diag(int n, MatSet LUset, int lda
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + 256 * k, r = e & (TI_NB - 1), c = e / TI_NB;
-      const zc id = mk(r == c ? 1.0 : 0.0);
-      Tc[c * TI_LD + r] = zsel(r < nbj && c < nbj, v[k], id);
-      Xs[c * TI_LD + r] = id;
+      Tc[c * TI_LD + r] = zsel(r < nbj && c < nbj, v[k], mk(r == c ? 1.0 : 0.0));
     }
   }
   __syncthreads();
+  if (upper && threadIdx.x < TI_NB) rdiag[threadIdx.x] = zdiv(mk(1.0), Tc[threadIdx.x * TI_LD + threadIdx.x]);
+  __syncthreads();
+  // lane (c, p) holds rows p, p + 4, ..., p + 60 of column c of the inverse; a quad is one column, so the
+  // elimination needs no barrier (rows pass between the four lanes by DPP)
   const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
-  zc* xc = Xs + c * TI_LD;
-  if (!upper) {
-    // row q of every column is final after step q - 1: rows r > q take x_r -= L[r][q] x_q
-#pragma unroll 1
-    for (int q = 0; q < TI_NB - 1; ++q) {
-      const zc xq = xc[q];
-      const zc* tq = Tc + q * TI_LD;
+  zc x[16];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = part + 4 * i;
-        if (r > q) xc[r] = xc[r] - tq[r] * xq;
-      }
-      __syncthreads();
-    }
-  } else {
-    // x_q /= U[q][q] by the lane that owns row q, then rows r < q take x_r -= U[r][q] x_q
-#pragma unroll 1
-    for (int q = TI_NB - 1; q >= 0; --q) {
-      const zc* tq = Tc + q * TI_LD;
-      if (part == (q & 3)) xc[q] = zdiv(xc[q], tq[q]);
-      __syncthreads();
-      const zc xq = xc[q];
+  for (int k = 0; k < 16; ++k) x[k] = mk(part + 4 * k == c ? 1.0 : 0.0);
+  if (upper)
+    ti_upper<TI_NB - 1>(x, Tc, rdiag, part);
+  else
+    ti_lower<0>(x, Tc, part);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = part + 4 * i;
-        if (r < q) xc[r] = xc[r] - tq[r] * xq;
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 16; ++i) D[(long)c * TI_NB + part + 4 * i] = xc[part + 4 * i];
+  for (int k = 0; k < 16; ++k) D[(long)c * TI_NB + part + 4 * k] = x[k];
 }
 
 }  // namespace
