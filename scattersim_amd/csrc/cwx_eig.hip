@@ -466,16 +466,37 @@ __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc*
     __syncthreads();
     if (j > 0) {
       // b -= Y(:, 0:j) conj(V(j-1, 0:j)): the right update of this column by the earlier reflectors
+      // loads four columns at a time (addresses clamped below j), so that they are in flight together;
+      // the terms are added in the same order as one at a time
       for (int r = tid; r < m; r += nt) {
         zc acc = bcol[r];
-        for (int q = 0; q < j; ++q) acc = acc - Yp(r, q) * conj(Vp(j - 1, q));
+        for (int q0 = 0; q0 < j; q0 += 4) {
+          zc yv[4], vc[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int q = min(q0 + k, j - 1);
+            yv[k] = Yp(r, q);
+            vc[k] = Vp(j - 1, q);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (q0 + k < j) acc = acc - yv[k] * conj(vc[k]);
+        }
         bcol[r] = acc;
       }
       __syncthreads();
       // b -= V T^H V^H b: the left update by the earlier reflectors
       for (int q = wid; q < j; q += nw) {
         zc acc = mk(0.0);
-        for (int r = q + lane; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), bcol[r]);
+        int r = q + lane;
+        for (; r + 192 < m; r += 256) {  // four loads in flight, the same order of terms
+          const zc v0 = Vp(r, q), v1 = Vp(r + 64, q), v2 = Vp(r + 128, q), v3 = Vp(r + 192, q);
+          fma_acc(acc, conj(v0), bcol[r]);
+          fma_acc(acc, conj(v1), bcol[r + 64]);
+          fma_acc(acc, conj(v2), bcol[r + 128]);
+          fma_acc(acc, conj(v3), bcol[r + 192]);
+        }
+        for (; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), bcol[r]);
         acc.re = wave_sum(acc.re);
         acc.im = wave_sum(acc.im);
         if (lane == 0) w[q] = acc;
@@ -489,7 +510,14 @@ __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc*
       __syncthreads();
       for (int r = tid; r < m; r += nt) {
         zc acc = bcol[r];
-        for (int q = 0; q < j; ++q) acc = acc - Vp(r, q) * t2[q];
+        for (int q0 = 0; q0 < j; q0 += 8) {
+          zc vr8[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) vr8[k] = Vp(r, min(q0 + k, j - 1));
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (q0 + k < j) acc = acc - vr8[k] * t2[q0 + k];
+        }
         bcol[r] = acc;
       }
       __syncthreads();
@@ -547,7 +575,15 @@ __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc*
     // t = V(:, 0:j)^H v
     for (int q = wid; q < j; q += nw) {
       zc acc = mk(0.0);
-      for (int r = j + lane; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), vv[r - j]);
+      int r = j + lane;
+      for (; r + 192 < m; r += 256) {  // four loads in flight, the same order of terms
+        const zc v0 = Vp(r, q), v1 = Vp(r + 64, q), v2 = Vp(r + 128, q), v3 = Vp(r + 192, q);
+        fma_acc(acc, conj(v0), vv[r - j]);
+        fma_acc(acc, conj(v1), vv[r + 64 - j]);
+        fma_acc(acc, conj(v2), vv[r + 128 - j]);
+        fma_acc(acc, conj(v3), vv[r + 192 - j]);
+      }
+      for (; r < m; r += 64) fma_acc(acc, conj(Vp(r, q)), vv[r - j]);
       acc.re = wave_sum(acc.re);
       acc.im = wave_sum(acc.im);
       if (lane == 0) w[q] = acc;
@@ -556,7 +592,14 @@ __global__ __launch_bounds__(HH_THREADS) void k_lahr2(int n, int i0, int nb, zc*
     // Y(:, j) = tau (y - Y(:, 0:j) t)
     for (int r = tid; r < m; r += nt) {
       zc acc = bcol[r];
-      for (int q = 0; q < j; ++q) acc = acc - Yp(r, q) * w[q];
+      for (int q0 = 0; q0 < j; q0 += 8) {
+        zc yr8[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) yr8[k] = Yp(r, min(q0 + k, j - 1));
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (q0 + k < j) acc = acc - yr8[k] * w[q0 + k];
+      }
       Yp(r, j) = tj * acc;
     }
     // T(0:j, j) = -tau T(0:j, 0:j) t,  T(j, j) = tau
