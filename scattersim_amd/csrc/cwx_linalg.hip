@@ -122,43 +122,43 @@ __global__ __launch_bounds__(256) void k_zgemm(Op opa, Op opb, int m, int n, int
 }
 
 // ------------------------------------------------------------------------------------------------
-// Matrix-vector products (n <= 4 right-hand sides, op(A) = A): 64 rows per workgroup, sixteen
-// wavefronts take the columns c = w (mod 16), four columns in flight per wavefront, and the partial
-// sums are combined in LDS in a fixed order (results do not depend on scheduling or on the batch).
-// A 900-row matrix thus keeps 15 workgroups of 16 wavefronts busy instead of 15 of four, whose
-// one-column-at-a-time loads were latency-bound.
+// Matrix-vector products (n <= 4 right-hand sides, op(A) = A): 16 rows per workgroup; the 64 column groups
+// (16 wavefronts x 4 quarter-wavefronts, each quarter on the 16 rows) take the columns c = g (mod 64), four
+// columns in flight per lane, and the partial sums are combined in LDS in a fixed order (results do not
+// depend on scheduling or on the batch).  A 900-row matrix keeps 57 workgroups busy, each lane with about
+// four rounds of loads (64 rows per workgroup took fourteen).
 // ------------------------------------------------------------------------------------------------
-constexpr int GV_MAXN = 4, GV_WAVES = 16, GV_UNROLL = 4;
+constexpr int GV_MAXN = 4, GV_WAVES = 16, GV_UNROLL = 4, GV_ROWS = 16, GV_GROUPS = GV_WAVES * 64 / GV_ROWS;
 
 __global__ __launch_bounds__(64 * GV_WAVES) void k_zgemv(int m, int n, int k, zc alpha, MatSet Aset, int lda,
                                                          MatSet Bset, int ldb, zc beta, MatSet Cset, int ldc) {
-  __shared__ zc part[GV_WAVES][64];
+  __shared__ zc part[GV_GROUPS][GV_ROWS];
   const int b = blockIdx.y;
   const GP A = gview(Aset.at(b));
   const GP B = gview(Bset.at(b));
   zc* __restrict__ C = Cset.at(b);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  const int rr = threadIdx.x % GV_ROWS, g = threadIdx.x / GV_ROWS;
+  const int r = blockIdx.x * GV_ROWS + rr;
   zc acc[GV_MAXN][GV_UNROLL];
 #pragma unroll
   for (int j = 0; j < GV_MAXN; ++j)
 #pragma unroll
     for (int u = 0; u < GV_UNROLL; ++u) acc[j][u] = mk(0.0);
   if (r < m) {
-    constexpr int step = GV_WAVES * GV_UNROLL;
-    int c = wv;
-    for (; c + (GV_UNROLL - 1) * GV_WAVES < k; c += step) {
+    constexpr int step = GV_GROUPS * GV_UNROLL;
+    int c = g;
+    for (; c + (GV_UNROLL - 1) * GV_GROUPS < k; c += step) {
       zc a[GV_UNROLL];
 #pragma unroll
-      for (int u = 0; u < GV_UNROLL; ++u) a[u] = A[(long)(c + u * GV_WAVES) * lda + r];
+      for (int u = 0; u < GV_UNROLL; ++u) a[u] = A[(long)(c + u * GV_GROUPS) * lda + r];
 #pragma unroll
       for (int j = 0; j < GV_MAXN; ++j)
         if (j < n) {
 #pragma unroll
-          for (int u = 0; u < GV_UNROLL; ++u) fma_acc(acc[j][u], a[u], B[(long)j * ldb + c + u * GV_WAVES]);
+          for (int u = 0; u < GV_UNROLL; ++u) fma_acc(acc[j][u], a[u], B[(long)j * ldb + c + u * GV_GROUPS]);
         }
     }
-    for (; c < k; c += GV_WAVES) {
+    for (; c < k; c += GV_GROUPS) {
       const zc a0 = A[(long)c * lda + r];
 #pragma unroll
       for (int j = 0; j < GV_MAXN; ++j)
@@ -170,17 +170,20 @@ __global__ __launch_bounds__(64 * GV_WAVES) void k_zgemv(int m, int n, int k, zc
   for (int j = 0; j < GV_MAXN; ++j) {
     if (j >= n) break;
     if (j > 0) __syncthreads();  // the previous column's partials have been read
-    part[wv][lane] = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
+    part[g][rr] = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
     __syncthreads();
-    if (wv == 0 && r < m) {
-      zc t[GV_WAVES / 2];
+    if (threadIdx.x < GV_ROWS && r < m) {
+      // the 64 groups in a fixed order, four running sums
+      zc t0 = part[0][rr], t1 = part[1][rr], t2 = part[2][rr], t3 = part[3][rr];
 #pragma unroll
-      for (int w = 0; w < GV_WAVES / 2; ++w) t[w] = part[2 * w][lane] + part[2 * w + 1][lane];
-#pragma unroll
-      for (int w = 0; w < GV_WAVES / 4; ++w) t[w] = t[2 * w] + t[2 * w + 1];
-      const zc sum = (t[0] + t[1]) + (t[2] + t[3]);
+      for (int w = 4; w < GV_GROUPS; w += 4) {
+        t0 = t0 + part[w][rr];
+        t1 = t1 + part[w + 1][rr];
+        t2 = t2 + part[w + 2][rr];
+        t3 = t3 + part[w + 3][rr];
+      }
       zc* cp = C + (long)j * ldc + r;
-      zc v = alpha * sum;
+      zc v = alpha * ((t0 + t1) + (t2 + t3));
       if (!beta0) v = v + beta * (*cp);
       *cp = v;
     }
@@ -1610,8 +1613,8 @@ void zgemm_set(Op opa, Op opb, int m, int n, int k, zc alpha, MatSet A, int lda,
   CWX_CHECK(batch <= 65535, "zgemm: batch too large");
   CWX_CHECK(k > 0, "zgemm: k must be positive");
   if (n <= GV_MAXN && opa == OpN && opb == OpN) {
-    hipLaunchKernelGGL(k_zgemv, dim3(cdiv(m, 64), batch), dim3(64 * GV_WAVES), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc);
+    hipLaunchKernelGGL(k_zgemv, dim3(cdiv(m, GV_ROWS), batch), dim3(64 * GV_WAVES), 0, s, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc);
     CWX_HIP(hipGetLastError());
     return;
   }
