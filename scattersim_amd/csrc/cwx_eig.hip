@@ -1692,9 +1692,24 @@ __global__ __launch_bounds__(NT, 2) void k_hqr(int n, zc* H, int ldh, long sH, z
         const int clo = max(kbase - 1, L);
         auto MBp = [&](int r, int c) -> zc& { return MB[(c - kbase + 1) * MS_LD + (r - kbase)]; };
         const int nr = rhi - rlo + 1, nc = rhi - clo + 1;
-        for (int e = tid; e < nr * nc; e += nt) {
-          const int r = rlo + e % nr, c = clo + e / nr;
-          MBp(r, c) = Hp(r, c);
+        {
+          // every lane's loads are issued before their LDS stores (a rolled loop would wait for each load
+          // before issuing the next)
+          // (four at a time: the kernel's registers are spoken for)
+          const int tot = nr * nc;
+          for (int e0 = 0; e0 < tot; e0 += 4 * nt) {
+            zc wv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int e = min(e0 + tid + k * nt, tot - 1);
+              wv[k] = Hp(rlo + e % nr, clo + e / nr);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int e = e0 + tid + k * nt;
+              if (e < tot) MBp(rlo + e % nr, clo + e / nr) = wv[k];
+            }
+          }
         }
         // reflector of bulge j at step t (zero when the bulge is outside the active block) and the
         // value it leaves in the bulge column
