@@ -1286,6 +1286,7 @@ struct SetBufs {
   // the two independent solves of the chain (V_g X = Vr_{g-1}, Q_g Z = QA_{g-1}) as one batch: V's then Q's
   size_t oK2LU = 0, oK2P = 0, oK2D = 0, oK2B = 0;
   size_t oK0V = 0, oK0PV = 0, oK0Q = 0, oK0PQ = 0, oK0B = 0, oK0A1 = 0, oK0A2 = 0;
+  size_t oK0VQ = 0, oK0PVQ = 0, oK0A12 = 0;  // the lists above, Vr then Q (a1 then a2)
   DevBuf<int> d_tg, d_first;
   bool used_block = false;
 };
@@ -1545,6 +1546,10 @@ void set_prepare(Context& ctx, const std::vector<VolumeOptions>& opts, const std
     b.oK0B = b.pl.add(k0B);
     b.oK0A1 = b.pl.add(k0A1);
     b.oK0A2 = b.pl.add(k0A2);
+    // the solves a1 = Vr^-1 u and w <- Q^-1 w are independent: one batch of 2 count
+    b.oK0VQ = b.pl.add(cat(k0V, k0Q));
+    b.oK0PVQ = b.pl.add(cat(k0PV, k0PQ));
+    b.oK0A12 = b.pl.add(cat(k0A1, k0A2));
     upload(b.d_first, first, s);
   }
   b.pl.commit(s);
@@ -1786,8 +1791,12 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
       // beta_0 = Gd_0^-1 r from the factors of Vr_0 and Q_0
       hipLaunchKernelGGL(k_copy_set, dim3(grid_for(n4), count), dim3(256), 0, s, (long)n4, pl.mats(b.oK0B),
                          pl.mats(b.oK0A1));
-      zgetrs_set(n2, 1, pl.mats(b.oK0V), n2, pl.ints(b.oK0PV), pl.mats(b.oK0A1), n2, count, s);
-      zgetrs_set(n2, 1, pl.mats(b.oK0Q), n2, pl.ints(b.oK0PQ), pl.mats(b.oK0A2), n2, count, s);
+      if (2L * count <= 65535) {
+        zgetrs_set(n2, 1, pl.mats(b.oK0VQ), n2, pl.ints(b.oK0PVQ), pl.mats(b.oK0A12), n2, 2 * count, s);
+      } else {
+        zgetrs_set(n2, 1, pl.mats(b.oK0V), n2, pl.ints(b.oK0PV), pl.mats(b.oK0A1), n2, count, s);
+        zgetrs_set(n2, 1, pl.mats(b.oK0Q), n2, pl.ints(b.oK0PQ), pl.mats(b.oK0A2), n2, count, s);
+      }
       zgetrs_set(n2, 1, pl.mats(b.oK0V), n2, pl.ints(b.oK0PV), pl.mats(b.oK0A2), n2, count, s);
       hipLaunchKernelGGL(k_beta0_block, dim3(grid_for(n4), count), dim3(256), 0, s, ia, b.d_first.p,
                          pl.mats(b.oK0A1), pl.mats(b.oK0A2), pl.mats(b.oK0B), gam.p, b.d_src.p, b.pscale.p, is.dz.p);
