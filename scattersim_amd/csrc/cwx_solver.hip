@@ -806,7 +806,8 @@ __global__ void k_bnd_b(InstArgs ia, zc* __restrict__ b) {
     zc v = mk(0.0);
     if (r >= 2 * MF) {
       const int rr = r - 2 * MF;
-      for (int c = 0; c < n3; ++c) {
+      // f1 is zero outside the columns c = rr (mod MF): the three that remain, in increasing c
+      for (int c = rr % MF; c < n3; c += MF) {
         const zc f = f_entry(a, 1, rr, c);
         if (f.re != 0.0 || f.im != 0.0) v = v + f * a.EF[c];
       }
@@ -877,7 +878,8 @@ __global__ void k_beta0_rhs(InstArgs ia, const zc* __restrict__ x, MatSet out) {
   zc* o = out.at(i);
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n4; r += gridDim.x * blockDim.x) {
     zc v = mk(0.0);
-    for (int c = 0; c < n3; ++c) {
+    // f1 and f2 are zero outside the columns c = r (mod MF): the three that remain, in increasing c
+    for (int c = r % a.MF; c < n3; c += a.MF) {
       const zc f1 = f_entry(a, 1, r, c), f2 = f_entry(a, 2, r, c);
       if (f1.re != 0.0 || f1.im != 0.0) v = v + f1 * a.EF[c];
       if (f2.re != 0.0 || f2.im != 0.0) v = v + f2 * x[c];
