@@ -523,11 +523,19 @@ __device__ inline zc mode_rot(const PairScale& sc, bool neg, int n2) {
   return conj(vk) * (sc.inv_norm / absz(vk));
 }
 
-__global__ void k_t_block(InstArgs ia, const int* __restrict__ tg, MatSet Xs, MatSet Ys, MatSet Ts,
-                          const zc* __restrict__ gam, const int* __restrict__ src, const PairScale* __restrict__ ps,
-                          const double* __restrict__ dz) {
+// Workgroup (tile, t): rows [64 rt, 64 rt + 64) and columns [32 ct, 32 ct + 32) of T for transfer t, tile =
+// rt + ct * (number of row tiles).  The row factors and column factors of the tile are formed once, in LDS; each lane then
+// takes one row and eight columns.
+constexpr int TB_R = 64, TB_C = 32;
+
+__global__ __launch_bounds__(256) void k_t_block(InstArgs ia, const int* __restrict__ tg, MatSet Xs, MatSet Ys,
+                                                 MatSet Ts, const zc* __restrict__ gam, const int* __restrict__ src,
+                                                 const PairScale* __restrict__ ps, const double* __restrict__ dz) {
+  __shared__ zc s_rf[TB_R], s_cf[TB_C];
   const int t = blockIdx.y, g = tg[t];
   const int n4 = 4 * ia.MF, n2 = n4 / 2;
+  const int rtiles = (n4 + TB_R - 1) / TB_R;
+  const int r0 = (int)(blockIdx.x % rtiles) * TB_R, c0 = (int)(blockIdx.x / rtiles) * TB_C;
   const double k = ia.k[ia.linst[g]];
   const zc* __restrict__ X = Xs.at(t);
   const zc* __restrict__ Y = Ys.at(t);
@@ -540,30 +548,52 @@ __global__ void k_t_block(InstArgs ia, const int* __restrict__ tg, MatSet Xs, Ma
   const PairScale* psp = ps + (long)(g - 1) * n2;
   const double dzg = dz[g], dzp = dz[g - 1];
   constexpr int kMask = (1 << 30) - 1;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n4 * n4; e += (long)gridDim.x * blockDim.x) {
-    const int row = (int)(e % n4), col = (int)(e / n4);
-    const int i = row >> 1, si = row & 1, j = col >> 1, tj = col & 1;
-    const int p = sg[2 * i] & kMask, q = sp[2 * j] & kMask;
+  const int tid = threadIdx.x;
+  if (tid < TB_R) {
+    const int row = r0 + tid;
+    if (row < n4) {
+      // row factor Dd_g^-1 R_g^-1: R = conj(vk) c with |R| = inv_norm, so R^-1 = vk / (|vk| inv_norm)
+      const int si = row & 1, p = sg[2 * (row >> 1)] & kMask;
+      const PairScale& a = psg[p];
+      zc vk = a.vk;
+      if (si && a.k >= n2) vk = -vk;
+      zc rf = vk * (1.0 / (absz(vk) * a.inv_norm));
+      if (si) {
+        const zc gr = gg[row];
+        rf = rf * zexp(mk(-gr.im * k * dzg, gr.re * k * dzg));  // 1 / exp(-i k gamma dz)
+      }
+      s_rf[tid] = rf;
+    }
+  } else if (tid < TB_R + TB_C) {
+    const int col = c0 + tid - TB_R;
+    if (col < n4) {
+      // column factor R_{g-1} Dc_{g-1}
+      const int tj = col & 1, q = sp[2 * (col >> 1)] & kMask;
+      zc cf = mode_rot(psp[q], tj != 0, n2);
+      if (!tj) {
+        const zc gc = gp[col];
+        cf = cf * zexp(mk(-gc.im * k * dzp, gc.re * k * dzp));  // exp(i k gamma dz)
+      }
+      s_cf[tid - TB_R] = cf;
+    }
+  }
+  __syncthreads();
+  const int row = r0 + (tid & (TB_R - 1));
+  if (row >= n4) return;
+  const int i = row >> 1, si = row & 1;
+  const int p = sg[2 * i] & kMask;
+  const zc rf = s_rf[tid & (TB_R - 1)];
+  const zc gi = gg[2 * i];
+  for (int cc = tid / TB_R; cc < TB_C; cc += 256 / TB_R) {
+    const int col = c0 + cc;
+    if (col >= n4) break;
+    const int j = col >> 1, tj = col & 1;
+    const int q = sp[2 * j] & kMask;
     const zc x = X[(long)q * n2 + p];
-    const zc y = Y[(long)q * n2 + p] * zdiv(gg[2 * i], gp[2 * j]);
+    const zc y = Y[(long)q * n2 + p] * zdiv(gi, gp[2 * j]);
     zc v = (si == tj) ? x + y : x - y;
     v = 0.5 * v;
-    // row factor Dd_g^-1 R_g^-1: R = conj(vk) c with |R| = inv_norm, so R^-1 = vk / (|vk| inv_norm)
-    const PairScale& a = psg[p];
-    zc vk = a.vk;
-    if (si && a.k >= n2) vk = -vk;
-    zc rf = vk * (1.0 / (absz(vk) * a.inv_norm));
-    if (si) {
-      const zc gr = gg[row];
-      rf = rf * zexp(mk(-gr.im * k * dzg, gr.re * k * dzg));  // 1 / exp(-i k gamma dz)
-    }
-    // column factor R_{g-1} Dc_{g-1}
-    zc cf = mode_rot(psp[q], tj != 0, n2);
-    if (!tj) {
-      const zc gc = gp[col];
-      cf = cf * zexp(mk(-gc.im * k * dzp, gc.re * k * dzp));  // exp(i k gamma dz)
-    }
-    T[e] = rf * v * cf;
+    T[(long)col * n4 + row] = rf * v * s_cf[cc];
   }
 }
 
@@ -1715,7 +1745,7 @@ void set_launch(Context& ctx, SetBufs& b, const zc* d_samples, int s1, int s2, S
         }
         zgetrs_set(n2, n2, pl.mats(b.oKV), n2, pl.ints(b.oKPV), pl.mats(b.oKZ), n2, b.nT, s);
       }
-      hipLaunchKernelGGL(k_t_block, dim3(grid_for(nn), (unsigned)b.nT), dim3(256), 0, s, ia, b.d_tg.p,
+      hipLaunchKernelGGL(k_t_block, dim3((unsigned)(cdiv(n4, TB_R) * cdiv(n4, TB_C)), (unsigned)b.nT), dim3(256), 0, s, ia, b.d_tg.p,
                          pl.mats(b.oKX), pl.mats(b.oKZ), pl.mats(b.oKT), gam.p, b.d_src.p, b.pscale.p, is.dz.p);
     }
   } else {
